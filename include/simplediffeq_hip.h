@@ -1,0 +1,247 @@
+/*
+ * libsimplediffeq_hip — C ABI of the MI355X (gfx950) ensemble integrator for
+ * SimpleDiffEq.jl's GPU-compatible out-of-place solvers.
+ *
+ * One call of sdeh_solve() replaces an ensemble of per-trajectory calls of
+ *
+ *   DiffEqBase.solve(prob::ODEProblem, alg::GPUSimpleTsit5;  saveat, save_everystep, dt)
+ *       src/tsit5/gpuatsit5.jl:55-147
+ *   DiffEqBase.solve(prob::ODEProblem, alg::GPUSimpleATsit5; dt, saveat, save_everystep,
+ *                    abstol, reltol)                         src/tsit5/gpuatsit5.jl:205-336
+ *   DiffEqBase.solve(prob::ODEProblem, alg::GPUSimpleVern7;  ...) src/verner/gpuvern7.jl:55-242
+ *   DiffEqBase.solve(prob::ODEProblem, alg::GPUSimpleAVern7; ...) src/verner/gpuvern7.jl:300-536
+ *   DiffEqBase.solve(prob::ODEProblem, alg::GPUSimpleVern9;  ...) src/verner/gpuvern9.jl:55-353
+ *   DiffEqBase.solve(prob::ODEProblem, alg::GPUSimpleAVern9; ...) src/verner/gpuvern9.jl:411-779
+ *
+ * that an EnsembleProblem/EnsembleThreads driver would make, one trajectory per
+ * GPU lane.  Keyword defaults mirror those signatures (sdeh_desc_init()).
+ *
+ * Data layout: every per-trajectory array is structure-of-arrays, component
+ * major: element (component c, trajectory i) of an array with n_traj
+ * trajectories is at index c * n_traj + i.  saveat output is
+ * [n_saveat][n_state][n_traj].  Time values are always binary64.
+ *
+ * Errors: every function returns an sdeh_status (0 = success).  A failing call
+ * leaves a message retrievable with sdeh_last_error().  Per-trajectory outcomes
+ * go to the retcode array (sdeh_retcode); the reference throws
+ * error("dt<dtmin") where this library records SDEH_RET_DTLESSTHANMIN for that
+ * trajectory, and sdeh_solve() then returns SDEH_E_DTMIN after writing all
+ * outputs.
+ *
+ * Threading: a context may be used by one host thread at a time.  Work is
+ * ordered on the context's stream (or the caller's stream, see sdeh_solve()).
+ */
+#ifndef SIMPLEDIFFEQ_HIP_H
+#define SIMPLEDIFFEQ_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDEH_ABI_VERSION 1
+
+typedef enum sdeh_status {
+  SDEH_OK = 0,
+  SDEH_E_INVALID = -1,      /* bad argument (message says which) */
+  SDEH_E_HIP = -2,          /* HIP runtime error */
+  SDEH_E_COMPILE = -3,      /* hipRTC compilation failed (log in sdeh_last_error) */
+  SDEH_E_UNSUPPORTED = -4,  /* valid request this build does not implement */
+  SDEH_E_NOMEM = -5,        /* device allocation failed */
+  SDEH_E_DTMIN = -6,        /* solve finished; >= 1 trajectory stopped at dt < dtmin */
+  SDEH_E_SAVE_OVERFLOW = -7 /* solve finished; >= 1 trajectory had more steps than max_saves */
+} sdeh_status;
+
+/* Algorithms: the six reference algorithm structs. */
+typedef enum sdeh_alg {
+  SDEH_ALG_TSIT5 = 0,   /* GPUSimpleTsit5   gpuatsit5.jl:52  */
+  SDEH_ALG_ATSIT5 = 1,  /* GPUSimpleATsit5  gpuatsit5.jl:200 */
+  SDEH_ALG_VERN7 = 2,   /* GPUSimpleVern7   gpuvern7.jl:52   */
+  SDEH_ALG_AVERN7 = 3,  /* GPUSimpleAVern7  gpuvern7.jl:295  */
+  SDEH_ALG_VERN9 = 4,   /* GPUSimpleVern9   gpuvern9.jl:52   */
+  SDEH_ALG_AVERN9 = 5   /* GPUSimpleAVern9  gpuvern9.jl:406  */
+} sdeh_alg;
+
+/* Element type of the state u (the reference's eltype(u0)). */
+typedef enum sdeh_dtype { SDEH_F64 = 0, SDEH_F32 = 1 } sdeh_dtype;
+
+/* Per-trajectory outcome (the reference's sol.retcode plus the failures it throws). */
+typedef enum sdeh_retcode {
+  SDEH_RET_DEFAULT = 0,        /* ReturnCode.Default: integrated to tf */
+  SDEH_RET_DTLESSTHANMIN = 1,  /* the reference's error("dt<dtmin") */
+  SDEH_RET_NAN = 2,            /* error estimate or time became NaN (reference: unspecified) */
+  SDEH_RET_MAXITERS = 3,       /* attempt cap reached (the reference has no cap) */
+  SDEH_RET_SAVE_OVERFLOW = 4   /* save_everystep output exceeded max_saves; integration completed */
+} sdeh_retcode;
+
+/* Built-in right-hand sides (compiled ahead of time into the library).
+ * Arbitrary RHS functions come from sdeh_compile_rhs(). */
+typedef enum sdeh_builtin_rhs {
+  SDEH_RHS_LORENZ = 1,    /* n_state 3, n_param 3 (sigma, rho, beta) */
+  SDEH_RHS_GLV16 = 2,     /* n_state 16, n_param 16 (r_i); p_shared: A, 16x16 row-major */
+  SDEH_RHS_EXPDECAY3 = 3, /* n_state 3, n_param 1: du = p0 * u */
+  SDEH_RHS_AFFINE1 = 4,   /* n_state 1, n_param 2: du = p0 * u + p1 * t (non-autonomous) */
+  SDEH_RHS_LINEAR2 = 5    /* n_state 2, n_param 4: du = [p0 p1; p2 p3] * u */
+} sdeh_builtin_rhs;
+
+/* Scheduling of trajectories onto lanes. */
+typedef enum sdeh_schedule {
+  SDEH_SCHED_AUTO = 0,       /* library choice (autotuned) */
+  SDEH_SCHED_ONE_LAUNCH = 1, /* one lane per trajectory, grid covers the ensemble */
+  SDEH_SCHED_PERSISTENT = 2  /* resident grid; lanes refill from a device atomic queue */
+} sdeh_schedule;
+
+/* Precision mode (north-star precision work). */
+typedef enum sdeh_precision {
+  SDEH_PREC_PLAIN = 0,  /* stages and update in the state type */
+  SDEH_PREC_KAHAN = 1   /* FP32 state with a compensated u update; t stays FP64 */
+} sdeh_precision;
+
+typedef enum sdeh_memory { SDEH_MEM_HOST = 0, SDEH_MEM_DEVICE = 1 } sdeh_memory;
+
+/* Solve descriptor: the keyword arguments of the reference solve methods plus
+ * ensemble and scheduling fields.  Initialise with sdeh_desc_init(). */
+typedef struct sdeh_solve_desc {
+  int32_t alg;              /* sdeh_alg */
+  int32_t dtype;            /* sdeh_dtype */
+  int64_t rhs;              /* sdeh_builtin_rhs or a handle from sdeh_compile_rhs */
+  int64_t n_traj;           /* ensemble size (trajectories) */
+  double t0, tf;            /* common tspan, used when io->tspan is NULL */
+  double dt;                /* fixed step, or initial step of the adaptive solvers;
+                               reference default 0.1f0 = 0.10000000149011612 */
+  double abstol, reltol;    /* reference defaults 1f-6 and 1f-3 (as binary64) */
+  int32_t save_everystep;   /* reference default true; with saveat == NULL,
+                               every accepted step is written (capacity max_saves) */
+  int32_t max_saves;        /* per-trajectory capacity of the save_everystep output */
+  const double* saveat;     /* host array of n_saveat save times, ascending, or NULL */
+  int64_t n_saveat;
+  int64_t maxiters;         /* attempt cap per trajectory (0 = library default 10^7) */
+  int32_t quirks;           /* 1: reproduce the reference's saveat stage quirks
+                               (fixed Vern9 stage aliasing, post-step stage times);
+                               0: corrected dense output */
+  int32_t schedule;         /* sdeh_schedule */
+  int32_t precision;        /* sdeh_precision */
+  int32_t block;            /* 0 = tuned; else workgroup size (multiple of 64, <= 1024) */
+  int32_t grid;             /* 0 = tuned; persistent schedule: number of workgroups */
+  int32_t chunk;            /* 0 = tuned; persistent schedule: trajectories per queue grab */
+  int32_t reserved[4];
+} sdeh_solve_desc;
+
+/* Solve buffers.  With memory == SDEH_MEM_HOST these are host pointers and the
+ * library stages them through device memory; with SDEH_MEM_DEVICE they are
+ * device pointers used in place.  Optional inputs may be NULL. */
+typedef struct sdeh_io {
+  const void* u0;          /* [n_state][n_traj] of dtype */
+  const void* p;           /* [n_param][n_traj] of dtype (may be NULL if n_param == 0) */
+  const void* p_shared;    /* RHS-specific block common to all trajectories, of dtype */
+  int64_t n_shared;        /* element count of p_shared */
+  const double* tspan;     /* optional [2][n_traj]: row 0 = t0, row 1 = tf */
+  const double* dt0;       /* optional [n_traj] initial dt per trajectory (adaptive) */
+  const int64_t* order;    /* optional [n_traj] permutation: the order in which
+                              trajectories are handed to lanes; results stay indexed
+                              by trajectory */
+  void* u_end;             /* [n_state][n_traj] state at the final time */
+  double* t_end;           /* [n_traj] final time */
+  int32_t* retcode;        /* [n_traj] sdeh_retcode */
+  int32_t* naccept;        /* [n_traj] accepted steps (optional) */
+  int32_t* nreject;        /* [n_traj] rejected attempts (optional) */
+  void* u_saveat;          /* [n_saveat][n_state][n_traj]; unreached points are NaN */
+  void* u_steps;           /* [max_saves][n_state][n_traj] save_everystep states */
+  double* t_steps;         /* [max_saves][n_traj] save_everystep times */
+  int32_t* n_steps;        /* [n_traj] number of saved steps (u0 included) */
+} sdeh_io;
+
+/* Timing and work counters of the last sdeh_solve() on a context. */
+typedef struct sdeh_stats {
+  double kernel_ms;        /* device time of the integration kernel(s) (HIP events) */
+  double total_ms;         /* host wall time of the call, staging copies included */
+  int64_t attempts;        /* sum over trajectories of naccept + nreject */
+  int64_t accepts;
+  int32_t block, grid, chunk, schedule; /* launch shape actually used */
+  int32_t vgprs;           /* reserved (0) */
+} sdeh_stats;
+
+typedef struct sdeh_ctx sdeh_ctx;
+
+/* Fill d with the reference solve defaults (alg ATSIT5, F64, dt 0.1f0,
+ * abstol 1f-6, reltol 1f-3, save_everystep true, quirks on, auto schedule). */
+void sdeh_desc_init(sdeh_solve_desc* d);
+
+int sdeh_abi_version(void);
+
+/* Create a context on HIP device `device`.  The caller owns *out. */
+int sdeh_create(sdeh_ctx** out, int device);
+void sdeh_destroy(sdeh_ctx* ctx);
+const char* sdeh_last_error(const sdeh_ctx* ctx);
+
+/* Shape of an RHS (built-in id or compiled handle). */
+int sdeh_rhs_info(sdeh_ctx* ctx, int64_t rhs, int32_t* n_state, int32_t* n_param,
+                  int64_t* n_shared);
+
+/* Register an out-of-place RHS given as HIP C source (the output of the RHS
+ * tracer).  `src` must define
+ *   template <class T> __device__ inline void <fn_name>(T* du, const T* u,
+ *       const T* p, const T* ps, double t);
+ * Integrator kernels for it are compiled with hipRTC for gfx950 on first use
+ * and cached (in memory, and on disk under $SDEH_CACHE_DIR when set).
+ * Replaces the Julia-side specialisation of solve() on the user's f. */
+int sdeh_compile_rhs(sdeh_ctx* ctx, const char* src, const char* fn_name, int32_t n_state,
+                     int32_t n_param, int64_t n_shared, int64_t* handle);
+
+/* Integrate an ensemble.  `stream` is a hipStream_t (NULL = the context's stream).
+ * With SDEH_MEM_HOST the call returns after results are back in host memory;
+ * with SDEH_MEM_DEVICE it returns once the work is queued, unless `sync` != 0. */
+int sdeh_solve(sdeh_ctx* ctx, const sdeh_solve_desc* d, const sdeh_io* io, int32_t memory,
+               void* stream, int32_t sync);
+
+int sdeh_get_stats(const sdeh_ctx* ctx, sdeh_stats* out);
+
+/* Repeated ensembles: capture one device-memory solve into a HIP graph and
+ * replay it.  The graph re-arms its own work queue on each replay. */
+int sdeh_graph_capture(sdeh_ctx* ctx, const sdeh_solve_desc* d, const sdeh_io* io,
+                       int64_t* graph_handle);
+int sdeh_graph_launch(sdeh_ctx* ctx, int64_t graph_handle, void* stream);
+int sdeh_graph_destroy(sdeh_ctx* ctx, int64_t graph_handle);
+
+/* Launch-shape autotuner: time candidate (block, grid, chunk, schedule) shapes
+ * for the given problem on device buffers and store the best one for
+ * (alg, rhs, dtype, save mode).  Later solves with tuned fields left 0 use it. */
+int sdeh_autotune(sdeh_ctx* ctx, const sdeh_solve_desc* d, const sdeh_io* io,
+                  sdeh_stats* best);
+
+/* Synchronise the context stream. */
+int sdeh_synchronize(sdeh_ctx* ctx);
+
+/* The fixed-step time grid t0:dt:tf with Julia's Float64 range rules (the
+ * `_ts = tspan[1]:dt:tspan[2]` of gpuatsit5.jl:92, gpuvern7.jl:100,
+ * gpuvern9.jl:100): returns length(t0:dt:tf) (the solvers take length - 1
+ * steps) and writes the first min(length, cap) elements to elems (may be
+ * NULL).  Returns -1 when dt == 0. */
+int64_t sdeh_fixed_grid(double t0, double dt, double tf, double* elems, int64_t cap);
+
+/* ---- Learned cost model: 6 -> 64 -> 64 -> 2 MLP (FP32), trained on device. ----
+ * Layout of the parameter vector (FP32, n = 4738): W1[64][6], b1[64],
+ * W2[64][64], b2[64], W3[2][64], b3[2].  Activations are tanh. */
+#define SDEH_MLP_NIN 6
+#define SDEH_MLP_NH 64
+#define SDEH_MLP_NOUT 2
+#define SDEH_MLP_NPARAM 4738
+
+/* Forward pass on device buffers: x [n][6] -> y [n][2] (row-major). */
+int sdeh_mlp_forward(sdeh_ctx* ctx, const float* params, const float* x, float* y, int64_t n,
+                     void* stream);
+/* Loss (mean squared error over n x 2 outputs) and its gradient with respect
+ * to the parameters, on device buffers.  grad has SDEH_MLP_NPARAM floats;
+ * loss is one float. */
+int sdeh_mlp_loss_grad(sdeh_ctx* ctx, const float* params, const float* x, const float* y,
+                       int64_t n, float* loss, float* grad, void* stream);
+/* One Adam update (in place) on device buffers; t is the 1-based step number. */
+int sdeh_mlp_adam(sdeh_ctx* ctx, float* params, const float* grad, float* m, float* v,
+                  float lr, float beta1, float beta2, float eps, int32_t t, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMPLEDIFFEQ_HIP_H */

@@ -1,0 +1,1163 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of SimpleDiffEq.jl's GPU-compatible out-of-place solve
+// bodies, used as the checker for the HIP integrators and as the timed CPU
+// baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load it; the product library never links or calls it.
+//
+// It follows, statement by statement:
+//   src/tsit5/gpuatsit5.jl:55-147   GPUSimpleTsit5   -> tsit5_fixed()
+//   src/tsit5/gpuatsit5.jl:205-336  GPUSimpleATsit5  -> atsit5()
+//   src/verner/gpuvern7.jl:55-242   GPUSimpleVern7   -> vern7_fixed()
+//   src/verner/gpuvern7.jl:300-536  GPUSimpleAVern7  -> avern7()
+//   src/verner/gpuvern9.jl:55-353   GPUSimpleVern9   -> vern9_fixed()
+//   src/verner/gpuvern9.jl:411-779  GPUSimpleAVern9  -> avern9()
+//   src/SimpleDiffEq.jl:67-77       controller constants -> Ctrl
+//   src/tsit5/tsit5.jl:385-399, src/verner/verner_tableaus.jl:1337-1398  bθs
+// and restates the external semantics they rely on:
+//   DiffEqBase.ODE_DEFAULT_NORM on a StaticArray: sqrt(sum(abs2, u) / length(u)),
+//     sum as a left fold (StaticArrays mapreduce), no fusion -> norm()
+//   Julia Base `start:step:stop` for Float64 (base/twiceprecision.jl: rat,
+//     the rational fast path and the literal fallback) -> JRange
+//   Base.evalpoly Horner form with muladd -> fma (muladd on binary64 scalars
+//     compiles to a fused multiply-add on FMA hardware)
+//   Julia min/max NaN propagation outside @fastmath (sdeh_math.h jl_min/jl_max)
+//
+// Arithmetic conventions (both are readings of the reference; which one the
+// reference's SVector code actually rounds like is not pinned by any of its
+// tests):
+//   ARITH_FUSED: every `x + y*z` inside the @muladd solve bodies is one fma;
+//     a sum of products a1*k1 + a2*k2 + ... is the chain
+//     acc = a1*k1; acc = fma(a2, k2, acc); ...  and `uprev + dt*(S)` is
+//     fma(dt, S, uprev).  This is the convention the HIP kernels implement,
+//     so GPU vs oracle is compared bit for bit in this mode.
+//   ARITH_PLAIN: every SVector operation rounds separately, sums left to right
+//     (Base's generic muladd(x, y, z) = x*y + z on non-scalar arguments).
+// Scalar muladds (stage times t + c*dt, the bθs polynomials) are fused in both.
+//
+// pow: POW_PORTABLE uses sdeh_math::pow_ctrl (bit-identical on CPU and GPU);
+// POW_LIBM uses the C library pow, as the reference's `@fastmath ^` does.
+//
+// The coefficient tables come from the product header sdeh_tableaus.h, which is
+// checked against the reference text and against order conditions by tests.
+//
+// Deviations, all outside what the reference defines:
+//   * a NaN error estimate ends the trajectory with retcode NaN (the reference
+//     accepts it and then runs @fastmath min/max on NaN, which is unspecified);
+//   * an attempt cap (maxiters) ends runaway trajectories (the reference has none);
+//   * dt<dtmin is a per-trajectory retcode instead of an exception;
+//   * the dead stage k10 of fixed-step Vern7 (gpuvern7.jl:130-133) is skipped,
+//     it feeds nothing;
+//   * FP32 states keep t, tspan and saveat in binary64 (the reference would use
+//     the tspan type), and the error-norm/controller scalars in the state type.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <initializer_list>
+#include <limits>
+#include <vector>
+
+#include "../include/simplediffeq_hip.h"
+#include "../simplediffeq.jl_amd/csrc/sdeh_math.h"
+#include "../simplediffeq.jl_amd/csrc/sdeh_tableaus.h"
+
+namespace oracle {
+
+enum { ARITH_FUSED = 0, ARITH_PLAIN = 1 };
+enum { POW_PORTABLE = 0, POW_LIBM = 1 };
+
+struct Mode {
+  int arith = ARITH_FUSED;
+  int pow = POW_PORTABLE;
+  int quirks = 1;
+  int64_t maxiters = 10000000;
+};
+
+using sdeh_math::jl_max;
+using sdeh_math::jl_min;
+
+// ---------------------------------------------------------------------------
+// Right-hand sides.  Each is the out-of-place f(u, p, t) of a test or
+// benchmark problem, evaluated exactly as written (no fusion).
+// ---------------------------------------------------------------------------
+
+// Lorenz, as in test/gpusimpleatsit5_tests.jl:3-13.
+template <class T>
+struct Lorenz {
+  static constexpr int N = 3, P = 3;
+  const T* ps = nullptr;
+  void operator()(T* du, const T* u, const T* p, double) const {
+    const T s = p[0], r = p[1], b = p[2];
+    du[0] = s * (u[1] - u[0]);
+    du[1] = u[0] * (r - u[2]) - u[1];
+    du[2] = u[0] * u[1] - b * u[2];
+  }
+};
+
+// Generalised Lotka-Volterra, 16 species: du_i = u_i (r_i + sum_j A_ij u_j),
+// the sum accumulated from r_i as fma(A_ij, u_j, acc), j = 0..15.  A is shared
+// (row-major in ps), r is per trajectory.
+template <class T>
+struct GLV16 {
+  static constexpr int N = 16, P = 16;
+  const T* ps = nullptr;
+  void operator()(T* du, const T* u, const T* p, double) const {
+    for (int i = 0; i < 16; ++i) {
+      T acc = p[i];
+      for (int j = 0; j < 16; ++j) acc = std::fma(ps[i * 16 + j], u[j], acc);
+      du[i] = u[i] * acc;
+    }
+  }
+};
+
+// u' = p0 u on three components (the problem of test/gpu_ode_regression.jl:2-9
+// with p0 = -1).
+template <class T>
+struct ExpDecay3 {
+  static constexpr int N = 3, P = 1;
+  const T* ps = nullptr;
+  void operator()(T* du, const T* u, const T* p, double) const {
+    for (int c = 0; c < 3; ++c) du[c] = p[0] * u[c];
+  }
+};
+
+// Non-autonomous scalar test problem u' = p0 u + p1 t.
+template <class T>
+struct Affine1 {
+  static constexpr int N = 1, P = 2;
+  const T* ps = nullptr;
+  void operator()(T* du, const T* u, const T* p, double t) const {
+    du[0] = p[0] * u[0] + p[1] * (T)t;
+  }
+};
+
+// u' = M u, M = [p0 p1; p2 p3] (test/simpleatsit5_tests.jl:87-112 uses [2 3; 4 5]).
+template <class T>
+struct Linear2 {
+  static constexpr int N = 2, P = 4;
+  const T* ps = nullptr;
+  void operator()(T* du, const T* u, const T* p, double) const {
+    du[0] = p[0] * u[0] + p[1] * u[1];
+    du[1] = p[2] * u[0] + p[3] * u[1];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Julia Float64 ranges (base/twiceprecision.jl, `(:)(start::T, step::T, stop::T)`
+// for T<:IEEEFloat): the fixed-step solvers iterate over tspan[1]:dt:tspan[2]
+// (gpuatsit5.jl:92, gpuvern7.jl:100, gpuvern9.jl:100).
+// ---------------------------------------------------------------------------
+
+// Base.rat: continued-fraction rational approximation, bounded by
+// maxintfloat(Float32, Int) = 2^24.
+inline void jl_rat(double x, int64_t* num, int64_t* den) {
+  double y = x;
+  int64_t a = 1, d = 1, b = 0, c = 0;
+  const double m = 16777216.0;
+  while (std::fabs(y) <= m) {
+    const int64_t f = (int64_t)std::trunc(y);
+    y -= (double)f;
+    const int64_t na = f * a + c;
+    c = a;
+    a = na;
+    const int64_t nb = f * b + d;
+    d = b;
+    b = nb;
+    if ((std::llabs(a) > std::llabs(b) ? std::llabs(a) : std::llabs(b)) > (int64_t)m) {
+      *num = c;
+      *den = d;
+      return;
+    }
+    if ((double)a / (double)b == x) break;
+    y = 1.0 / y;
+  }
+  *num = a;
+  *den = b;
+}
+
+inline int64_t jl_gcd(int64_t a, int64_t b) {
+  a = std::llabs(a);
+  b = std::llabs(b);
+  while (b) {
+    int64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+inline bool jl_isbetween(double a, double x, double b) {
+  return (a <= x && x <= b) || (b <= x && x <= a);
+}
+
+// Round to nearest, ties to even (Julia's round(Int, x)).
+inline int64_t jl_round_int(double x) { return (int64_t)std::nearbyint(x); }
+
+struct JRange {
+  int rational = 0;  // 1: elements are (start_n + i*step_n) / den
+  int64_t start_n = 0, step_n = 0, den = 1;
+  double start = 0.0, step = 0.0;
+  int64_t len = 0;
+
+  // Element i (0-based) of the range, to the correctly rounded value the
+  // TwicePrecision arithmetic of StepRangeLen targets.
+  double elem(int64_t i) const {
+    if (rational) return (double)(start_n + i * step_n) / (double)den;
+    const double di = (double)i;
+    const double hi = di * step;
+    const double lo = std::fma(di, step, -hi);
+    const double s = start + hi;
+    const double bb = s - start;
+    const double e = (start - (s - bb)) + (hi - bb);
+    return s + (e + lo);
+  }
+};
+
+inline JRange jl_range(double start, double step, double stop) {
+  JRange r;
+  r.start = start;
+  r.step = step;
+  int64_t step_n, step_d;
+  jl_rat(step, &step_n, &step_d);
+  if (step_d != 0 && (double)step_n / (double)step_d == step) {
+    int64_t start_n, start_d, stop_n, stop_d;
+    jl_rat(start, &start_n, &start_d);
+    jl_rat(stop, &stop_n, &stop_d);
+    if (start_d != 0 && stop_d != 0 && (double)start_n / (double)start_d == start &&
+        (double)stop_n / (double)stop_d == stop) {
+      const int64_t den = start_d / jl_gcd(start_d, step_d) * step_d;  // lcm
+      const double m = 9007199254740992.0;                             // maxintfloat(Float64)
+      if (den != 0 && std::fabs(start * (double)den) <= m && std::fabs(step * (double)den) <= m &&
+          den % start_d == 0 && den % step_d == 0) {
+        const int64_t sn = jl_round_int(start * (double)den);
+        const int64_t pn = jl_round_int(step * (double)den);
+        int64_t len = (den * stop_n - stop_d * sn + pn * stop_d) / (pn * stop_d);
+        if (len < 0) len = 0;
+        if (jl_isbetween(start, start + (double)(len - 1) * step, stop + step / 2) &&
+            !jl_isbetween(start, start + (double)len * step, stop)) {
+          r.rational = 1;
+          r.start_n = sn;
+          r.step_n = pn;
+          r.den = den;
+          r.len = len;
+          return r;
+        }
+      }
+    }
+  }
+  const double lf = (stop - start) / step;
+  int64_t len;
+  if (lf < 0) {
+    len = 0;
+  } else if (lf == 0) {
+    len = 1;
+  } else {
+    len = jl_round_int(lf) + 1;
+    const double stop2 = start + (double)(len - 1) * step;
+    len -= (int64_t)((start < stop && stop < stop2) + (start > stop && stop > stop2));
+  }
+  r.len = len;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Arithmetic helpers.
+// ---------------------------------------------------------------------------
+
+template <class T>
+struct Term {
+  T a;
+  const T* k;
+};
+
+// out = base + h * (sum_j a_j k_j)  (the `uprev + dt * (a1*k1 + ...)` pattern)
+template <class T, int N>
+inline void comb(int arith, T* out, const T* base, T h, std::initializer_list<Term<T>> terms) {
+  for (int c = 0; c < N; ++c) {
+    auto it = terms.begin();
+    T acc = it->a * it->k[c];
+    for (++it; it != terms.end(); ++it)
+      acc = (arith == ARITH_FUSED) ? std::fma(it->a, it->k[c], acc) : acc + it->a * it->k[c];
+    out[c] = (arith == ARITH_FUSED) ? std::fma(h, acc, base[c]) : base[c] + h * acc;
+  }
+}
+
+// out = base + s * k with s = dt * a (the one-term first stage,
+// gpuatsit5.jl:99 `uprev + dt * a21 * k1`, gpuvern7.jl:104-105 `a = dt * a021`)
+template <class T, int N>
+inline void comb1(int arith, T* out, const T* base, T s, const T* k) {
+  for (int c = 0; c < N; ++c)
+    out[c] = (arith == ARITH_FUSED) ? std::fma(s, k[c], base[c]) : base[c] + s * k[c];
+}
+
+// out = h * (sum_j a_j k_j)  (the error estimate `dt * (btilde1*k1 + ...)`)
+template <class T, int N>
+inline void scaled_sum(int arith, T* out, T h, std::initializer_list<Term<T>> terms) {
+  for (int c = 0; c < N; ++c) {
+    auto it = terms.begin();
+    T acc = it->a * it->k[c];
+    for (++it; it != terms.end(); ++it)
+      acc = (arith == ARITH_FUSED) ? std::fma(it->a, it->k[c], acc) : acc + it->a * it->k[c];
+    out[c] = h * acc;
+  }
+}
+
+// Stage time t + c*dt: a scalar muladd, fused in both conventions.
+template <class T>
+inline double stage_t(double t, T c, T dt) {
+  return std::fma((double)c, (double)dt, t);
+}
+
+// Horner evaluation of Base.evalpoly(θ, (0, [0,] r_lo, ..., r_hi)):
+// h = r_hi; h = fma(θ, h, r) for each lower coefficient; then one fma(θ, h, 0)
+// per leading zero coefficient.
+template <class T>
+inline T evalpoly_z(T th, const T* r, int nr, int nzero) {
+  T h = r[nr - 1];
+  for (int i = nr - 2; i >= 0; --i) h = std::fma(th, h, r[i]);
+  for (int z = 0; z < nzero; ++z) h = std::fma(th, h, T(0));
+  return h;
+}
+
+// ODE_DEFAULT_NORM on an SVector: sqrt(sum(abs2, e) / length(e)), left fold.
+template <class T, int N>
+inline T norm(const T* e) {
+  T s = e[0] * e[0];
+  for (int c = 1; c < N; ++c) s = s + e[c] * e[c];
+  return std::sqrt(s / T(N));
+}
+
+template <class T>
+inline T ctrl_pow(int kind, T x, T y) {
+  if (kind == POW_LIBM) return std::pow(x, y);
+  return (T)sdeh_math::pow_ctrl((double)x, (double)y);
+}
+
+// build_adaptive_controller_cache(T), src/SimpleDiffEq.jl:67-77.
+template <class T>
+struct Ctrl {
+  T beta1 = T(7.0 / 50), beta2 = T(2.0 / 25), qmax = T(10.0), qmin = T(1.0 / 5),
+    gamma = T(9.0 / 10), qoldinit = T(1.0e-4);
+};
+
+// Tolerance-scaled error vector and its norm (gpuatsit5.jl:272-277 and the
+// Verner equivalents):
+//   tmp ./ (abstol .+ max.(abs.(uprev), abs.(u)) * reltol)
+template <class T, int N>
+inline T scaled_error_norm(int arith, T* tmp, const T* uprev, const T* u, T abstol,
+                           T reltol) {
+  for (int c = 0; c < N; ++c) {
+    const T m = jl_max(std::fabs(uprev[c]), std::fabs(u[c]));
+    const T sc = (arith == ARITH_FUSED) ? std::fma(m, reltol, abstol) : abstol + m * reltol;
+    tmp[c] = tmp[c] / sc;
+  }
+  return norm<T, N>(tmp);
+}
+
+// ---------------------------------------------------------------------------
+// One trajectory: inputs and outputs.
+// ---------------------------------------------------------------------------
+
+template <class T>
+struct Traj {
+  const T* u0;           // N values
+  const T* p;            // P values
+  double t0, tf;
+  double dt;             // fixed step / initial dt
+  T abstol, reltol;
+  const double* saveat;  // n_saveat values or null
+  int64_t n_saveat;
+  int save_everystep;
+  // outputs
+  T* u_end;              // N
+  double t_end = 0.0;
+  int retcode = SDEH_RET_DEFAULT;
+  int64_t naccept = 0, nreject = 0;
+  T* us = nullptr;       // [n_saveat][N] (AoS for this trajectory), NaN-initialised
+  std::vector<double>* steps_t = nullptr;  // save_everystep output
+  std::vector<T>* steps_u = nullptr;
+};
+
+template <class T, int N>
+inline void cpy(T* d, const T* s) {
+  for (int c = 0; c < N; ++c) d[c] = s[c];
+}
+
+template <class T, int N>
+inline void push_step(Traj<T>& tr, double t, const T* u) {
+  if (tr.steps_t) {
+    tr.steps_t->push_back(t);
+    for (int c = 0; c < N; ++c) tr.steps_u->push_back(u[c]);
+  }
+}
+
+// saveat bookkeeping at the start of a solve (gpuatsit5.jl:76-83 / 230-237):
+// returns the 0-based index of the first save point still to fill.
+template <class T, int N>
+inline int64_t saveat_init(Traj<T>& tr) {
+  if (!tr.saveat || tr.n_saveat == 0) return 0;
+  if (tr.t0 == tr.saveat[0]) {
+    cpy<T, N>(tr.us, tr.u0);
+    return 1;
+  }
+  return 0;
+}
+
+// ===========================================================================
+// GPUSimpleTsit5 — gpuatsit5.jl:55-147
+// ===========================================================================
+template <class T, class F>
+void tsit5_fixed(const F& f, Traj<T>& tr, const Mode& md) {
+  constexpr int N = F::N;
+  namespace tb = sdeh_tab::tsit5;
+  const int ar = md.arith;
+  const T c1 = T(tb::c1), c2 = T(tb::c2), c3 = T(tb::c3), c4 = T(tb::c4);
+  const T a21 = T(tb::a21), a31 = T(tb::a31), a32 = T(tb::a32), a41 = T(tb::a41),
+          a42 = T(tb::a42), a43 = T(tb::a43), a51 = T(tb::a51), a52 = T(tb::a52),
+          a53 = T(tb::a53), a54 = T(tb::a54), a61 = T(tb::a61), a62 = T(tb::a62),
+          a63 = T(tb::a63), a64 = T(tb::a64), a65 = T(tb::a65), a71 = T(tb::a71),
+          a72 = T(tb::a72), a73 = T(tb::a73), a74 = T(tb::a74), a75 = T(tb::a75),
+          a76 = T(tb::a76);
+  const T rs[22] = {T(tb::r11), T(tb::r12), T(tb::r13), T(tb::r14), T(tb::r22), T(tb::r23),
+                    T(tb::r24), T(tb::r32), T(tb::r33), T(tb::r34), T(tb::r42), T(tb::r43),
+                    T(tb::r44), T(tb::r52), T(tb::r53), T(tb::r54), T(tb::r62), T(tb::r63),
+                    T(tb::r64), T(tb::r72), T(tb::r73), T(tb::r74)};
+  const T* p = tr.p;
+  const T dt = (T)tr.dt;
+  double t = tr.t0;
+  int64_t cur = saveat_init<T, N>(tr);                              // :69-83
+  if (tr.save_everystep && !tr.saveat) push_step<T, N>(tr, t, tr.u0);
+
+  T u[N], uprev[N], k1[N], k2[N], k3[N], k4[N], k5[N], k6[N], k7[N], tmp[N];
+  cpy<T, N>(u, tr.u0);                                              // :85
+  f(k7, u, p, t);                                                   // :86
+  const JRange rg = jl_range(tr.t0, tr.dt, tr.tf);                  // :92
+
+  for (int64_t i = 1; i < rg.len; ++i) {                            // :95
+    cpy<T, N>(uprev, u);                                            // :96
+    cpy<T, N>(k1, k7);                                              // :97
+    t = rg.elem(i - 1);                                             // :98
+    comb1<T, N>(ar, tmp, uprev, dt * a21, k1);                      // :99
+    f(k2, tmp, p, stage_t(t, c1, dt));                              // :100
+    comb<T, N>(ar, tmp, uprev, dt, {{a31, k1}, {a32, k2}});         // :101
+    f(k3, tmp, p, stage_t(t, c2, dt));
+    comb<T, N>(ar, tmp, uprev, dt, {{a41, k1}, {a42, k2}, {a43, k3}});
+    f(k4, tmp, p, stage_t(t, c3, dt));
+    comb<T, N>(ar, tmp, uprev, dt, {{a51, k1}, {a52, k2}, {a53, k3}, {a54, k4}});
+    f(k5, tmp, p, stage_t(t, c4, dt));
+    comb<T, N>(ar, tmp, uprev, dt, {{a61, k1}, {a62, k2}, {a63, k3}, {a64, k4}, {a65, k5}});
+    f(k6, tmp, p, t + (double)dt);                                  // :108
+    comb<T, N>(ar, u, uprev, dt,
+               {{a71, k1}, {a72, k2}, {a73, k3}, {a74, k4}, {a75, k5}, {a76, k6}});  // :109
+    f(k7, u, p, t + (double)dt);                                    // :110
+    t += (double)dt;                                                // :111
+    tr.naccept++;
+    if (!tr.saveat && tr.save_everystep) {
+      push_step<T, N>(tr, t, u);                                    // :112-114
+    } else if (tr.saveat) {
+      while (cur < tr.n_saveat && tr.saveat[cur] <= t) {            // :116
+        const double savet = tr.saveat[cur];
+        const T th = (T)((savet - (t - (double)dt)) / (double)dt);  // :118
+        const T b1 = evalpoly_z(th, rs + 0, 4, 1), b2 = evalpoly_z(th, rs + 4, 3, 2),
+                b3 = evalpoly_z(th, rs + 7, 3, 2), b4 = evalpoly_z(th, rs + 10, 3, 2),
+                b5 = evalpoly_z(th, rs + 13, 3, 2), b6 = evalpoly_z(th, rs + 16, 3, 2),
+                b7 = evalpoly_z(th, rs + 19, 3, 2);
+        comb<T, N>(ar, tr.us + cur * N, uprev, dt,
+                   {{b1, k1}, {b2, k2}, {b3, k3}, {b4, k4}, {b5, k5}, {b6, k6}, {b7, k7}});
+        ++cur;
+      }
+    }
+  }
+  cpy<T, N>(tr.u_end, u);
+  tr.t_end = t;
+  if (!tr.saveat && !tr.save_everystep) push_step<T, N>(tr, t, u);  // :131-134
+}
+
+// ===========================================================================
+// Adaptive controller step shared by the three adaptive solvers.
+// Returns 1 on accept, 0 on reject, -1 on NaN.
+// gpuatsit5.jl:279-299, gpuvern7.jl:406-426, gpuvern9.jl:575-595.
+// ===========================================================================
+template <class T>
+struct CtrlState {
+  T dt;       // current step (state type)
+  T qold;
+  double t, tf;
+  T dtold;
+  double told;
+};
+
+template <class T>
+inline int controller(CtrlState<T>& s, T EEst, const Ctrl<T>& k, int powkind, double snap) {
+  if (EEst != EEst) return -1;
+  const T q11 = ctrl_pow(powkind, EEst, k.beta1);                     // :279
+  T q;
+  if (EEst == T(0)) {                                                 // :280
+    q = T(1) / k.qmax;
+  } else {
+    q = q11 / ctrl_pow(powkind, s.qold, k.beta2);                     // :283
+  }
+  if (EEst > T(1)) {                                                  // :286
+    s.dt = s.dt / jl_min(T(1) / k.qmin, q11 / k.gamma);               // :287
+    return 0;
+  }
+  q = sdeh_math::fast_max(T(1) / k.qmax, sdeh_math::fast_min(T(1) / k.qmin, q / k.gamma));  // :289
+  s.qold = jl_max(EEst, k.qoldinit);                                  // :290
+  s.dtold = s.dt;                                                     // :291
+  s.dt = s.dt / q;                                                    // :292
+  s.dt = (T)jl_min((double)std::fabs(s.dt), std::fabs(s.tf - s.t - (double)s.dtold));  // :293
+  s.told = s.t;                                                       // :294
+  if ((s.tf - s.t - (double)s.dtold) < snap) {                        // :295
+    s.t = s.tf;
+  } else {
+    s.t += (double)s.dtold;                                           // :298
+  }
+  return 1;
+}
+
+// ===========================================================================
+// GPUSimpleATsit5 — gpuatsit5.jl:205-336
+// ===========================================================================
+template <class T, class F>
+void atsit5(const F& f, Traj<T>& tr, const Mode& md) {
+  constexpr int N = F::N;
+  namespace tb = sdeh_tab::tsit5;
+  const int ar = md.arith;
+  const Ctrl<T> K;                                                    // :218-219
+  const T c1 = T(tb::c1), c2 = T(tb::c2), c3 = T(tb::c3), c4 = T(tb::c4);
+  const T a21 = T(tb::a21), a31 = T(tb::a31), a32 = T(tb::a32), a41 = T(tb::a41),
+          a42 = T(tb::a42), a43 = T(tb::a43), a51 = T(tb::a51), a52 = T(tb::a52),
+          a53 = T(tb::a53), a54 = T(tb::a54), a61 = T(tb::a61), a62 = T(tb::a62),
+          a63 = T(tb::a63), a64 = T(tb::a64), a65 = T(tb::a65), a71 = T(tb::a71),
+          a72 = T(tb::a72), a73 = T(tb::a73), a74 = T(tb::a74), a75 = T(tb::a75),
+          a76 = T(tb::a76);
+  const T bt1 = T(tb::btilde1), bt2 = T(tb::btilde2), bt3 = T(tb::btilde3),
+          bt4 = T(tb::btilde4), bt5 = T(tb::btilde5), bt6 = T(tb::btilde6),
+          bt7 = T(tb::btilde7);
+  const T rs[22] = {T(tb::r11), T(tb::r12), T(tb::r13), T(tb::r14), T(tb::r22), T(tb::r23),
+                    T(tb::r24), T(tb::r32), T(tb::r33), T(tb::r34), T(tb::r42), T(tb::r43),
+                    T(tb::r44), T(tb::r52), T(tb::r53), T(tb::r54), T(tb::r62), T(tb::r63),
+                    T(tb::r64), T(tb::r72), T(tb::r73), T(tb::r74)};
+  const T* p = tr.p;
+  CtrlState<T> s;
+  s.t = tr.t0;                                                        // :221
+  s.tf = tr.tf;                                                       // :222
+  s.dt = (T)tr.dt;
+  s.qold = K.qoldinit;                                                // :240
+  int64_t cur = saveat_init<T, N>(tr);                                // :223-237
+  if (tr.save_everystep && !tr.saveat) push_step<T, N>(tr, s.t, tr.u0);
+
+  T u[N], uprev[N], k1[N], k2[N], k3[N], k4[N], k5[N], k6[N], k7[N], tmp[N];
+  cpy<T, N>(u, tr.u0);                                                // :239
+  f(k7, u, p, s.t);                                                   // :241
+  int64_t attempts = 0;
+  while (s.t < s.tf) {                                                // :250
+    cpy<T, N>(uprev, u);                                              // :251
+    cpy<T, N>(k1, k7);                                                // :252
+    T EEst = std::numeric_limits<T>::infinity();                      // :253
+    while (EEst > T(1)) {                                             // :255
+      if ((double)s.dt < 1.0e-14) {                                   // :256
+        tr.retcode = SDEH_RET_DTLESSTHANMIN;
+        goto done;
+      }
+      if (attempts >= md.maxiters) {
+        tr.retcode = SDEH_RET_MAXITERS;
+        goto done;
+      }
+      ++attempts;
+      const T dt = s.dt;
+      const double t = s.t;
+      comb1<T, N>(ar, tmp, uprev, dt * a21, k1);                      // :258
+      f(k2, tmp, p, stage_t(t, c1, dt));                              // :259
+      comb<T, N>(ar, tmp, uprev, dt, {{a31, k1}, {a32, k2}});         // :260
+      f(k3, tmp, p, stage_t(t, c2, dt));
+      comb<T, N>(ar, tmp, uprev, dt, {{a41, k1}, {a42, k2}, {a43, k3}});
+      f(k4, tmp, p, stage_t(t, c3, dt));
+      comb<T, N>(ar, tmp, uprev, dt, {{a51, k1}, {a52, k2}, {a53, k3}, {a54, k4}});
+      f(k5, tmp, p, stage_t(t, c4, dt));
+      comb<T, N>(ar, tmp, uprev, dt, {{a61, k1}, {a62, k2}, {a63, k3}, {a64, k4}, {a65, k5}});
+      f(k6, tmp, p, t + (double)dt);                                  // :267
+      comb<T, N>(ar, u, uprev, dt,
+                 {{a71, k1}, {a72, k2}, {a73, k3}, {a74, k4}, {a75, k5}, {a76, k6}});  // :268
+      f(k7, u, p, t + (double)dt);                                    // :270
+      scaled_sum<T, N>(ar, tmp, dt,
+                       {{bt1, k1}, {bt2, k2}, {bt3, k3}, {bt4, k4}, {bt5, k5}, {bt6, k6},
+                        {bt7, k7}});                                  // :272-275
+      EEst = scaled_error_norm<T, N>(ar, tmp, uprev, u, tr.abstol, tr.reltol);  // :276-277
+      const int acc = controller<T>(s, EEst, K, md.pow, 1.0e-14);     // :279-299
+      if (acc < 0) {
+        tr.retcode = SDEH_RET_NAN;
+        goto done;
+      }
+      if (acc == 0) {
+        tr.nreject++;
+        continue;
+      }
+      tr.naccept++;
+      if (!tr.saveat && tr.save_everystep) {
+        push_step<T, N>(tr, s.t, u);                                  // :301-303
+      } else if (tr.saveat) {
+        while (cur < tr.n_saveat && tr.saveat[cur] <= s.t) {          // :305
+          const double savet = tr.saveat[cur];
+          const T th = (T)((savet - s.told) / (double)s.dtold);       // :307
+          const T b1 = evalpoly_z(th, rs + 0, 4, 1), b2 = evalpoly_z(th, rs + 4, 3, 2),
+                  b3 = evalpoly_z(th, rs + 7, 3, 2), b4 = evalpoly_z(th, rs + 10, 3, 2),
+                  b5 = evalpoly_z(th, rs + 13, 3, 2), b6 = evalpoly_z(th, rs + 16, 3, 2),
+                  b7 = evalpoly_z(th, rs + 19, 3, 2);
+          comb<T, N>(ar, tr.us + cur * N, uprev, s.dtold,
+                     {{b1, k1}, {b2, k2}, {b3, k3}, {b4, k4}, {b5, k5}, {b6, k6}, {b7, k7}});
+          ++cur;
+        }
+      }
+    }
+  }
+done:
+  cpy<T, N>(tr.u_end, u);
+  tr.t_end = s.t;
+  if (!tr.saveat && !tr.save_everystep) push_step<T, N>(tr, s.t, u);  // :322-325
+}
+
+// ===========================================================================
+// Verner 7 stage block (gpuvern7.jl:104-133 / 360-402).  Computes k1..k10
+// from uprev at (t, dt); k10 only when `want_k10` (the adaptive error needs it).
+// ===========================================================================
+template <class T, int N>
+struct V7Stages {
+  T k1[N], k2[N], k3[N], k4[N], k5[N], k6[N], k7[N], k8[N], k9[N], k10[N];
+};
+
+template <class T, class F>
+inline void vern7_stages(const F& f, int ar, const T* uprev, const T* p, double t, T dt,
+                         V7Stages<T, F::N>& K, T* u, bool want_k10) {
+  constexpr int N = F::N;
+  namespace tb = sdeh_tab::vern7;
+  T tmp[N], g10[N];
+  f(K.k1, uprev, p, t);
+  comb1<T, N>(ar, tmp, uprev, dt * T(tb::a021), K.k1);
+  f(K.k2, tmp, p, stage_t(t, T(tb::c2), dt));
+  comb<T, N>(ar, tmp, uprev, dt, {{T(tb::a031), K.k1}, {T(tb::a032), K.k2}});
+  f(K.k3, tmp, p, stage_t(t, T(tb::c3), dt));
+  comb<T, N>(ar, tmp, uprev, dt, {{T(tb::a041), K.k1}, {T(tb::a043), K.k3}});
+  f(K.k4, tmp, p, stage_t(t, T(tb::c4), dt));
+  comb<T, N>(ar, tmp, uprev, dt, {{T(tb::a051), K.k1}, {T(tb::a053), K.k3}, {T(tb::a054), K.k4}});
+  f(K.k5, tmp, p, stage_t(t, T(tb::c5), dt));
+  comb<T, N>(ar, tmp, uprev, dt,
+             {{T(tb::a061), K.k1}, {T(tb::a063), K.k3}, {T(tb::a064), K.k4}, {T(tb::a065), K.k5}});
+  f(K.k6, tmp, p, stage_t(t, T(tb::c6), dt));
+  comb<T, N>(ar, tmp, uprev, dt,
+             {{T(tb::a071), K.k1}, {T(tb::a073), K.k3}, {T(tb::a074), K.k4}, {T(tb::a075), K.k5},
+              {T(tb::a076), K.k6}});
+  f(K.k7, tmp, p, stage_t(t, T(tb::c7), dt));
+  comb<T, N>(ar, tmp, uprev, dt,
+             {{T(tb::a081), K.k1}, {T(tb::a083), K.k3}, {T(tb::a084), K.k4}, {T(tb::a085), K.k5},
+              {T(tb::a086), K.k6}, {T(tb::a087), K.k7}});
+  f(K.k8, tmp, p, stage_t(t, T(tb::c8), dt));
+  comb<T, N>(ar, tmp, uprev, dt,
+             {{T(tb::a091), K.k1}, {T(tb::a093), K.k3}, {T(tb::a094), K.k4}, {T(tb::a095), K.k5},
+              {T(tb::a096), K.k6}, {T(tb::a097), K.k7}, {T(tb::a098), K.k8}});  // g9
+  if (want_k10) {
+    comb<T, N>(ar, g10, uprev, dt,
+               {{T(tb::a101), K.k1}, {T(tb::a103), K.k3}, {T(tb::a104), K.k4},
+                {T(tb::a105), K.k5}, {T(tb::a106), K.k6}, {T(tb::a107), K.k7}});
+  }
+  f(K.k9, tmp, p, t + (double)dt);
+  if (want_k10) f(K.k10, g10, p, t + (double)dt);
+  comb<T, N>(ar, u, uprev, dt,
+             {{T(tb::b1), K.k1}, {T(tb::b4), K.k4}, {T(tb::b5), K.k5}, {T(tb::b6), K.k6},
+              {T(tb::b7), K.k7}, {T(tb::b8), K.k8}, {T(tb::b9), K.k9}});
+}
+
+// Vern7 dense output at θ over a step (uprev, h) with stages K: the six extra
+// stages k11..k16 evaluated at times ts + c1x*h, then 13 weight polynomials
+// (gpuvern7.jl:140-196 fixed, 428-510 adaptive; bθs at
+// verner_tableaus.jl:1337-1360).
+template <class T, class F>
+inline void vern7_extra(const F& f, int ar, const T* uprev, const T* p, double ts, T h,
+                        const V7Stages<T, F::N>& K, T (*x)[F::N]) {
+  constexpr int N = F::N;
+  namespace tx = sdeh_tab::vern7x;
+  T tmp[N];
+  comb<T, N>(ar, tmp, uprev, h,
+             {{T(tx::a1101), K.k1}, {T(tx::a1104), K.k4}, {T(tx::a1105), K.k5},
+              {T(tx::a1106), K.k6}, {T(tx::a1107), K.k7}, {T(tx::a1108), K.k8},
+              {T(tx::a1109), K.k9}});
+  f(x[0], tmp, p, stage_t(ts, T(tx::c11), h));
+  comb<T, N>(ar, tmp, uprev, h,
+             {{T(tx::a1201), K.k1}, {T(tx::a1204), K.k4}, {T(tx::a1205), K.k5},
+              {T(tx::a1206), K.k6}, {T(tx::a1207), K.k7}, {T(tx::a1208), K.k8},
+              {T(tx::a1209), K.k9}, {T(tx::a1211), x[0]}});
+  f(x[1], tmp, p, stage_t(ts, T(tx::c12), h));
+  comb<T, N>(ar, tmp, uprev, h,
+             {{T(tx::a1301), K.k1}, {T(tx::a1304), K.k4}, {T(tx::a1305), K.k5},
+              {T(tx::a1306), K.k6}, {T(tx::a1307), K.k7}, {T(tx::a1308), K.k8},
+              {T(tx::a1309), K.k9}, {T(tx::a1311), x[0]}, {T(tx::a1312), x[1]}});
+  f(x[2], tmp, p, stage_t(ts, T(tx::c13), h));
+  comb<T, N>(ar, tmp, uprev, h,
+             {{T(tx::a1401), K.k1}, {T(tx::a1404), K.k4}, {T(tx::a1405), K.k5},
+              {T(tx::a1406), K.k6}, {T(tx::a1407), K.k7}, {T(tx::a1408), K.k8},
+              {T(tx::a1409), K.k9}, {T(tx::a1411), x[0]}, {T(tx::a1412), x[1]},
+              {T(tx::a1413), x[2]}});
+  f(x[3], tmp, p, stage_t(ts, T(tx::c14), h));
+  comb<T, N>(ar, tmp, uprev, h,
+             {{T(tx::a1501), K.k1}, {T(tx::a1504), K.k4}, {T(tx::a1505), K.k5},
+              {T(tx::a1506), K.k6}, {T(tx::a1507), K.k7}, {T(tx::a1508), K.k8},
+              {T(tx::a1509), K.k9}, {T(tx::a1511), x[0]}, {T(tx::a1512), x[1]},
+              {T(tx::a1513), x[2]}});
+  f(x[4], tmp, p, stage_t(ts, T(tx::c15), h));
+  comb<T, N>(ar, tmp, uprev, h,
+             {{T(tx::a1601), K.k1}, {T(tx::a1604), K.k4}, {T(tx::a1605), K.k5},
+              {T(tx::a1606), K.k6}, {T(tx::a1607), K.k7}, {T(tx::a1608), K.k8},
+              {T(tx::a1609), K.k9}, {T(tx::a1611), x[0]}, {T(tx::a1612), x[1]},
+              {T(tx::a1613), x[2]}});
+  f(x[5], tmp, p, stage_t(ts, T(tx::c16), h));
+}
+
+template <class T, int N>
+inline void vern7_dense(int ar, T* out, const T* uprev, T h, T th, const V7Stages<T, N>& K,
+                        const T (*x)[N]) {
+  namespace ti = sdeh_tab::vern7i;
+  const T r1[7] = {T(ti::r011), T(ti::r012), T(ti::r013), T(ti::r014), T(ti::r015), T(ti::r016),
+                   T(ti::r017)};
+#define SDEH_R7(j) \
+  const T r##j[6] = {T(ti::r##j##2), T(ti::r##j##3), T(ti::r##j##4), T(ti::r##j##5), \
+                     T(ti::r##j##6), T(ti::r##j##7)};
+  SDEH_R7(04) SDEH_R7(05) SDEH_R7(06) SDEH_R7(07) SDEH_R7(08) SDEH_R7(09)
+  SDEH_R7(11) SDEH_R7(12) SDEH_R7(13) SDEH_R7(14) SDEH_R7(15) SDEH_R7(16)
+#undef SDEH_R7
+  const T b1 = evalpoly_z(th, r1, 7, 1);
+  const T b4 = evalpoly_z(th, r04, 6, 2), b5 = evalpoly_z(th, r05, 6, 2),
+          b6 = evalpoly_z(th, r06, 6, 2), b7 = evalpoly_z(th, r07, 6, 2),
+          b8 = evalpoly_z(th, r08, 6, 2), b9 = evalpoly_z(th, r09, 6, 2),
+          b11 = evalpoly_z(th, r11, 6, 2), b12 = evalpoly_z(th, r12, 6, 2),
+          b13 = evalpoly_z(th, r13, 6, 2), b14 = evalpoly_z(th, r14, 6, 2),
+          b15 = evalpoly_z(th, r15, 6, 2), b16 = evalpoly_z(th, r16, 6, 2);
+  comb<T, N>(ar, out, uprev, h,
+             {{b1, K.k1}, {b4, K.k4}, {b5, K.k5}, {b6, K.k6}, {b7, K.k7}, {b8, K.k8},
+              {b9, K.k9}, {b11, x[0]}, {b12, x[1]}, {b13, x[2]}, {b14, x[3]}, {b15, x[4]},
+              {b16, x[5]}});
+}
+
+// ===========================================================================
+// GPUSimpleVern7 — gpuvern7.jl:55-242
+// ===========================================================================
+template <class T, class F>
+void vern7_fixed(const F& f, Traj<T>& tr, const Mode& md) {
+  constexpr int N = F::N;
+  const int ar = md.arith;
+  const T* p = tr.p;
+  const T dt = (T)tr.dt;
+  double t = tr.t0;
+  int64_t cur = saveat_init<T, N>(tr);
+  if (tr.save_everystep && !tr.saveat) push_step<T, N>(tr, t, tr.u0);
+  T u[N], uprev[N], x[6][N];
+  V7Stages<T, N> K;
+  cpy<T, N>(u, tr.u0);
+  const JRange rg = jl_range(tr.t0, tr.dt, tr.tf);                  // :100
+  for (int64_t i = 1; i < rg.len; ++i) {                            // :103
+    cpy<T, N>(uprev, u);
+    t = rg.elem(i - 1);                                             // :105
+    vern7_stages<T, F>(f, ar, uprev, p, t, dt, K, u, false);        // :106-136
+    const double told = t;
+    t += (double)dt;                                                // :138
+    tr.naccept++;
+    if (!tr.saveat && tr.save_everystep) {
+      push_step<T, N>(tr, t, u);
+    } else if (tr.saveat) {
+      bool have_x = false;
+      while (cur < tr.n_saveat && tr.saveat[cur] <= t) {            // :143
+        const double savet = tr.saveat[cur];
+        const T th = (T)((savet - (t - (double)dt)) / (double)dt);  // :145
+        // The extra stages do not depend on θ; the reference recomputes them per
+        // point with identical inputs, so computing them once is the same.
+        // Quirk: stage times use the post-step t (gpuvern7.jl:160).
+        if (!have_x) {
+          vern7_extra<T, F>(f, ar, uprev, p, md.quirks ? t : told, dt, K, x);
+          have_x = true;
+        }
+        vern7_dense<T, N>(ar, tr.us + cur * N, uprev, dt, th, K, x);
+        ++cur;
+      }
+    }
+  }
+  cpy<T, N>(tr.u_end, u);
+  tr.t_end = t;
+  if (!tr.saveat && !tr.save_everystep) push_step<T, N>(tr, t, u);
+}
+
+// ===========================================================================
+// GPUSimpleAVern7 — gpuvern7.jl:300-536
+// ===========================================================================
+template <class T, class F>
+void avern7(const F& f, Traj<T>& tr, const Mode& md) {
+  constexpr int N = F::N;
+  namespace tb = sdeh_tab::vern7;
+  const int ar = md.arith;
+  const Ctrl<T> Kc;
+  const T* p = tr.p;
+  CtrlState<T> s;
+  s.t = tr.t0;
+  s.tf = tr.tf;
+  s.dt = (T)tr.dt;
+  s.qold = Kc.qoldinit;
+  int64_t cur = saveat_init<T, N>(tr);
+  if (tr.save_everystep && !tr.saveat) push_step<T, N>(tr, s.t, tr.u0);
+  T u[N], uprev[N], tmp[N], x[6][N];
+  V7Stages<T, N> K;
+  cpy<T, N>(u, tr.u0);
+  int64_t attempts = 0;
+  while (s.t < s.tf) {                                              // :353
+    cpy<T, N>(uprev, u);
+    T EEst = std::numeric_limits<T>::infinity();
+    while (EEst > T(1)) {                                           // :357
+      if ((double)s.dt < 1.0e-14) {                                 // :358
+        tr.retcode = SDEH_RET_DTLESSTHANMIN;
+        goto done;
+      }
+      if (attempts >= md.maxiters) {
+        tr.retcode = SDEH_RET_MAXITERS;
+        goto done;
+      }
+      ++attempts;
+      const T dt = s.dt;
+      const double t = s.t;
+      vern7_stages<T, F>(f, ar, uprev, p, t, dt, K, u, true);       // :360-395
+      scaled_sum<T, N>(ar, tmp, dt,
+                       {{T(tb::btilde1), K.k1}, {T(tb::btilde4), K.k4},
+                        {T(tb::btilde5), K.k5}, {T(tb::btilde6), K.k6},
+                        {T(tb::btilde7), K.k7}, {T(tb::btilde8), K.k8},
+                        {T(tb::btilde9), K.k9}, {T(tb::btilde10), K.k10}});  // :397-402
+      EEst = scaled_error_norm<T, N>(ar, tmp, uprev, u, tr.abstol, tr.reltol);  // :403-404
+      const int acc = controller<T>(s, EEst, Kc, md.pow, 1.0e-14);  // :406-426
+      if (acc < 0) {
+        tr.retcode = SDEH_RET_NAN;
+        goto done;
+      }
+      if (acc == 0) {
+        tr.nreject++;
+        continue;
+      }
+      tr.naccept++;
+      if (!tr.saveat && tr.save_everystep) {
+        push_step<T, N>(tr, s.t, u);
+      } else if (tr.saveat) {
+        bool have_x = false;
+        while (cur < tr.n_saveat && tr.saveat[cur] <= s.t) {        // :432
+          const double savet = tr.saveat[cur];
+          const T th = (T)((savet - s.told) / (double)s.dtold);     // :434
+          // Quirk: extra-stage times t + c*dtold use the post-step t (:449).
+          if (!have_x) {
+            vern7_extra<T, F>(f, ar, uprev, p, md.quirks ? s.t : s.told, s.dtold, K, x);
+            have_x = true;
+          }
+          vern7_dense<T, N>(ar, tr.us + cur * N, uprev, s.dtold, th, K, x);
+          ++cur;
+        }
+      }
+    }
+  }
+done:
+  cpy<T, N>(tr.u_end, u);
+  tr.t_end = s.t;
+  if (!tr.saveat && !tr.save_everystep) push_step<T, N>(tr, s.t, u);
+}
+
+// ===========================================================================
+// Verner 9 stage block (gpuvern9.jl:104-165 / 468-546).  Computes k1..k16
+// (k16 only when want_k16) and u.
+// ===========================================================================
+template <class T, int N>
+struct V9Stages {
+  T k[17][N];  // k[1]..k[16]
+};
+
+template <class T, class F>
+inline void vern9_stages(const F& f, int ar, const T* uprev, const T* p, double t, T dt,
+                         V9Stages<T, F::N>& S, T* u, bool want_k16) {
+  constexpr int N = F::N;
+  namespace tb = sdeh_tab::vern9;
+  T tmp[N], g16[N];
+  T(*k)[N] = S.k;
+  f(k[1], uprev, p, t);
+  comb1<T, N>(ar, tmp, uprev, dt * T(tb::a0201), k[1]);
+  f(k[2], tmp, p, stage_t(t, T(tb::c1), dt));
+  comb<T, N>(ar, tmp, uprev, dt, {{T(tb::a0301), k[1]}, {T(tb::a0302), k[2]}});
+  f(k[3], tmp, p, stage_t(t, T(tb::c2), dt));
+  comb<T, N>(ar, tmp, uprev, dt, {{T(tb::a0401), k[1]}, {T(tb::a0403), k[3]}});
+  f(k[4], tmp, p, stage_t(t, T(tb::c3), dt));
+  comb<T, N>(ar, tmp, uprev, dt, {{T(tb::a0501), k[1]}, {T(tb::a0503), k[3]}, {T(tb::a0504), k[4]}});
+  f(k[5], tmp, p, stage_t(t, T(tb::c4), dt));
+  comb<T, N>(ar, tmp, uprev, dt, {{T(tb::a0601), k[1]}, {T(tb::a0604), k[4]}, {T(tb::a0605), k[5]}});
+  f(k[6], tmp, p, stage_t(t, T(tb::c5), dt));
+  comb<T, N>(ar, tmp, uprev, dt,
+             {{T(tb::a0701), k[1]}, {T(tb::a0704), k[4]}, {T(tb::a0705), k[5]}, {T(tb::a0706), k[6]}});
+  f(k[7], tmp, p, stage_t(t, T(tb::c6), dt));
+  comb<T, N>(ar, tmp, uprev, dt, {{T(tb::a0801), k[1]}, {T(tb::a0806), k[6]}, {T(tb::a0807), k[7]}});
+  f(k[8], tmp, p, stage_t(t, T(tb::c7), dt));
+  comb<T, N>(ar, tmp, uprev, dt,
+             {{T(tb::a0901), k[1]}, {T(tb::a0906), k[6]}, {T(tb::a0907), k[7]}, {T(tb::a0908), k[8]}});
+  f(k[9], tmp, p, stage_t(t, T(tb::c8), dt));
+  comb<T, N>(ar, tmp, uprev, dt,
+             {{T(tb::a1001), k[1]}, {T(tb::a1006), k[6]}, {T(tb::a1007), k[7]}, {T(tb::a1008), k[8]},
+              {T(tb::a1009), k[9]}});
+  f(k[10], tmp, p, stage_t(t, T(tb::c9), dt));
+  comb<T, N>(ar, tmp, uprev, dt,
+             {{T(tb::a1101), k[1]}, {T(tb::a1106), k[6]}, {T(tb::a1107), k[7]}, {T(tb::a1108), k[8]},
+              {T(tb::a1109), k[9]}, {T(tb::a1110), k[10]}});
+  f(k[11], tmp, p, stage_t(t, T(tb::c10), dt));
+  comb<T, N>(ar, tmp, uprev, dt,
+             {{T(tb::a1201), k[1]}, {T(tb::a1206), k[6]}, {T(tb::a1207), k[7]}, {T(tb::a1208), k[8]},
+              {T(tb::a1209), k[9]}, {T(tb::a1210), k[10]}, {T(tb::a1211), k[11]}});
+  f(k[12], tmp, p, stage_t(t, T(tb::c11), dt));
+  comb<T, N>(ar, tmp, uprev, dt,
+             {{T(tb::a1301), k[1]}, {T(tb::a1306), k[6]}, {T(tb::a1307), k[7]}, {T(tb::a1308), k[8]},
+              {T(tb::a1309), k[9]}, {T(tb::a1310), k[10]}, {T(tb::a1311), k[11]},
+              {T(tb::a1312), k[12]}});
+  f(k[13], tmp, p, stage_t(t, T(tb::c12), dt));
+  comb<T, N>(ar, tmp, uprev, dt,
+             {{T(tb::a1401), k[1]}, {T(tb::a1406), k[6]}, {T(tb::a1407), k[7]}, {T(tb::a1408), k[8]},
+              {T(tb::a1409), k[9]}, {T(tb::a1410), k[10]}, {T(tb::a1411), k[11]},
+              {T(tb::a1412), k[12]}, {T(tb::a1413), k[13]}});
+  f(k[14], tmp, p, stage_t(t, T(tb::c13), dt));
+  comb<T, N>(ar, tmp, uprev, dt,
+             {{T(tb::a1501), k[1]}, {T(tb::a1506), k[6]}, {T(tb::a1507), k[7]}, {T(tb::a1508), k[8]},
+              {T(tb::a1509), k[9]}, {T(tb::a1510), k[10]}, {T(tb::a1511), k[11]},
+              {T(tb::a1512), k[12]}, {T(tb::a1513), k[13]}, {T(tb::a1514), k[14]}});  // g15
+  if (want_k16) {
+    comb<T, N>(ar, g16, uprev, dt,
+               {{T(tb::a1601), k[1]}, {T(tb::a1606), k[6]}, {T(tb::a1607), k[7]},
+                {T(tb::a1608), k[8]}, {T(tb::a1609), k[9]}, {T(tb::a1610), k[10]},
+                {T(tb::a1611), k[11]}, {T(tb::a1612), k[12]}, {T(tb::a1613), k[13]}});
+  }
+  f(k[15], tmp, p, t + (double)dt);
+  if (want_k16) f(k[16], g16, p, t + (double)dt);
+  comb<T, N>(ar, u, uprev, dt,
+             {{T(tb::b1), k[1]}, {T(tb::b8), k[8]}, {T(tb::b9), k[9]}, {T(tb::b10), k[10]},
+              {T(tb::b11), k[11]}, {T(tb::b12), k[12]}, {T(tb::b13), k[13]}, {T(tb::b14), k[14]},
+              {T(tb::b15), k[15]}});
+}
+
+// Vern9 dense output.  The nine main-stage slots s[0..8] are the vectors that
+// multiply b1Θ, b8Θ..b15Θ and feed a17xx..a26xx (k1 and the renamed k8..k15 in
+// the adaptive solver, gpuvern9.jl:564-573; k1..k9 in the fixed solver, quirk).
+// Extra stages x[0..9] (k11..k20 in the reference) at times ts + c*h
+// (gpuvern9.jl:204-331 fixed, 608-757 adaptive; bθs at
+// verner_tableaus.jl:1362-1398).
+template <class T, class F>
+inline void vern9_extra(const F& f, int ar, const T* uprev, const T* p, double ts, T h,
+                        const T* const* s, T (*x)[F::N]) {
+  constexpr int N = F::N;
+  namespace tx = sdeh_tab::vern9x;
+  T tmp[N];
+#define SDEH_M9(r) {T(tx::a##r##01), s[0]}, {T(tx::a##r##08), s[1]}, {T(tx::a##r##09), s[2]}, \
+    {T(tx::a##r##10), s[3]}, {T(tx::a##r##11), s[4]}, {T(tx::a##r##12), s[5]},               \
+    {T(tx::a##r##13), s[6]}, {T(tx::a##r##14), s[7]}, {T(tx::a##r##15), s[8]}
+  comb<T, N>(ar, tmp, uprev, h, {SDEH_M9(17)});
+  f(x[0], tmp, p, stage_t(ts, T(tx::c17), h));
+  comb<T, N>(ar, tmp, uprev, h, {SDEH_M9(18), {T(tx::a1817), x[0]}});
+  f(x[1], tmp, p, stage_t(ts, T(tx::c18), h));
+  comb<T, N>(ar, tmp, uprev, h, {SDEH_M9(19), {T(tx::a1917), x[0]}, {T(tx::a1918), x[1]}});
+  f(x[2], tmp, p, stage_t(ts, T(tx::c19), h));
+  comb<T, N>(ar, tmp, uprev, h,
+             {SDEH_M9(20), {T(tx::a2017), x[0]}, {T(tx::a2018), x[1]}, {T(tx::a2019), x[2]}});
+  f(x[3], tmp, p, stage_t(ts, T(tx::c20), h));
+  comb<T, N>(ar, tmp, uprev, h,
+             {SDEH_M9(21), {T(tx::a2117), x[0]}, {T(tx::a2118), x[1]}, {T(tx::a2119), x[2]},
+              {T(tx::a2120), x[3]}});
+  f(x[4], tmp, p, stage_t(ts, T(tx::c21), h));
+  comb<T, N>(ar, tmp, uprev, h,
+             {SDEH_M9(22), {T(tx::a2217), x[0]}, {T(tx::a2218), x[1]}, {T(tx::a2219), x[2]},
+              {T(tx::a2220), x[3]}, {T(tx::a2221), x[4]}});
+  f(x[5], tmp, p, stage_t(ts, T(tx::c22), h));
+  comb<T, N>(ar, tmp, uprev, h,
+             {SDEH_M9(23), {T(tx::a2317), x[0]}, {T(tx::a2318), x[1]}, {T(tx::a2319), x[2]},
+              {T(tx::a2320), x[3]}, {T(tx::a2321), x[4]}});
+  f(x[6], tmp, p, stage_t(ts, T(tx::c23), h));
+  comb<T, N>(ar, tmp, uprev, h,
+             {SDEH_M9(24), {T(tx::a2417), x[0]}, {T(tx::a2418), x[1]}, {T(tx::a2419), x[2]},
+              {T(tx::a2420), x[3]}, {T(tx::a2421), x[4]}});
+  f(x[7], tmp, p, stage_t(ts, T(tx::c24), h));
+  comb<T, N>(ar, tmp, uprev, h,
+             {SDEH_M9(25), {T(tx::a2517), x[0]}, {T(tx::a2518), x[1]}, {T(tx::a2519), x[2]},
+              {T(tx::a2520), x[3]}, {T(tx::a2521), x[4]}});
+  f(x[8], tmp, p, stage_t(ts, T(tx::c25), h));
+  comb<T, N>(ar, tmp, uprev, h,
+             {SDEH_M9(26), {T(tx::a2617), x[0]}, {T(tx::a2618), x[1]}, {T(tx::a2619), x[2]},
+              {T(tx::a2620), x[3]}, {T(tx::a2621), x[4]}});
+  f(x[9], tmp, p, stage_t(ts, T(tx::c26), h));
+#undef SDEH_M9
+}
+
+template <class T, int N>
+inline void vern9_dense(int ar, T* out, const T* uprev, T h, T th, const T* const* s,
+                        const T (*x)[N]) {
+  namespace ti = sdeh_tab::vern9i;
+  const T r1[9] = {T(ti::r011), T(ti::r012), T(ti::r013), T(ti::r014), T(ti::r015),
+                   T(ti::r016), T(ti::r017), T(ti::r018), T(ti::r019)};
+#define SDEH_R9(j)                                                                         \
+  const T r##j[8] = {T(ti::r##j##2), T(ti::r##j##3), T(ti::r##j##4), T(ti::r##j##5),      \
+                     T(ti::r##j##6), T(ti::r##j##7), T(ti::r##j##8), T(ti::r##j##9)};
+  SDEH_R9(08) SDEH_R9(09) SDEH_R9(10) SDEH_R9(11) SDEH_R9(12) SDEH_R9(13) SDEH_R9(14)
+  SDEH_R9(15) SDEH_R9(17) SDEH_R9(18) SDEH_R9(19) SDEH_R9(20) SDEH_R9(21) SDEH_R9(22)
+  SDEH_R9(23) SDEH_R9(24) SDEH_R9(25) SDEH_R9(26)
+#undef SDEH_R9
+  const T b1 = evalpoly_z(th, r1, 9, 1);
+#define SDEH_B9(j) const T b##j = evalpoly_z(th, r##j, 8, 2);
+  SDEH_B9(08) SDEH_B9(09) SDEH_B9(10) SDEH_B9(11) SDEH_B9(12) SDEH_B9(13) SDEH_B9(14)
+  SDEH_B9(15) SDEH_B9(17) SDEH_B9(18) SDEH_B9(19) SDEH_B9(20) SDEH_B9(21) SDEH_B9(22)
+  SDEH_B9(23) SDEH_B9(24) SDEH_B9(25) SDEH_B9(26)
+#undef SDEH_B9
+  comb<T, N>(ar, out, uprev, h,
+             {{b1, s[0]}, {b08, s[1]}, {b09, s[2]}, {b10, s[3]}, {b11, s[4]}, {b12, s[5]},
+              {b13, s[6]}, {b14, s[7]}, {b15, s[8]}, {b17, x[0]}, {b18, x[1]}, {b19, x[2]},
+              {b20, x[3]}, {b21, x[4]}, {b22, x[5]}, {b23, x[6]}, {b24, x[7]}, {b25, x[8]},
+              {b26, x[9]}});
+}
+
+// ===========================================================================
+// GPUSimpleVern9 — gpuvern9.jl:55-353
+// ===========================================================================
+template <class T, class F>
+void vern9_fixed(const F& f, Traj<T>& tr, const Mode& md) {
+  constexpr int N = F::N;
+  const int ar = md.arith;
+  const T* p = tr.p;
+  const T dt = (T)tr.dt;
+  double t = tr.t0;
+  int64_t cur = saveat_init<T, N>(tr);
+  if (tr.save_everystep && !tr.saveat) push_step<T, N>(tr, t, tr.u0);
+  T u[N], uprev[N], x[10][N];
+  V9Stages<T, N> S;
+  cpy<T, N>(u, tr.u0);
+  const JRange rg = jl_range(tr.t0, tr.dt, tr.tf);                  // :100
+  for (int64_t i = 1; i < rg.len; ++i) {                            // :103
+    cpy<T, N>(uprev, u);
+    t = rg.elem(i - 1);                                             // :105
+    vern9_stages<T, F>(f, ar, uprev, p, t, dt, S, u, false);        // :106-174
+    const double told = t;
+    t += (double)dt;                                                // :185
+    tr.naccept++;
+    if (!tr.saveat && tr.save_everystep) {
+      push_step<T, N>(tr, t, u);
+    } else if (tr.saveat) {
+      // Quirk Q1: the fixed solver does not rename k8..k15 into k2..k9 before
+      // the dense output (contrast gpuvern9.jl:564-573), so the slots take
+      // k1..k9; quirk Q2: extra-stage times use the post-step t (:222).
+      const T* sl_q[9] = {S.k[1], S.k[2], S.k[3], S.k[4], S.k[5], S.k[6], S.k[7], S.k[8], S.k[9]};
+      const T* sl_c[9] = {S.k[1], S.k[8], S.k[9], S.k[10], S.k[11], S.k[12], S.k[13], S.k[14],
+                          S.k[15]};
+      const T* const* sl = md.quirks ? sl_q : sl_c;
+      bool have_x = false;
+      while (cur < tr.n_saveat && tr.saveat[cur] <= t) {            // :190
+        const double savet = tr.saveat[cur];
+        const T th = (T)((savet - (t - (double)dt)) / (double)dt);  // :192
+        if (!have_x) {
+          vern9_extra<T, F>(f, ar, uprev, p, md.quirks ? t : told, dt, sl, x);
+          have_x = true;
+        }
+        vern9_dense<T, N>(ar, tr.us + cur * N, uprev, dt, th, sl, x);
+        ++cur;
+      }
+    }
+  }
+  cpy<T, N>(tr.u_end, u);
+  tr.t_end = t;
+  if (!tr.saveat && !tr.save_everystep) push_step<T, N>(tr, t, u);
+}
+
+// ===========================================================================
+// GPUSimpleAVern9 — gpuvern9.jl:411-779
+// ===========================================================================
+template <class T, class F>
+void avern9(const F& f, Traj<T>& tr, const Mode& md) {
+  constexpr int N = F::N;
+  namespace tb = sdeh_tab::vern9;
+  const int ar = md.arith;
+  const Ctrl<T> Kc;
+  const double dtmin = (double)1.0e-7f;                             // :466, :591 (1f-7)
+  const T* p = tr.p;
+  CtrlState<T> s;
+  s.t = tr.t0;
+  s.tf = tr.tf;
+  s.dt = (T)tr.dt;
+  s.qold = Kc.qoldinit;
+  int64_t cur = saveat_init<T, N>(tr);
+  if (tr.save_everystep && !tr.saveat) push_step<T, N>(tr, s.t, tr.u0);
+  T u[N], uprev[N], tmp[N], x[10][N];
+  V9Stages<T, N> S;
+  cpy<T, N>(u, tr.u0);
+  int64_t attempts = 0;
+  while (s.t < s.tf) {                                              // :461
+    cpy<T, N>(uprev, u);
+    T EEst = std::numeric_limits<T>::infinity();
+    while (EEst > T(1)) {                                           // :465
+      if ((double)s.dt < dtmin) {                                   // :466
+        tr.retcode = SDEH_RET_DTLESSTHANMIN;
+        goto done;
+      }
+      if (attempts >= md.maxiters) {
+        tr.retcode = SDEH_RET_MAXITERS;
+        goto done;
+      }
+      ++attempts;
+      const T dt = s.dt;
+      const double t = s.t;
+      vern9_stages<T, F>(f, ar, uprev, p, t, dt, S, u, true);       // :468-554
+      scaled_sum<T, N>(ar, tmp, dt,
+                       {{T(tb::btilde1), S.k[1]}, {T(tb::btilde8), S.k[8]},
+                        {T(tb::btilde9), S.k[9]}, {T(tb::btilde10), S.k[10]},
+                        {T(tb::btilde11), S.k[11]}, {T(tb::btilde12), S.k[12]},
+                        {T(tb::btilde13), S.k[13]}, {T(tb::btilde14), S.k[14]},
+                        {T(tb::btilde15), S.k[15]}, {T(tb::btilde16), S.k[16]}});  // :556-561
+      EEst = scaled_error_norm<T, N>(ar, tmp, uprev, u, tr.abstol, tr.reltol);  // :561-562
+      const int acc = controller<T>(s, EEst, Kc, md.pow, dtmin);    // :575-595
+      if (acc < 0) {
+        tr.retcode = SDEH_RET_NAN;
+        goto done;
+      }
+      if (acc == 0) {
+        tr.nreject++;
+        continue;
+      }
+      tr.naccept++;
+      if (!tr.saveat && tr.save_everystep) {
+        push_step<T, N>(tr, s.t, u);
+      } else if (tr.saveat) {
+        // Renamed stages k1, k8..k15 (gpuvern9.jl:564-573); times told + c*dtold.
+        const T* sl[9] = {S.k[1], S.k[8], S.k[9], S.k[10], S.k[11], S.k[12], S.k[13], S.k[14],
+                          S.k[15]};
+        bool have_x = false;
+        while (cur < tr.n_saveat && tr.saveat[cur] <= s.t) {        // :600
+          const double savet = tr.saveat[cur];
+          const T th = (T)((savet - s.told) / (double)s.dtold);     // :602
+          if (!have_x) {
+            vern9_extra<T, F>(f, ar, uprev, p, s.told, s.dtold, sl, x);
+            have_x = true;
+          }
+          vern9_dense<T, N>(ar, tr.us + cur * N, uprev, s.dtold, th, sl, x);
+          ++cur;
+        }
+      }
+    }
+  }
+done:
+  cpy<T, N>(tr.u_end, u);
+  tr.t_end = s.t;
+  if (!tr.saveat && !tr.save_everystep) push_step<T, N>(tr, s.t, u);
+}
+
+template <class T, class F>
+inline void solve_one(int alg, const F& f, Traj<T>& tr, const Mode& md) {
+  switch (alg) {
+    case SDEH_ALG_TSIT5: tsit5_fixed<T, F>(f, tr, md); break;
+    case SDEH_ALG_ATSIT5: atsit5<T, F>(f, tr, md); break;
+    case SDEH_ALG_VERN7: vern7_fixed<T, F>(f, tr, md); break;
+    case SDEH_ALG_AVERN7: avern7<T, F>(f, tr, md); break;
+    case SDEH_ALG_VERN9: vern9_fixed<T, F>(f, tr, md); break;
+    case SDEH_ALG_AVERN9: avern9<T, F>(f, tr, md); break;
+    default: break;
+  }
+}
+
+}  // namespace oracle

@@ -1,0 +1,940 @@
+// Device integrators: the six GPU-compatible solvers of SimpleDiffEq.jl as
+// HIP kernels for gfx950, one trajectory per lane.
+//
+//   GPUSimpleTsit5   src/tsit5/gpuatsit5.jl:55-147    sdeh_fixed<..., ALG_TSIT5>
+//   GPUSimpleATsit5  src/tsit5/gpuatsit5.jl:205-336   sdeh_adaptive<..., ALG_ATSIT5>
+//   GPUSimpleVern7   src/verner/gpuvern7.jl:55-242    sdeh_fixed<..., ALG_VERN7>
+//   GPUSimpleAVern7  src/verner/gpuvern7.jl:300-536   sdeh_adaptive<..., ALG_AVERN7>
+//   GPUSimpleVern9   src/verner/gpuvern9.jl:55-353    sdeh_fixed<..., ALG_VERN9>
+//   GPUSimpleAVern9  src/verner/gpuvern9.jl:411-779   sdeh_adaptive<..., ALG_AVERN9>
+//
+// Execution model.  The reference runs one sequential `while t < tf / while
+// EEst > 1` nest per trajectory.  Here a lane owns a trajectory and the nest is
+// flattened: every iteration of the kernel loop is one step attempt for every
+// live lane, accepted or rejected, so a lane that accepts never waits for a
+// neighbour that rejects.  Lanes whose trajectory ended either idle until the
+// wave drains (one-launch schedule) or take the next trajectory from a device
+// atomic queue (persistent schedule): the wave claims `chunk` queue slots with
+// one atomic and hands them to its free lanes in lane order (ballot + mbcnt).
+// The queue hands out slots < n only, so every wave reaches its exit once all
+// slots are claimed and its lanes have finished; every trajectory ends after
+// at most `maxiters` attempts.
+//
+// Numerics.  State, stages and the controller scalars are in T (double or
+// float); time t, tf and saveat are binary64.  Arithmetic follows one fixed
+// order, the fused reading of the reference's @muladd bodies:
+//   uprev + dt*(a1*k1 + ... + am*km):  acc = a1*k1; acc = fma(aj, kj, acc);
+//                                      out = fma(dt, acc, uprev)
+//   uprev + dt*a21*k1 (one term):     out = fma(dt*a21, k1, uprev)
+//   stage time t + c*dt:               fma(c, dt, t) in binary64
+//   abstol + max(|uprev|,|u|)*reltol:  fma(max, reltol, abstol)
+//   ODE_DEFAULT_NORM:                  sqrt(((e1*e1 + e2*e2) + ...) / N)
+//   bθs polynomials:                   Horner with fma (Base.evalpoly)
+// and the library is compiled with -ffp-contract=off, so the compiler adds no
+// other fusion.  In binary64 every operation used is correctly rounded on
+// gfx950 (fma, +, *, /, sqrt) and the controller's pow is sdeh_math::pow_ctrl,
+// so a lane reproduces the CPU restatement (oracle, fused mode) bit for bit.
+// The binary32 mode uses hardware reciprocal, square root, log2 and exp2 in the
+// controller and error scaling: it is a precision mode judged by its error
+// against binary64 results, not by bits.
+#pragma once
+
+#include "sdeh_kargs.h"
+#include "sdeh_math.h"
+#include "sdeh_tableaus.h"
+
+namespace sdeh {
+
+enum { ALG_TSIT5 = 0, ALG_ATSIT5 = 1, ALG_VERN7 = 2, ALG_AVERN7 = 3, ALG_VERN9 = 4,
+       ALG_AVERN9 = 5 };
+enum { SAVE_END = 0, SAVE_AT = 1, SAVE_STEPS = 2 };
+enum { RET_DEFAULT = 0, RET_DTMIN = 1, RET_NAN = 2, RET_MAXITERS = 3, RET_SAVE_OVERFLOW = 4 };
+
+template <class T>
+struct Id {
+  typedef T type;
+};
+
+// ---------------------------------------------------------------------------
+// Scalar operations by state type.
+// ---------------------------------------------------------------------------
+template <class T>
+struct Ops;
+
+template <>
+struct Ops<double> {
+  __device__ static inline double div(double a, double b) { return a / b; }
+  __device__ static inline double root(double x) { return __builtin_sqrt(x); }
+  __device__ static inline double pw(double x, double y) { return sdeh_math::pow_ctrl(x, y); }
+};
+
+template <>
+struct Ops<float> {
+  __device__ static inline float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  __device__ static inline float root(float x) { return __builtin_amdgcn_sqrtf(x); }
+  __device__ static inline float pw(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Linear combinations over stage vectors (arrays held in registers).
+// ---------------------------------------------------------------------------
+template <class T, int N>
+__device__ __forceinline__ void acc_more(T (&)[N]) {}
+
+template <class T, int N, class... R>
+__device__ __forceinline__ void acc_more(T (&acc)[N], const typename Id<T>::type a,
+                                         const T (&k)[N], const R&... rest) {
+#pragma unroll
+  for (int c = 0; c < N; ++c) acc[c] = fma(a, k[c], acc[c]);
+  acc_more(acc, rest...);
+}
+
+// acc = a0*k0 + a1*k1 + ... (fma chain, left to right)
+template <class T, int N, class... R>
+__device__ __forceinline__ void lsum(T (&acc)[N], const typename Id<T>::type a0,
+                                     const T (&k0)[N], const R&... rest) {
+#pragma unroll
+  for (int c = 0; c < N; ++c) acc[c] = a0 * k0[c];
+  acc_more(acc, rest...);
+}
+
+// out = base + h * (a0*k0 + a1*k1 + ...)
+template <class T, int N, class... R>
+__device__ __forceinline__ void comb(T (&out)[N], const T (&base)[N],
+                                     const typename Id<T>::type h, const R&... terms) {
+  T acc[N];
+  lsum(acc, terms...);
+#pragma unroll
+  for (int c = 0; c < N; ++c) out[c] = fma(h, acc[c], base[c]);
+}
+
+// out = base + s*k (first stage, s = dt * a21)
+template <class T, int N>
+__device__ __forceinline__ void comb1(T (&out)[N], const T (&base)[N],
+                                      const typename Id<T>::type s, const T (&k)[N]) {
+#pragma unroll
+  for (int c = 0; c < N; ++c) out[c] = fma(s, k[c], base[c]);
+}
+
+// Compensated form of out = base + h*acc for the binary32 precision mode:
+// y = h*acc - comp; out = base + y; comp' = (out - base) - y.
+template <class T, int N>
+__device__ __forceinline__ void kahan_update(T (&out)[N], T (&comp_out)[N], const T (&base)[N],
+                                             const T (&comp)[N], T h, const T (&acc)[N]) {
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    const T y = fma(h, acc[c], -comp[c]);
+    const T s = base[c] + y;
+    comp_out[c] = (s - base[c]) - y;
+    out[c] = s;
+  }
+}
+
+template <class T, int N>
+__device__ __forceinline__ void copyv(T (&d)[N], const T (&s)[N]) {
+#pragma unroll
+  for (int c = 0; c < N; ++c) d[c] = s[c];
+}
+
+__device__ __forceinline__ double stage_t(double t, double c, double dt) {
+  return fma(c, dt, t);
+}
+
+// tmp <- tmp / (abstol + max(|uprev|, |u|) * reltol); return ODE_DEFAULT_NORM(tmp).
+template <class T, int N>
+__device__ __forceinline__ T err_norm(T (&tmp)[N], const T (&uprev)[N], const T (&u)[N],
+                                      T abstol, T reltol) {
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    const T m = sdeh_math::jl_max(fabs(uprev[c]), fabs(u[c]));
+    tmp[c] = Ops<T>::div(tmp[c], fma(m, reltol, abstol));
+  }
+  T s = tmp[0] * tmp[0];
+#pragma unroll
+  for (int c = 1; c < N; ++c) s = s + tmp[c] * tmp[c];
+  return Ops<T>::root(Ops<T>::div(s, T(N)));
+}
+
+// Horner evaluation of evalpoly(θ, (0, r...)) / evalpoly(θ, (0, 0, r...)).
+template <class T, int NR>
+__device__ __forceinline__ T poly1(T th, const T (&r)[NR]) {
+  T h = r[NR - 1];
+#pragma unroll
+  for (int i = NR - 2; i >= 0; --i) h = fma(th, h, r[i]);
+  return fma(th, h, T(0));
+}
+template <class T, int NR>
+__device__ __forceinline__ T poly2(T th, const T (&r)[NR]) {
+  T h = r[NR - 1];
+#pragma unroll
+  for (int i = NR - 2; i >= 0; --i) h = fma(th, h, r[i]);
+  h = fma(th, h, T(0));
+  return fma(th, h, T(0));
+}
+
+// ---------------------------------------------------------------------------
+// Stage blocks.  Each computes the trial state u from uprev over (t, dt).
+// `kahan` variants return the compensation of the u update.
+// ---------------------------------------------------------------------------
+
+#define TS(x) T(sdeh_tab::tsit5::x)
+
+// Tsit5 stages k2..k7 from k1 (FSAL); gpuatsit5.jl:99-110 / 258-270.
+template <class T, class R, bool KAHAN>
+__device__ __forceinline__ void tsit5_stages(const T (&uprev)[R::N], const T (&k1)[R::N],
+                                             T (&k2)[R::N], T (&k3)[R::N], T (&k4)[R::N],
+                                             T (&k5)[R::N], T (&k6)[R::N], T (&k7)[R::N],
+                                             T (&u)[R::N], const T (&comp)[R::N],
+                                             T (&comp_new)[R::N], const T* p, const T* ps,
+                                             double t, T dt) {
+  constexpr int N = R::N;
+  T tmp[N];
+  comb1(tmp, uprev, dt * TS(a21), k1);
+  R::f(k2, tmp, p, ps, stage_t(t, (double)TS(c1), (double)dt));
+  comb(tmp, uprev, dt, TS(a31), k1, TS(a32), k2);
+  R::f(k3, tmp, p, ps, stage_t(t, (double)TS(c2), (double)dt));
+  comb(tmp, uprev, dt, TS(a41), k1, TS(a42), k2, TS(a43), k3);
+  R::f(k4, tmp, p, ps, stage_t(t, (double)TS(c3), (double)dt));
+  comb(tmp, uprev, dt, TS(a51), k1, TS(a52), k2, TS(a53), k3, TS(a54), k4);
+  R::f(k5, tmp, p, ps, stage_t(t, (double)TS(c4), (double)dt));
+  comb(tmp, uprev, dt, TS(a61), k1, TS(a62), k2, TS(a63), k3, TS(a64), k4, TS(a65), k5);
+  R::f(k6, tmp, p, ps, t + (double)dt);
+  if constexpr (KAHAN) {
+    T acc[N];
+    lsum(acc, TS(a71), k1, TS(a72), k2, TS(a73), k3, TS(a74), k4, TS(a75), k5, TS(a76), k6);
+    kahan_update(u, comp_new, uprev, comp, dt, acc);
+  } else {
+    comb(u, uprev, dt, TS(a71), k1, TS(a72), k2, TS(a73), k3, TS(a74), k4, TS(a75), k5,
+         TS(a76), k6);
+  }
+  R::f(k7, u, p, ps, t + (double)dt);
+}
+
+// Tsit5 dense output (tsit5.jl:385-399 weights) at θ over (uprev, h).
+template <class T, int N>
+__device__ __forceinline__ void tsit5_dense(T (&out)[N], const T (&uprev)[N], T h, T th,
+                                            const T (&k1)[N], const T (&k2)[N],
+                                            const T (&k3)[N], const T (&k4)[N],
+                                            const T (&k5)[N], const T (&k6)[N],
+                                            const T (&k7)[N]) {
+  const T r1[4] = {TS(r11), TS(r12), TS(r13), TS(r14)};
+  const T r2[3] = {TS(r22), TS(r23), TS(r24)};
+  const T r3[3] = {TS(r32), TS(r33), TS(r34)};
+  const T r4[3] = {TS(r42), TS(r43), TS(r44)};
+  const T r5[3] = {TS(r52), TS(r53), TS(r54)};
+  const T r6[3] = {TS(r62), TS(r63), TS(r64)};
+  const T r7[3] = {TS(r72), TS(r73), TS(r74)};
+  comb(out, uprev, h, poly1(th, r1), k1, poly2(th, r2), k2, poly2(th, r3), k3, poly2(th, r4),
+       k4, poly2(th, r5), k5, poly2(th, r6), k6, poly2(th, r7), k7);
+}
+#undef TS
+
+#define V7(x) T(sdeh_tab::vern7::x)
+#define V7X(x) T(sdeh_tab::vern7x::x)
+#define V7I(x) T(sdeh_tab::vern7i::x)
+
+template <class T, int N>
+struct V7K {
+  T k1[N], k4[N], k5[N], k6[N], k7[N], k8[N], k9[N];
+};
+
+// Verner 7 stages k1..k9 (+k10, error) and u; gpuvern7.jl:104-136 / 360-402.
+// Returns the embedded error accumulator in err (adaptive only).
+template <class T, class R, bool ADAPT, bool KAHAN>
+__device__ __forceinline__ void vern7_stages(const T (&uprev)[R::N], V7K<T, R::N>& K,
+                                             T (&u)[R::N], T (&err)[R::N],
+                                             const T (&comp)[R::N], T (&comp_new)[R::N],
+                                             const T* p, const T* ps, double t, T dt) {
+  constexpr int N = R::N;
+  T tmp[N], k2[N], k3[N];
+  R::f(K.k1, uprev, p, ps, t);
+  comb1(tmp, uprev, dt * V7(a021), K.k1);
+  R::f(k2, tmp, p, ps, stage_t(t, (double)V7(c2), (double)dt));
+  comb(tmp, uprev, dt, V7(a031), K.k1, V7(a032), k2);
+  R::f(k3, tmp, p, ps, stage_t(t, (double)V7(c3), (double)dt));
+  comb(tmp, uprev, dt, V7(a041), K.k1, V7(a043), k3);
+  R::f(K.k4, tmp, p, ps, stage_t(t, (double)V7(c4), (double)dt));
+  comb(tmp, uprev, dt, V7(a051), K.k1, V7(a053), k3, V7(a054), K.k4);
+  R::f(K.k5, tmp, p, ps, stage_t(t, (double)V7(c5), (double)dt));
+  comb(tmp, uprev, dt, V7(a061), K.k1, V7(a063), k3, V7(a064), K.k4, V7(a065), K.k5);
+  R::f(K.k6, tmp, p, ps, stage_t(t, (double)V7(c6), (double)dt));
+  comb(tmp, uprev, dt, V7(a071), K.k1, V7(a073), k3, V7(a074), K.k4, V7(a075), K.k5, V7(a076),
+       K.k6);
+  R::f(K.k7, tmp, p, ps, stage_t(t, (double)V7(c7), (double)dt));
+  comb(tmp, uprev, dt, V7(a081), K.k1, V7(a083), k3, V7(a084), K.k4, V7(a085), K.k5, V7(a086),
+       K.k6, V7(a087), K.k7);
+  R::f(K.k8, tmp, p, ps, stage_t(t, (double)V7(c8), (double)dt));
+  comb(tmp, uprev, dt, V7(a091), K.k1, V7(a093), k3, V7(a094), K.k4, V7(a095), K.k5, V7(a096),
+       K.k6, V7(a097), K.k7, V7(a098), K.k8);
+  T k10[N];
+  if constexpr (ADAPT) {
+    T g10[N];
+    comb(g10, uprev, dt, V7(a101), K.k1, V7(a103), k3, V7(a104), K.k4, V7(a105), K.k5,
+         V7(a106), K.k6, V7(a107), K.k7);
+    R::f(K.k9, tmp, p, ps, t + (double)dt);
+    R::f(k10, g10, p, ps, t + (double)dt);
+  } else {
+    R::f(K.k9, tmp, p, ps, t + (double)dt);
+  }
+  if constexpr (KAHAN) {
+    T acc[N];
+    lsum(acc, V7(b1), K.k1, V7(b4), K.k4, V7(b5), K.k5, V7(b6), K.k6, V7(b7), K.k7, V7(b8),
+         K.k8, V7(b9), K.k9);
+    kahan_update(u, comp_new, uprev, comp, dt, acc);
+  } else {
+    comb(u, uprev, dt, V7(b1), K.k1, V7(b4), K.k4, V7(b5), K.k5, V7(b6), K.k6, V7(b7), K.k7,
+         V7(b8), K.k8, V7(b9), K.k9);
+  }
+  if constexpr (ADAPT) {
+    lsum(err, V7(btilde1), K.k1, V7(btilde4), K.k4, V7(btilde5), K.k5, V7(btilde6), K.k6,
+         V7(btilde7), K.k7, V7(btilde8), K.k8, V7(btilde9), K.k9, V7(btilde10), k10);
+  }
+}
+
+// Vern7 extra stages k11..k16 at times ts + c*h (gpuvern7.jl:147-190 / 436-508).
+template <class T, class R>
+__device__ __forceinline__ void vern7_extra(const T (&uprev)[R::N], const V7K<T, R::N>& K,
+                                            T (&x)[6][R::N], const T* p, const T* ps,
+                                            double ts, T h) {
+  constexpr int N = R::N;
+  T tmp[N];
+  comb(tmp, uprev, h, V7X(a1101), K.k1, V7X(a1104), K.k4, V7X(a1105), K.k5, V7X(a1106), K.k6,
+       V7X(a1107), K.k7, V7X(a1108), K.k8, V7X(a1109), K.k9);
+  R::f(x[0], tmp, p, ps, stage_t(ts, (double)V7X(c11), (double)h));
+  comb(tmp, uprev, h, V7X(a1201), K.k1, V7X(a1204), K.k4, V7X(a1205), K.k5, V7X(a1206), K.k6,
+       V7X(a1207), K.k7, V7X(a1208), K.k8, V7X(a1209), K.k9, V7X(a1211), x[0]);
+  R::f(x[1], tmp, p, ps, stage_t(ts, (double)V7X(c12), (double)h));
+  comb(tmp, uprev, h, V7X(a1301), K.k1, V7X(a1304), K.k4, V7X(a1305), K.k5, V7X(a1306), K.k6,
+       V7X(a1307), K.k7, V7X(a1308), K.k8, V7X(a1309), K.k9, V7X(a1311), x[0], V7X(a1312),
+       x[1]);
+  R::f(x[2], tmp, p, ps, stage_t(ts, (double)V7X(c13), (double)h));
+  comb(tmp, uprev, h, V7X(a1401), K.k1, V7X(a1404), K.k4, V7X(a1405), K.k5, V7X(a1406), K.k6,
+       V7X(a1407), K.k7, V7X(a1408), K.k8, V7X(a1409), K.k9, V7X(a1411), x[0], V7X(a1412),
+       x[1], V7X(a1413), x[2]);
+  R::f(x[3], tmp, p, ps, stage_t(ts, (double)V7X(c14), (double)h));
+  comb(tmp, uprev, h, V7X(a1501), K.k1, V7X(a1504), K.k4, V7X(a1505), K.k5, V7X(a1506), K.k6,
+       V7X(a1507), K.k7, V7X(a1508), K.k8, V7X(a1509), K.k9, V7X(a1511), x[0], V7X(a1512),
+       x[1], V7X(a1513), x[2]);
+  R::f(x[4], tmp, p, ps, stage_t(ts, (double)V7X(c15), (double)h));
+  comb(tmp, uprev, h, V7X(a1601), K.k1, V7X(a1604), K.k4, V7X(a1605), K.k5, V7X(a1606), K.k6,
+       V7X(a1607), K.k7, V7X(a1608), K.k8, V7X(a1609), K.k9, V7X(a1611), x[0], V7X(a1612),
+       x[1], V7X(a1613), x[2]);
+  R::f(x[5], tmp, p, ps, stage_t(ts, (double)V7X(c16), (double)h));
+}
+
+// Vern7 dense output (bθs at verner_tableaus.jl:1337-1360).
+template <class T, int N>
+__device__ __forceinline__ void vern7_dense(T (&out)[N], const T (&uprev)[N], T h, T th,
+                                            const V7K<T, N>& K, const T (&x)[6][N]) {
+  const T r1[7] = {V7I(r011), V7I(r012), V7I(r013), V7I(r014), V7I(r015), V7I(r016),
+                   V7I(r017)};
+#define SDEH_R7(j) \
+  const T r##j[6] = {V7I(r##j##2), V7I(r##j##3), V7I(r##j##4), V7I(r##j##5), V7I(r##j##6), \
+                     V7I(r##j##7)};
+  SDEH_R7(04) SDEH_R7(05) SDEH_R7(06) SDEH_R7(07) SDEH_R7(08) SDEH_R7(09)
+  SDEH_R7(11) SDEH_R7(12) SDEH_R7(13) SDEH_R7(14) SDEH_R7(15) SDEH_R7(16)
+#undef SDEH_R7
+  comb(out, uprev, h, poly1(th, r1), K.k1, poly2(th, r04), K.k4, poly2(th, r05), K.k5,
+       poly2(th, r06), K.k6, poly2(th, r07), K.k7, poly2(th, r08), K.k8, poly2(th, r09), K.k9,
+       poly2(th, r11), x[0], poly2(th, r12), x[1], poly2(th, r13), x[2], poly2(th, r14), x[3],
+       poly2(th, r15), x[4], poly2(th, r16), x[5]);
+}
+#undef V7
+#undef V7X
+#undef V7I
+
+#define V9(x) T(sdeh_tab::vern9::x)
+#define V9X(x) T(sdeh_tab::vern9x::x)
+#define V9I(x) T(sdeh_tab::vern9i::x)
+
+// The nine stage vectors the Vern9 dense output reads (slots of b1Θ, b8Θ..b15Θ).
+template <class T, int N>
+struct V9S {
+  T s[9][N];
+};
+
+// Verner 9 stages k1..k15 (+k16, error) and u; gpuvern9.jl:104-174 / 468-561.
+// On return `S` holds the dense-output slots: k1, k8..k15 (renamed as at
+// gpuvern9.jl:564-573) or, when `alias_quirk`, k1..k9 (the fixed solver's
+// unrenamed variables, gpuvern9.jl:216-331).
+template <class T, class R, bool ADAPT, bool KAHAN, bool KEEP>
+__device__ __forceinline__ void vern9_stages(const T (&uprev)[R::N], V9S<T, R::N>& S,
+                                             T (&u)[R::N], T (&err)[R::N],
+                                             const T (&comp)[R::N], T (&comp_new)[R::N],
+                                             const T* p, const T* ps, double t, T dt,
+                                             bool alias_quirk) {
+  constexpr int N = R::N;
+  T tmp[N], k1[N], k2[N], k3[N], k4[N], k5[N], k6[N], k7[N], k8[N], k9[N], k10[N], k11[N],
+      k12[N], k13[N], k14[N], k15[N];
+  R::f(k1, uprev, p, ps, t);
+  comb1(tmp, uprev, dt * V9(a0201), k1);
+  R::f(k2, tmp, p, ps, stage_t(t, (double)V9(c1), (double)dt));
+  comb(tmp, uprev, dt, V9(a0301), k1, V9(a0302), k2);
+  R::f(k3, tmp, p, ps, stage_t(t, (double)V9(c2), (double)dt));
+  comb(tmp, uprev, dt, V9(a0401), k1, V9(a0403), k3);
+  R::f(k4, tmp, p, ps, stage_t(t, (double)V9(c3), (double)dt));
+  comb(tmp, uprev, dt, V9(a0501), k1, V9(a0503), k3, V9(a0504), k4);
+  R::f(k5, tmp, p, ps, stage_t(t, (double)V9(c4), (double)dt));
+  comb(tmp, uprev, dt, V9(a0601), k1, V9(a0604), k4, V9(a0605), k5);
+  R::f(k6, tmp, p, ps, stage_t(t, (double)V9(c5), (double)dt));
+  comb(tmp, uprev, dt, V9(a0701), k1, V9(a0704), k4, V9(a0705), k5, V9(a0706), k6);
+  R::f(k7, tmp, p, ps, stage_t(t, (double)V9(c6), (double)dt));
+  comb(tmp, uprev, dt, V9(a0801), k1, V9(a0806), k6, V9(a0807), k7);
+  R::f(k8, tmp, p, ps, stage_t(t, (double)V9(c7), (double)dt));
+  comb(tmp, uprev, dt, V9(a0901), k1, V9(a0906), k6, V9(a0907), k7, V9(a0908), k8);
+  R::f(k9, tmp, p, ps, stage_t(t, (double)V9(c8), (double)dt));
+  if constexpr (KEEP) {
+    if (alias_quirk) {
+      copyv(S.s[1], k2); copyv(S.s[2], k3); copyv(S.s[3], k4); copyv(S.s[4], k5);
+      copyv(S.s[5], k6); copyv(S.s[6], k7); copyv(S.s[7], k8); copyv(S.s[8], k9);
+    }
+  }
+  comb(tmp, uprev, dt, V9(a1001), k1, V9(a1006), k6, V9(a1007), k7, V9(a1008), k8, V9(a1009),
+       k9);
+  R::f(k10, tmp, p, ps, stage_t(t, (double)V9(c9), (double)dt));
+  comb(tmp, uprev, dt, V9(a1101), k1, V9(a1106), k6, V9(a1107), k7, V9(a1108), k8, V9(a1109),
+       k9, V9(a1110), k10);
+  R::f(k11, tmp, p, ps, stage_t(t, (double)V9(c10), (double)dt));
+  comb(tmp, uprev, dt, V9(a1201), k1, V9(a1206), k6, V9(a1207), k7, V9(a1208), k8, V9(a1209),
+       k9, V9(a1210), k10, V9(a1211), k11);
+  R::f(k12, tmp, p, ps, stage_t(t, (double)V9(c11), (double)dt));
+  comb(tmp, uprev, dt, V9(a1301), k1, V9(a1306), k6, V9(a1307), k7, V9(a1308), k8, V9(a1309),
+       k9, V9(a1310), k10, V9(a1311), k11, V9(a1312), k12);
+  R::f(k13, tmp, p, ps, stage_t(t, (double)V9(c12), (double)dt));
+  comb(tmp, uprev, dt, V9(a1401), k1, V9(a1406), k6, V9(a1407), k7, V9(a1408), k8, V9(a1409),
+       k9, V9(a1410), k10, V9(a1411), k11, V9(a1412), k12, V9(a1413), k13);
+  R::f(k14, tmp, p, ps, stage_t(t, (double)V9(c13), (double)dt));
+  comb(tmp, uprev, dt, V9(a1501), k1, V9(a1506), k6, V9(a1507), k7, V9(a1508), k8, V9(a1509),
+       k9, V9(a1510), k10, V9(a1511), k11, V9(a1512), k12, V9(a1513), k13, V9(a1514), k14);
+  T k16[N];
+  if constexpr (ADAPT) {
+    T g16[N];
+    comb(g16, uprev, dt, V9(a1601), k1, V9(a1606), k6, V9(a1607), k7, V9(a1608), k8,
+         V9(a1609), k9, V9(a1610), k10, V9(a1611), k11, V9(a1612), k12, V9(a1613), k13);
+    R::f(k15, tmp, p, ps, t + (double)dt);
+    R::f(k16, g16, p, ps, t + (double)dt);
+  } else {
+    R::f(k15, tmp, p, ps, t + (double)dt);
+  }
+  if constexpr (KAHAN) {
+    T acc[N];
+    lsum(acc, V9(b1), k1, V9(b8), k8, V9(b9), k9, V9(b10), k10, V9(b11), k11, V9(b12), k12,
+         V9(b13), k13, V9(b14), k14, V9(b15), k15);
+    kahan_update(u, comp_new, uprev, comp, dt, acc);
+  } else {
+    comb(u, uprev, dt, V9(b1), k1, V9(b8), k8, V9(b9), k9, V9(b10), k10, V9(b11), k11, V9(b12),
+         k12, V9(b13), k13, V9(b14), k14, V9(b15), k15);
+  }
+  if constexpr (ADAPT) {
+    lsum(err, V9(btilde1), k1, V9(btilde8), k8, V9(btilde9), k9, V9(btilde10), k10,
+         V9(btilde11), k11, V9(btilde12), k12, V9(btilde13), k13, V9(btilde14), k14,
+         V9(btilde15), k15, V9(btilde16), k16);
+  }
+  if constexpr (KEEP) {
+    copyv(S.s[0], k1);
+    if (!alias_quirk) {
+      copyv(S.s[1], k8); copyv(S.s[2], k9); copyv(S.s[3], k10); copyv(S.s[4], k11);
+      copyv(S.s[5], k12); copyv(S.s[6], k13); copyv(S.s[7], k14); copyv(S.s[8], k15);
+    }
+  }
+}
+
+// Vern9 extra stages (k11..k20 of the reference) at ts + c*h
+// (gpuvern9.jl:200-317 / 608-745).
+template <class T, class R>
+__device__ __forceinline__ void vern9_extra(const T (&uprev)[R::N], const V9S<T, R::N>& S,
+                                            T (&x)[10][R::N], const T* p, const T* ps,
+                                            double ts, T h) {
+  constexpr int N = R::N;
+  T tmp[N];
+#define SDEH_M9(r)                                                                         \
+  V9X(a##r##01), S.s[0], V9X(a##r##08), S.s[1], V9X(a##r##09), S.s[2], V9X(a##r##10),      \
+      S.s[3], V9X(a##r##11), S.s[4], V9X(a##r##12), S.s[5], V9X(a##r##13), S.s[6],         \
+      V9X(a##r##14), S.s[7], V9X(a##r##15), S.s[8]
+  comb(tmp, uprev, h, SDEH_M9(17));
+  R::f(x[0], tmp, p, ps, stage_t(ts, (double)V9X(c17), (double)h));
+  comb(tmp, uprev, h, SDEH_M9(18), V9X(a1817), x[0]);
+  R::f(x[1], tmp, p, ps, stage_t(ts, (double)V9X(c18), (double)h));
+  comb(tmp, uprev, h, SDEH_M9(19), V9X(a1917), x[0], V9X(a1918), x[1]);
+  R::f(x[2], tmp, p, ps, stage_t(ts, (double)V9X(c19), (double)h));
+  comb(tmp, uprev, h, SDEH_M9(20), V9X(a2017), x[0], V9X(a2018), x[1], V9X(a2019), x[2]);
+  R::f(x[3], tmp, p, ps, stage_t(ts, (double)V9X(c20), (double)h));
+  comb(tmp, uprev, h, SDEH_M9(21), V9X(a2117), x[0], V9X(a2118), x[1], V9X(a2119), x[2],
+       V9X(a2120), x[3]);
+  R::f(x[4], tmp, p, ps, stage_t(ts, (double)V9X(c21), (double)h));
+  comb(tmp, uprev, h, SDEH_M9(22), V9X(a2217), x[0], V9X(a2218), x[1], V9X(a2219), x[2],
+       V9X(a2220), x[3], V9X(a2221), x[4]);
+  R::f(x[5], tmp, p, ps, stage_t(ts, (double)V9X(c22), (double)h));
+  comb(tmp, uprev, h, SDEH_M9(23), V9X(a2317), x[0], V9X(a2318), x[1], V9X(a2319), x[2],
+       V9X(a2320), x[3], V9X(a2321), x[4]);
+  R::f(x[6], tmp, p, ps, stage_t(ts, (double)V9X(c23), (double)h));
+  comb(tmp, uprev, h, SDEH_M9(24), V9X(a2417), x[0], V9X(a2418), x[1], V9X(a2419), x[2],
+       V9X(a2420), x[3], V9X(a2421), x[4]);
+  R::f(x[7], tmp, p, ps, stage_t(ts, (double)V9X(c24), (double)h));
+  comb(tmp, uprev, h, SDEH_M9(25), V9X(a2517), x[0], V9X(a2518), x[1], V9X(a2519), x[2],
+       V9X(a2520), x[3], V9X(a2521), x[4]);
+  R::f(x[8], tmp, p, ps, stage_t(ts, (double)V9X(c25), (double)h));
+  comb(tmp, uprev, h, SDEH_M9(26), V9X(a2617), x[0], V9X(a2618), x[1], V9X(a2619), x[2],
+       V9X(a2620), x[3], V9X(a2621), x[4]);
+  R::f(x[9], tmp, p, ps, stage_t(ts, (double)V9X(c26), (double)h));
+#undef SDEH_M9
+}
+
+// Vern9 dense output (bθs at verner_tableaus.jl:1362-1398).
+template <class T, int N>
+__device__ __forceinline__ void vern9_dense(T (&out)[N], const T (&uprev)[N], T h, T th,
+                                            const V9S<T, N>& S, const T (&x)[10][N]) {
+  const T r1[9] = {V9I(r011), V9I(r012), V9I(r013), V9I(r014), V9I(r015),
+                   V9I(r016), V9I(r017), V9I(r018), V9I(r019)};
+#define SDEH_R9(j)                                                                      \
+  const T r##j[8] = {V9I(r##j##2), V9I(r##j##3), V9I(r##j##4), V9I(r##j##5),           \
+                     V9I(r##j##6), V9I(r##j##7), V9I(r##j##8), V9I(r##j##9)};
+  SDEH_R9(08) SDEH_R9(09) SDEH_R9(10) SDEH_R9(11) SDEH_R9(12) SDEH_R9(13) SDEH_R9(14)
+  SDEH_R9(15) SDEH_R9(17) SDEH_R9(18) SDEH_R9(19) SDEH_R9(20) SDEH_R9(21) SDEH_R9(22)
+  SDEH_R9(23) SDEH_R9(24) SDEH_R9(25) SDEH_R9(26)
+#undef SDEH_R9
+  comb(out, uprev, h, poly1(th, r1), S.s[0], poly2(th, r08), S.s[1], poly2(th, r09), S.s[2],
+       poly2(th, r10), S.s[3], poly2(th, r11), S.s[4], poly2(th, r12), S.s[5], poly2(th, r13),
+       S.s[6], poly2(th, r14), S.s[7], poly2(th, r15), S.s[8], poly2(th, r17), x[0],
+       poly2(th, r18), x[1], poly2(th, r19), x[2], poly2(th, r20), x[3], poly2(th, r21), x[4],
+       poly2(th, r22), x[5], poly2(th, r23), x[6], poly2(th, r24), x[7], poly2(th, r25), x[8],
+       poly2(th, r26), x[9]);
+}
+#undef V9
+#undef V9X
+#undef V9I
+
+// ---------------------------------------------------------------------------
+// Output helpers (structure-of-arrays, component major).
+// ---------------------------------------------------------------------------
+template <class T, int N>
+__device__ __forceinline__ void store_state(void* base, int64_t row, int64_t n, int64_t traj,
+                                            const T (&u)[N]) {
+  T* o = (T*)base;
+#pragma unroll
+  for (int c = 0; c < N; ++c) o[(row * N + c) * n + traj] = u[c];
+}
+
+template <class T>
+__device__ __forceinline__ T nan_of() {
+  return (T)__builtin_nan("");
+}
+
+// ---------------------------------------------------------------------------
+// Wave-level work queue for the persistent schedule.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned lane_rank(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+// Claim queue slots for the lanes flagged in `need` (called in uniform control
+// flow by the whole wave).  Returns this lane's slot or -1.  `base`/`lim` is
+// the wave's current claimed range; `drained` becomes true once the queue is
+// exhausted.
+__device__ __forceinline__ int64_t wave_claim(const sdeh_kargs& a, bool need_me, int64_t& base,
+                                              int64_t& lim, bool& drained) {
+  int64_t mine = -1;
+  unsigned long long need = __ballot(need_me);
+  while (need != 0ull && !drained) {
+    if (base >= lim) {
+      unsigned long long old = 0;
+      if (__lane_id() == 0) old = atomicAdd(a.queue, (unsigned long long)a.chunk);
+      old = (unsigned long long)__shfl((long long)old, 0);
+      base = (int64_t)old;
+      lim = base + a.chunk < a.n ? base + a.chunk : a.n;
+      if (base >= a.n) {
+        drained = true;
+        break;
+      }
+    }
+    const int64_t avail = lim - base;
+    const int cnt = __popcll(need);
+    const bool in_need = ((need >> __lane_id()) & 1ull) != 0ull;
+    const int64_t rank = (int64_t)lane_rank(need);
+    if (in_need && rank < avail) mine = base + rank;
+    const int64_t taken = (int64_t)cnt < avail ? (int64_t)cnt : avail;
+    base += taken;
+    need = __ballot(in_need && mine < 0);
+  }
+  return mine;
+}
+
+// ---------------------------------------------------------------------------
+// Adaptive solvers: ATsit5, AVern7, AVern9.
+// ---------------------------------------------------------------------------
+template <class T, class R, int ALG, int SAVE, bool KAHAN>
+__device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
+  constexpr int N = R::N;
+  constexpr int P = R::P > 0 ? R::P : 1;
+  constexpr double DTMIN = (ALG == ALG_AVERN9) ? (double)1.0e-7f : 1.0e-14;  // gpuvern9.jl:466
+  constexpr double SNAP = DTMIN;                                              // gpuvern9.jl:591
+  // build_adaptive_controller_cache(T), src/SimpleDiffEq.jl:67-77
+  const T beta1 = T(7.0 / 50), beta2 = T(2.0 / 25), qmax = T(10.0), qmin = T(1.0 / 5),
+          gamma = T(9.0 / 10), qoldinit = T(1.0e-4);
+  const T inv_qmax = T(1) / qmax, inv_qmin = T(1) / qmin;
+  const T qoldinit_pow = Ops<T>::pw(qoldinit, beta2);
+  const T abstol = (T)a.abstol, reltol = (T)a.reltol;
+  const T* ps = (const T*)a.ps;
+  const int64_t n = a.n;
+
+  // Lane state.
+  T uprev[N], comp[N], fsal[N], pl[P];
+  double t = 0.0, tf = 0.0;
+  T dt = T(0), qold = T(0), qold_pow = T(0);
+  int32_t nacc = 0, nrej = 0, cur = 0, nsaved = 0;
+  int64_t attempts = 0, traj = -1;
+  bool active = false;
+
+  int64_t qbase = 0, qlim = 0;
+  bool drained = !a.persistent;
+  if (!a.persistent) {
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot < n) {
+      traj = a.order ? a.order[slot] : slot;
+      active = true;
+    }
+  }
+  bool fresh = active;
+
+  for (;;) {
+    if (a.persistent) {
+      const int64_t slot = wave_claim(a, !active, qbase, qlim, drained);
+      if (slot >= 0) {
+        traj = a.order ? a.order[slot] : slot;
+        active = true;
+        fresh = true;
+      }
+    }
+    if (__ballot(active) == 0ull) break;
+    if (!active) continue;
+
+    if (fresh) {  // start trajectory `traj` (gpuatsit5.jl:221-241)
+      fresh = false;
+#pragma unroll
+      for (int c = 0; c < N; ++c) {
+        uprev[c] = ((const T*)a.u0)[c * n + traj];
+        comp[c] = T(0);
+      }
+#pragma unroll
+      for (int c = 0; c < R::P; ++c) pl[c] = ((const T*)a.p)[c * n + traj];
+      t = a.tspan ? a.tspan[traj] : a.t0;
+      tf = a.tspan ? a.tspan[n + traj] : a.tf;
+      dt = (T)(a.dt0 ? a.dt0[traj] : a.dt);
+      qold = qoldinit;
+      qold_pow = qoldinit_pow;
+      nacc = nrej = 0;
+      attempts = 0;
+      cur = 0;
+      nsaved = 0;
+      if constexpr (SAVE == SAVE_AT) {
+        if (a.n_saveat > 0 && t == a.saveat[0]) {
+          store_state<T, N>(a.u_saveat, 0, n, traj, uprev);
+          cur = 1;
+        }
+      }
+      if constexpr (SAVE == SAVE_STEPS) {
+        if (a.max_saves > 0) {
+          store_state<T, N>(a.u_steps, 0, n, traj, uprev);
+          a.t_steps[traj] = t;
+        }
+        nsaved = 1;
+      }
+      if constexpr (ALG == ALG_ATSIT5) R::f(fsal, uprev, pl, ps, t);  // FSAL seed, :241
+    }
+
+    int ret = -1;  // -1: running
+    if (!(t < tf)) {
+      ret = RET_DEFAULT;
+    } else if ((double)dt < DTMIN) {  // :256
+      ret = RET_DTMIN;
+    } else if (attempts >= a.maxiters) {
+      ret = RET_MAXITERS;
+    } else {
+      ++attempts;
+      T u[N], err[N], comp_new[N];
+      // ---- one attempt ----
+      if constexpr (ALG == ALG_ATSIT5) {
+        T k2[N], k3[N], k4[N], k5[N], k6[N], k7[N];
+        tsit5_stages<T, R, KAHAN>(uprev, fsal, k2, k3, k4, k5, k6, k7, u, comp, comp_new, pl,
+                                  ps, t, dt);
+        // dt * (btilde1*k1 + ... + btilde7*k7), gpuatsit5.jl:272-275
+        lsum(err, T(sdeh_tab::tsit5::btilde1), fsal, T(sdeh_tab::tsit5::btilde2), k2,
+             T(sdeh_tab::tsit5::btilde3), k3, T(sdeh_tab::tsit5::btilde4), k4,
+             T(sdeh_tab::tsit5::btilde5), k5, T(sdeh_tab::tsit5::btilde6), k6,
+             T(sdeh_tab::tsit5::btilde7), k7);
+#pragma unroll
+        for (int c = 0; c < N; ++c) err[c] = dt * err[c];
+        const T EEst = err_norm(err, uprev, u, abstol, reltol);
+        // ---- PI controller, gpuatsit5.jl:279-299 ----
+        if (EEst != EEst) {
+          ret = RET_NAN;
+        } else {
+          const T q11 = Ops<T>::pw(EEst, beta1);
+          if (EEst > T(1)) {
+            dt = Ops<T>::div(dt, sdeh_math::jl_min(inv_qmin, Ops<T>::div(q11, gamma)));
+            ++nrej;
+          } else {
+            T q = (EEst == T(0)) ? inv_qmax : Ops<T>::div(q11, qold_pow);
+            q = sdeh_math::fast_max(inv_qmax, sdeh_math::fast_min(inv_qmin, Ops<T>::div(q, gamma)));
+            qold = sdeh_math::jl_max(EEst, qoldinit);
+            qold_pow = Ops<T>::pw(qold, beta2);
+            const T dtold = dt;
+            dt = Ops<T>::div(dt, q);
+            const double rem = tf - t - (double)dtold;
+            dt = (T)sdeh_math::jl_min((double)fabs(dt), fabs(rem));
+            const double told = t;
+            t = (rem < SNAP) ? tf : t + (double)dtold;
+            ++nacc;
+            if constexpr (SAVE == SAVE_AT) {  // :305-316
+              while (cur < a.n_saveat && a.saveat[cur] <= t) {
+                const T th = (T)((a.saveat[cur] - told) / (double)dtold);
+                T o[N];
+                tsit5_dense(o, uprev, dtold, th, fsal, k2, k3, k4, k5, k6, k7);
+                store_state<T, N>(a.u_saveat, cur, n, traj, o);
+                ++cur;
+              }
+            }
+            if constexpr (SAVE == SAVE_STEPS) {
+              if (nsaved < a.max_saves) {
+                store_state<T, N>(a.u_steps, nsaved, n, traj, u);
+                a.t_steps[(int64_t)nsaved * n + traj] = t;
+              }
+              ++nsaved;
+            }
+            copyv(uprev, u);
+            copyv(fsal, k7);
+            if constexpr (KAHAN) copyv(comp, comp_new);
+            if (!(t < tf)) ret = RET_DEFAULT;
+          }
+        }
+      } else {
+        // Verner: k1 is evaluated at the start of every attempt (gpuvern7.jl:360,
+        // gpuvern9.jl:468).
+        constexpr bool V9 = (ALG == ALG_AVERN9);
+        constexpr int NS = V9 ? 9 : 7;
+        (void)NS;
+        V7K<T, N> K7;
+        V9S<T, N> S9;
+        if constexpr (V9) {
+          vern9_stages<T, R, true, KAHAN, SAVE == SAVE_AT>(uprev, S9, u, err, comp, comp_new, pl,
+                                                            ps, t, dt, false);
+        } else {
+          vern7_stages<T, R, true, KAHAN>(uprev, K7, u, err, comp, comp_new, pl, ps, t, dt);
+        }
+#pragma unroll
+        for (int c = 0; c < N; ++c) err[c] = dt * err[c];
+        const T EEst = err_norm(err, uprev, u, abstol, reltol);
+        if (EEst != EEst) {
+          ret = RET_NAN;
+        } else {
+          const T q11 = Ops<T>::pw(EEst, beta1);
+          if (EEst > T(1)) {
+            dt = Ops<T>::div(dt, sdeh_math::jl_min(inv_qmin, Ops<T>::div(q11, gamma)));
+            ++nrej;
+          } else {
+            T q = (EEst == T(0)) ? inv_qmax : Ops<T>::div(q11, qold_pow);
+            q = sdeh_math::fast_max(inv_qmax, sdeh_math::fast_min(inv_qmin, Ops<T>::div(q, gamma)));
+            qold = sdeh_math::jl_max(EEst, qoldinit);
+            qold_pow = Ops<T>::pw(qold, beta2);
+            const T dtold = dt;
+            dt = Ops<T>::div(dt, q);
+            const double rem = tf - t - (double)dtold;
+            dt = (T)sdeh_math::jl_min((double)fabs(dt), fabs(rem));
+            const double told = t;
+            t = (rem < SNAP) ? tf : t + (double)dtold;
+            ++nacc;
+            if constexpr (SAVE == SAVE_AT) {
+              if (cur < a.n_saveat && a.saveat[cur] <= t) {
+                // Extra stages depend only on the step, not on θ: computed once.
+                if constexpr (V9) {
+                  T x[10][N];
+                  vern9_extra<T, R>(uprev, S9, x, pl, ps, told, dtold);  // told + c*dtold
+                  while (cur < a.n_saveat && a.saveat[cur] <= t) {
+                    const T th = (T)((a.saveat[cur] - told) / (double)dtold);
+                    T o[N];
+                    vern9_dense(o, uprev, dtold, th, S9, x);
+                    store_state<T, N>(a.u_saveat, cur, n, traj, o);
+                    ++cur;
+                  }
+                } else {
+                  T x[6][N];
+                  // Quirk (gpuvern7.jl:449): stage times from the post-step t.
+                  vern7_extra<T, R>(uprev, K7, x, pl, ps, a.quirks ? t : told, dtold);
+                  while (cur < a.n_saveat && a.saveat[cur] <= t) {
+                    const T th = (T)((a.saveat[cur] - told) / (double)dtold);
+                    T o[N];
+                    vern7_dense(o, uprev, dtold, th, K7, x);
+                    store_state<T, N>(a.u_saveat, cur, n, traj, o);
+                    ++cur;
+                  }
+                }
+              }
+            }
+            if constexpr (SAVE == SAVE_STEPS) {
+              if (nsaved < a.max_saves) {
+                store_state<T, N>(a.u_steps, nsaved, n, traj, u);
+                a.t_steps[(int64_t)nsaved * n + traj] = t;
+              }
+              ++nsaved;
+            }
+            copyv(uprev, u);
+            if constexpr (KAHAN) copyv(comp, comp_new);
+            if (!(t < tf)) ret = RET_DEFAULT;
+          }
+        }
+      }
+    }
+
+    if (ret >= 0) {  // trajectory ends: write outputs (gpuatsit5.jl:322-329)
+      if (SAVE == SAVE_STEPS && ret == RET_DEFAULT && nsaved > a.max_saves)
+        ret = RET_SAVE_OVERFLOW;
+      store_state<T, N>(a.u_end, 0, n, traj, uprev);
+      a.t_end[traj] = t;
+      a.retcode[traj] = ret;
+      if (a.naccept) a.naccept[traj] = nacc;
+      if (a.nreject) a.nreject[traj] = nrej;
+      if constexpr (SAVE == SAVE_STEPS)
+        a.n_steps[traj] = nsaved < a.max_saves ? nsaved : a.max_saves;
+      active = false;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-step solvers: Tsit5, Vern7, Vern9 over t0:dt:tf.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double grid_elem(const sdeh_kargs& a, int64_t i) {
+  if (a.g_rational) return (double)(a.g_start_n + i * a.g_step_n) / (double)a.g_den;
+  const double di = (double)i;
+  const double hi = di * a.g_step;
+  const double lo = fma(di, a.g_step, -hi);
+  const double s = a.g_start + hi;
+  const double bb = s - a.g_start;
+  const double e = (a.g_start - (s - bb)) + (hi - bb);
+  return s + (e + lo);
+}
+
+template <class T, class R, int ALG, int SAVE>
+__device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
+  constexpr int N = R::N;
+  constexpr int P = R::P > 0 ? R::P : 1;
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= a.n) return;
+  const int64_t n = a.n;
+  const int64_t traj = a.order ? a.order[slot] : slot;
+  const T* ps = (const T*)a.ps;
+  T u[N], uprev[N], pl[P], zero[N], dummy[N], err[N];
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    u[c] = ((const T*)a.u0)[c * n + traj];
+    zero[c] = T(0);
+  }
+#pragma unroll
+  for (int c = 0; c < R::P; ++c) pl[c] = ((const T*)a.p)[c * n + traj];
+  const T dt = (T)a.dt;
+  double t = a.t0;
+  int32_t cur = 0, nsaved = 0;
+  if constexpr (SAVE == SAVE_AT) {
+    if (a.n_saveat > 0 && t == a.saveat[0]) {
+      store_state<T, N>(a.u_saveat, 0, n, traj, u);
+      cur = 1;
+    }
+  }
+  if constexpr (SAVE == SAVE_STEPS) {
+    if (a.max_saves > 0) {
+      store_state<T, N>(a.u_steps, 0, n, traj, u);
+      a.t_steps[traj] = t;
+    }
+    nsaved = 1;
+  }
+  T k7[N];
+  if constexpr (ALG == ALG_TSIT5) R::f(k7, u, pl, ps, t);  // gpuatsit5.jl:86
+  for (int64_t i = 1; i < a.g_len; ++i) {
+    copyv(uprev, u);
+    t = grid_elem(a, i - 1);  // gpuatsit5.jl:98
+    if constexpr (ALG == ALG_TSIT5) {
+      T k1[N], k2[N], k3[N], k4[N], k5[N], k6[N];
+      copyv(k1, k7);
+      tsit5_stages<T, R, false>(uprev, k1, k2, k3, k4, k5, k6, k7, u, zero, dummy, pl, ps, t, dt);
+      t += (double)dt;
+      if constexpr (SAVE == SAVE_AT) {  // :116-127
+        while (cur < a.n_saveat && a.saveat[cur] <= t) {
+          const T th = (T)((a.saveat[cur] - (t - (double)dt)) / (double)dt);
+          T o[N];
+          tsit5_dense(o, uprev, dt, th, k1, k2, k3, k4, k5, k6, k7);
+          store_state<T, N>(a.u_saveat, cur, n, traj, o);
+          ++cur;
+        }
+      }
+    } else if constexpr (ALG == ALG_VERN7) {
+      V7K<T, N> K;
+      vern7_stages<T, R, false, false>(uprev, K, u, err, zero, dummy, pl, ps, t, dt);
+      const double told = t;
+      t += (double)dt;  // gpuvern7.jl:138
+      if constexpr (SAVE == SAVE_AT) {
+        if (cur < a.n_saveat && a.saveat[cur] <= t) {
+          T x[6][N];
+          vern7_extra<T, R>(uprev, K, x, pl, ps, a.quirks ? t : told, dt);  // quirk :160
+          while (cur < a.n_saveat && a.saveat[cur] <= t) {
+            const T th = (T)((a.saveat[cur] - (t - (double)dt)) / (double)dt);
+            T o[N];
+            vern7_dense(o, uprev, dt, th, K, x);
+            store_state<T, N>(a.u_saveat, cur, n, traj, o);
+            ++cur;
+          }
+        }
+      }
+    } else {
+      V9S<T, N> S;
+      vern9_stages<T, R, false, false, SAVE == SAVE_AT>(uprev, S, u, err, zero, dummy, pl, ps, t,
+                                                         dt, a.quirks != 0);
+      const double told = t;
+      t += (double)dt;  // gpuvern9.jl:185
+      if constexpr (SAVE == SAVE_AT) {
+        if (cur < a.n_saveat && a.saveat[cur] <= t) {
+          T x[10][N];
+          vern9_extra<T, R>(uprev, S, x, pl, ps, a.quirks ? t : told, dt);  // quirk :222
+          while (cur < a.n_saveat && a.saveat[cur] <= t) {
+            const T th = (T)((a.saveat[cur] - (t - (double)dt)) / (double)dt);
+            T o[N];
+            vern9_dense(o, uprev, dt, th, S, x);
+            store_state<T, N>(a.u_saveat, cur, n, traj, o);
+            ++cur;
+          }
+        }
+      }
+    }
+    if constexpr (SAVE == SAVE_STEPS) {
+      if (nsaved < a.max_saves) {
+        store_state<T, N>(a.u_steps, nsaved, n, traj, u);
+        a.t_steps[(int64_t)nsaved * n + traj] = t;
+      }
+      ++nsaved;
+    }
+  }
+  store_state<T, N>(a.u_end, 0, n, traj, u);
+  a.t_end[traj] = t;
+  int ret = RET_DEFAULT;
+  if (SAVE == SAVE_STEPS && nsaved > a.max_saves) ret = RET_SAVE_OVERFLOW;
+  a.retcode[traj] = ret;
+  if (a.naccept) a.naccept[traj] = (int32_t)(a.g_len > 0 ? a.g_len - 1 : 0);
+  if (a.nreject) a.nreject[traj] = 0;
+  if constexpr (SAVE == SAVE_STEPS) a.n_steps[traj] = nsaved < a.max_saves ? nsaved : a.max_saves;
+}
+
+}  // namespace sdeh
+
+// Kernel entry templates.  Instantiated with extern "C" names by the
+// ahead-of-time translation unit and by the hipRTC wrapper.
+template <class T, class R, int ALG, int SAVE, bool KAHAN>
+__global__ void __launch_bounds__(256) sdeh_adaptive_kernel(sdeh_kargs a) {
+  sdeh::adaptive_body<T, R, ALG, SAVE, KAHAN>(a);
+}
+
+template <class T, class R, int ALG, int SAVE>
+__global__ void __launch_bounds__(256) sdeh_fixed_kernel(sdeh_kargs a) {
+  sdeh::fixed_body<T, R, ALG, SAVE>(a);
+}

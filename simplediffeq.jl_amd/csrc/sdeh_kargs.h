@@ -1,0 +1,48 @@
+// Kernel argument block shared by the host runtime and the device integrators
+// (ahead-of-time and hipRTC-compiled alike).  Plain data only; passed by value
+// as the single kernel argument.
+#pragma once
+
+#if !defined(__HIPCC_RTC__)
+#include <stdint.h>
+#else
+// hipRTC declares the fixed-width integer types in __hip_internal only.
+using __hip_internal::int32_t;
+using __hip_internal::int64_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
+#endif
+
+struct sdeh_kargs {
+  const void* u0;          // [N][n] state type
+  const void* p;           // [P][n] state type
+  const void* ps;          // shared RHS block, state type
+  const double* tspan;     // [2][n] or null (then t0, tf)
+  const double* dt0;       // [n] or null (then dt)
+  const int64_t* order;    // [n] lane hand-out order or null (identity)
+  void* u_end;             // [N][n]
+  double* t_end;           // [n]
+  int32_t* retcode;        // [n]
+  int32_t* naccept;        // [n] or null
+  int32_t* nreject;        // [n] or null
+  const double* saveat;    // [n_saveat] device copy
+  void* u_saveat;          // [n_saveat][N][n]
+  void* u_steps;           // [max_saves][N][n]
+  double* t_steps;         // [max_saves][n]
+  int32_t* n_steps;        // [n]
+  unsigned long long* queue;  // persistent schedule: next unclaimed slot
+  int64_t n;
+  int64_t n_saveat;
+  int64_t maxiters;
+  double t0, tf, dt, abstol, reltol;
+  // Fixed-step time grid t0:dt:tf (Julia range semantics), element i:
+  //   rational: (g_start_n + i * g_step_n) / g_den, else g_start + i * g_step.
+  int64_t g_len, g_start_n, g_step_n, g_den;
+  double g_start, g_step;
+  int32_t g_rational;
+  int32_t max_saves;
+  int32_t quirks;
+  int32_t persistent;      // 1: lanes refill from *queue in chunks of `chunk`
+  int32_t chunk;
+  int32_t pad_;
+};

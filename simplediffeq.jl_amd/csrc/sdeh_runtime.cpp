@@ -1,0 +1,991 @@
+// Host runtime of libsimplediffeq_hip: the C ABI declared in
+// include/simplediffeq_hip.h.
+//
+// One sdeh_solve() is the whole ensemble loop that SciMLBase's EnsembleThreads
+// would run over DiffEqBase.solve(prob_i, alg; kwargs...) (the six GPUSimple*
+// methods of src/tsit5/gpuatsit5.jl and src/verner/gpuvern{7,9}.jl): it
+// validates the request, stages host buffers through device memory when asked,
+// builds the fixed-step grid with Julia's range rules, picks the integrator
+// kernel (compiled ahead of time for the built-in right-hand sides, or by
+// hipRTC for a registered RHS source), picks the launch shape (tuned or
+// default), arms the persistent queue, launches, and gathers counters.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/simplediffeq_hip.h"
+#include "sdeh_kargs.h"
+#include "sdeh_range.h"
+
+#define SDEH_HIDDEN __attribute__((visibility("hidden")))
+
+extern "C" {
+SDEH_HIDDEN const void* sdeh_aot_Lorenz(int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_GLV16(int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_ExpDecay3(int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_Affine1(int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_Linear2(int, int, int, int);
+SDEH_HIDDEN hipError_t sdeh_util_fill_f64(double*, double, int64_t, hipStream_t);
+SDEH_HIDDEN hipError_t sdeh_util_fill_f32(float*, float, int64_t, hipStream_t);
+SDEH_HIDDEN hipError_t sdeh_util_zero_u64(unsigned long long*, int, hipStream_t);
+SDEH_HIDDEN hipError_t sdeh_util_count(const int32_t*, const int32_t*, const int32_t*, int64_t,
+                                       unsigned long long*, hipStream_t);
+SDEH_HIDDEN hipError_t sdeh_util_check_order(const int64_t*, int64_t, int32_t*, unsigned long long*,
+                                             hipStream_t);
+SDEH_HIDDEN int sdeh_mlp_forward_impl(const float*, const float*, float*, int64_t, hipStream_t);
+SDEH_HIDDEN int sdeh_mlp_loss_grad_impl(const float*, const float*, const float*, int64_t, float*,
+                                        float*, float*, hipStream_t);
+SDEH_HIDDEN int sdeh_mlp_adam_impl(float*, const float*, float*, float*, float, float, float, float,
+                                   int32_t, hipStream_t);
+}
+
+#include "sdeh_embedded.inc"  // sdeh_embedded_names / sdeh_embedded_bodies / sdeh_embedded_count
+
+namespace {
+
+enum { SAVE_END = 0, SAVE_AT = 1, SAVE_STEPS = 2 };
+constexpr int64_t kUserBase = 1000;
+constexpr int64_t kDefaultMaxiters = 10000000;
+
+struct Builtin {
+  int64_t id;
+  const char* name;
+  int32_t n, p;
+  int64_t s;
+  const void* (*lookup)(int, int, int, int);
+};
+
+const Builtin kBuiltins[] = {
+    {SDEH_RHS_LORENZ, "lorenz", 3, 3, 0, sdeh_aot_Lorenz},
+    {SDEH_RHS_GLV16, "glv16", 16, 16, 256, sdeh_aot_GLV16},
+    {SDEH_RHS_EXPDECAY3, "expdecay3", 3, 1, 0, sdeh_aot_ExpDecay3},
+    {SDEH_RHS_AFFINE1, "affine1", 1, 2, 0, sdeh_aot_Affine1},
+    {SDEH_RHS_LINEAR2, "linear2", 2, 4, 0, sdeh_aot_Linear2},
+};
+
+const Builtin* find_builtin(int64_t id) {
+  for (const auto& b : kBuiltins)
+    if (b.id == id) return &b;
+  return nullptr;
+}
+
+struct UserRhs {
+  std::string src, fn;
+  int32_t n = 0, p = 0;
+  int64_t s = 0;
+};
+
+struct KernelRef {
+  const void* aot = nullptr;
+  hipFunction_t fn = nullptr;
+};
+
+struct Shape {
+  int32_t schedule = SDEH_SCHED_ONE_LAUNCH, block = 256, grid = 0, chunk = 64;
+};
+
+using TuneKey = std::tuple<int64_t, int, int, int, int>;  // rhs, alg, dtype, save, kahan
+
+struct GraphRec {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void* mem = nullptr;  // queue counter + saveat copy owned by the graph
+};
+
+bool is_adaptive(int alg) {
+  return alg == SDEH_ALG_ATSIT5 || alg == SDEH_ALG_AVERN7 || alg == SDEH_ALG_AVERN9;
+}
+
+uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ull) {
+  for (unsigned char ch : s) {
+    h ^= ch;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+}  // namespace
+
+struct sdeh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  sdeh_stats stats{};
+  int cus = 0;
+  char* arena = nullptr;
+  size_t arena_cap = 0;
+  unsigned long long* counters = nullptr;  // [0] queue, [1..4] count_kernel outputs
+  std::vector<UserRhs> user;
+  std::map<std::string, std::pair<hipModule_t, hipFunction_t>> rtc;
+  std::map<TuneKey, Shape> tuned;
+  std::map<int64_t, GraphRec> graphs;
+  int64_t next_graph = 1;
+};
+
+namespace {
+
+int fail(sdeh_ctx* c, int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail((c), SDEH_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));         \
+  } while (0)
+
+int rhs_shape(sdeh_ctx* c, int64_t rhs, int32_t* n, int32_t* p, int64_t* s) {
+  if (const Builtin* b = find_builtin(rhs)) {
+    *n = b->n;
+    *p = b->p;
+    *s = b->s;
+    return SDEH_OK;
+  }
+  const int64_t k = rhs - kUserBase;
+  if (k >= 0 && k < (int64_t)c->user.size()) {
+    *n = c->user[k].n;
+    *p = c->user[k].p;
+    *s = c->user[k].s;
+    return SDEH_OK;
+  }
+  return fail(c, SDEH_E_INVALID, "unknown rhs %lld", (long long)rhs);
+}
+
+const std::vector<std::string>& rtc_options() {
+  static const std::vector<std::string> o = {
+      "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-mllvm",
+      "-disable-machine-licm", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  return o;
+}
+
+// Compile (or fetch) the integrator kernel for a registered RHS.
+int user_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kahan,
+                KernelRef* out) {
+  const UserRhs& u = c->user[rhs - kUserBase];
+  char key[128];
+  snprintf(key, sizeof key, "%lld/%d/%d/%d/%d", (long long)rhs, alg, dtype, save, kahan);
+  auto it = c->rtc.find(key);
+  if (it != c->rtc.end()) {
+    out->fn = it->second.second;
+    return SDEH_OK;
+  }
+  const char* T = dtype == SDEH_F64 ? "double" : "float";
+  std::string src = "#include \"sdeh_integrators.h\"\n";
+  src += u.src;
+  char wrap[2048];
+  snprintf(wrap, sizeof wrap,
+           "\nstruct sdeh_user_rhs {\n"
+           "  static constexpr int N = %d, P = %d, S = %lld;\n"
+           "  template <class T>\n"
+           "  __device__ static inline void f(T* du, const T* u, const T* p,\n"
+           "                                  const T* __restrict__ ps, double t) {\n"
+           "    %s<T>(du, u, p, ps, t);\n"
+           "  }\n"
+           "};\n",
+           u.n, u.p, (long long)u.s, u.fn.c_str());
+  src += wrap;
+  if (is_adaptive(alg))
+    snprintf(wrap, sizeof wrap,
+             "extern \"C\" __global__ void __launch_bounds__(256) sdeh_user_kernel(sdeh_kargs a) "
+             "{ sdeh::adaptive_body<%s, sdeh_user_rhs, %d, %d, %s>(a); }\n",
+             T, alg, save, kahan ? "true" : "false");
+  else
+    snprintf(wrap, sizeof wrap,
+             "extern \"C\" __global__ void __launch_bounds__(256) sdeh_user_kernel(sdeh_kargs a) "
+             "{ sdeh::fixed_body<%s, sdeh_user_rhs, %d, %d>(a); }\n",
+             T, alg, save);
+  src += wrap;
+
+  std::string opts_key;
+  for (const auto& o : rtc_options()) opts_key += o + " ";
+  std::vector<char> code;
+  const char* cache_dir = getenv("SDEH_CACHE_DIR");
+  std::string cache_path;
+  if (cache_dir && *cache_dir) {
+    char hex[32];
+    uint64_t h = fnv1a(src, fnv1a(opts_key));
+    for (int i = 0; i < sdeh_embedded_count; ++i) h = fnv1a(sdeh_embedded_bodies[i], h);
+    snprintf(hex, sizeof hex, "%016llx", (unsigned long long)h);
+    cache_path = std::string(cache_dir) + "/sdeh_" + hex + ".co";
+    if (FILE* f = fopen(cache_path.c_str(), "rb")) {
+      fseek(f, 0, SEEK_END);
+      const long sz = ftell(f);
+      fseek(f, 0, SEEK_SET);
+      if (sz > 0) {
+        code.resize((size_t)sz);
+        if (fread(code.data(), 1, (size_t)sz, f) != (size_t)sz) code.clear();
+      }
+      fclose(f);
+    }
+  }
+  if (code.empty()) {
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "sdeh_user.hip", sdeh_embedded_count,
+                            sdeh_embedded_bodies, sdeh_embedded_names) != HIPRTC_SUCCESS)
+      return fail(c, SDEH_E_COMPILE, "hiprtcCreateProgram failed");
+    std::vector<const char*> o;
+    for (const auto& s : rtc_options()) o.push_back(s.c_str());
+    const hiprtcResult r = hiprtcCompileProgram(prog, (int)o.size(), o.data());
+    if (r != HIPRTC_SUCCESS) {
+      size_t ls = 0;
+      hiprtcGetProgramLogSize(prog, &ls);
+      std::string log(ls, '\0');
+      if (ls) hiprtcGetProgramLog(prog, &log[0]);
+      hiprtcDestroyProgram(&prog);
+      if (log.size() > 900) log.resize(900);
+      return fail(c, SDEH_E_COMPILE, "hipRTC: %s\n%s", hiprtcGetErrorString(r), log.c_str());
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    if (!cache_path.empty()) {
+      const std::string tmp = cache_path + ".tmp";
+      if (FILE* f = fopen(tmp.c_str(), "wb")) {
+        const bool ok = fwrite(code.data(), 1, code.size(), f) == code.size();
+        fclose(f);
+        if (ok) rename(tmp.c_str(), cache_path.c_str());
+      }
+    }
+  }
+  hipModule_t mod;
+  HIPCHK(c, hipModuleLoadData(&mod, code.data()));
+  hipFunction_t fn;
+  HIPCHK(c, hipModuleGetFunction(&fn, mod, "sdeh_user_kernel"));
+  c->rtc[key] = {mod, fn};
+  out->fn = fn;
+  return SDEH_OK;
+}
+
+int resolve_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kahan,
+                   KernelRef* out) {
+  if (const Builtin* b = find_builtin(rhs)) {
+    out->aot = b->lookup(dtype, alg, save, kahan);
+    if (!out->aot)
+      return fail(c, SDEH_E_UNSUPPORTED,
+                  "no built-in kernel for rhs %s, alg %d, dtype %d, save mode %d, precision %d",
+                  b->name, alg, dtype, save, kahan);
+    return SDEH_OK;
+  }
+  return user_kernel(c, rhs, alg, dtype, save, kahan, out);
+}
+
+hipError_t launch(const KernelRef& k, int grid, int block, sdeh_kargs* a, hipStream_t s) {
+  void* args[] = {a};
+  if (k.aot) return hipLaunchKernel(k.aot, dim3(grid), dim3(block), args, 0, s);
+  return hipModuleLaunchKernel(k.fn, grid, 1, 1, block, 1, 1, 0, s, args, nullptr);
+}
+
+int blocks_per_cu(sdeh_ctx* c, const KernelRef& k, int block) {
+  int nb = 0;
+  hipError_t e = k.aot ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k.aot, block, 0)
+                       : hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k.fn, block, 0);
+  if (e != hipSuccess || nb < 1) nb = 1;
+  return nb;
+}
+
+// Device-pointer check: the range [p, p + bytes) must lie in one device
+// allocation.  Refusing here is what keeps a bad caller buffer from becoming a
+// GPU memory fault.
+int check_dev(sdeh_ctx* c, const void* p, size_t bytes, const char* what) {
+  if (bytes == 0) return SDEH_OK;
+  if (!p) return fail(c, SDEH_E_INVALID, "%s is NULL", what);
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess || !base) {
+    (void)hipGetLastError();
+    return fail(c, SDEH_E_INVALID, "%s is not a device allocation", what);
+  }
+  const char* b = (const char*)base;
+  const char* q = (const char*)p;
+  if (q < b || q + bytes > b + size)
+    return fail(c, SDEH_E_INVALID, "%s: %zu bytes do not fit its allocation", what, bytes);
+  return SDEH_OK;
+}
+
+struct Arena {
+  char* base;
+  size_t off = 0;
+  void* take(size_t bytes) {
+    if (bytes == 0) return nullptr;
+    void* p = base + off;
+    off += (bytes + 255) & ~size_t(255);
+    return p;
+  }
+};
+
+size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+int ensure_arena(sdeh_ctx* c, size_t need) {
+  if (need <= c->arena_cap) return SDEH_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->arena) HIPCHK(c, hipFree(c->arena));
+  c->arena = nullptr;
+  c->arena_cap = 0;
+  size_t cap = need + need / 4 + 4096;
+  if (hipMalloc((void**)&c->arena, cap) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, SDEH_E_NOMEM, "device allocation of %zu bytes failed", cap);
+  }
+  c->arena_cap = cap;
+  return SDEH_OK;
+}
+
+struct Plan {
+  int32_t n_state = 0, n_param = 0;
+  int64_t n_shared = 0;
+  int save = SAVE_END;
+  int kahan = 0;
+  size_t es = 8;  // element size of the state type
+  KernelRef k;
+  Shape shape;
+  sdeh_kargs a{};
+};
+
+// Validate the descriptor and fill everything except the device pointers.
+int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* pl) {
+  if (!d || !io) return fail(c, SDEH_E_INVALID, "desc and io must not be NULL");
+  if (d->alg < SDEH_ALG_TSIT5 || d->alg > SDEH_ALG_AVERN9)
+    return fail(c, SDEH_E_INVALID, "alg %d out of range", d->alg);
+  if (d->dtype != SDEH_F64 && d->dtype != SDEH_F32)
+    return fail(c, SDEH_E_INVALID, "dtype %d out of range", d->dtype);
+  if (d->n_traj < 0) return fail(c, SDEH_E_INVALID, "n_traj < 0");
+  int r = rhs_shape(c, d->rhs, &pl->n_state, &pl->n_param, &pl->n_shared);
+  if (r) return r;
+  if (pl->n_shared > 0 && io->n_shared != pl->n_shared)
+    return fail(c, SDEH_E_INVALID, "rhs needs %lld shared elements, io->n_shared = %lld",
+                (long long)pl->n_shared, (long long)io->n_shared);
+  pl->es = d->dtype == SDEH_F64 ? 8 : 4;
+  const bool adaptive = is_adaptive(d->alg);
+  if (d->saveat && d->n_saveat > 0) {
+    pl->save = SAVE_AT;
+    for (int64_t j = 1; j < d->n_saveat; ++j)
+      if (!(d->saveat[j] >= d->saveat[j - 1]))
+        return fail(c, SDEH_E_INVALID, "saveat must be ascending (entry %lld)", (long long)j);
+  } else if (d->save_everystep) {
+    pl->save = SAVE_STEPS;
+    if (d->max_saves < 1)
+      return fail(c, SDEH_E_INVALID,
+                  "save_everystep needs max_saves >= 1 and u_steps/t_steps/n_steps buffers "
+                  "(set save_everystep = 0 for endpoint-only output)");
+  } else {
+    pl->save = SAVE_END;
+  }
+  if (d->precision == SDEH_PREC_KAHAN) {
+    if (d->dtype != SDEH_F32 || !adaptive)
+      return fail(c, SDEH_E_UNSUPPORTED, "the compensated update is a binary32 adaptive mode");
+    pl->kahan = 1;
+  } else if (d->precision != SDEH_PREC_PLAIN) {
+    return fail(c, SDEH_E_INVALID, "precision %d out of range", d->precision);
+  }
+  if (!adaptive && io->tspan)
+    return fail(c, SDEH_E_UNSUPPORTED, "fixed-step solvers take one common tspan");
+  if (d->block != 0 && (d->block < 64 || d->block > 256 || d->block % 64 != 0))
+    return fail(c, SDEH_E_INVALID, "block must be 64, 128, 192 or 256");
+  if (d->grid < 0 || d->chunk < 0) return fail(c, SDEH_E_INVALID, "grid/chunk must be >= 0");
+
+  sdeh_kargs& a = pl->a;
+  a.n = d->n_traj;
+  a.n_saveat = pl->save == SAVE_AT ? d->n_saveat : 0;
+  a.maxiters = d->maxiters > 0 ? d->maxiters : kDefaultMaxiters;
+  a.t0 = d->t0;
+  a.tf = d->tf;
+  a.dt = d->dt;
+  a.abstol = d->abstol;
+  a.reltol = d->reltol;
+  a.max_saves = pl->save == SAVE_STEPS ? d->max_saves : 0;
+  a.quirks = d->quirks;
+  if (!adaptive) {
+    if (d->dt == 0.0) return fail(c, SDEH_E_INVALID, "range step cannot be zero");
+    const sdeh_range::Grid g = sdeh_range::make(d->t0, d->dt, d->tf);
+    a.g_len = g.len;
+    a.g_rational = g.rational;
+    a.g_start_n = g.start_n;
+    a.g_step_n = g.step_n;
+    a.g_den = g.den;
+    a.g_start = g.start;
+    a.g_step = g.step;
+  }
+  r = resolve_kernel(c, d->rhs, d->alg, d->dtype, pl->save, pl->kahan, &pl->k);
+  if (r) return r;
+
+  // launch shape
+  Shape sh;
+  if (adaptive) {
+    auto it = c->tuned.find(TuneKey(d->rhs, d->alg, d->dtype, pl->save, pl->kahan));
+    if (it != c->tuned.end()) sh = it->second;
+    else sh.schedule = SDEH_SCHED_PERSISTENT;
+    if (d->schedule != SDEH_SCHED_AUTO) sh.schedule = d->schedule;
+    if (d->block) sh.block = d->block;
+    if (d->chunk) sh.chunk = d->chunk;
+    if (sh.schedule == SDEH_SCHED_PERSISTENT) {
+      if (d->grid) sh.grid = d->grid;
+      else if (it == c->tuned.end() || d->block) sh.grid = c->cus * blocks_per_cu(c, pl->k, sh.block);
+      if (sh.grid < 1) sh.grid = 1;
+    }
+  } else {
+    sh.schedule = SDEH_SCHED_ONE_LAUNCH;
+    if (d->block) sh.block = d->block;
+  }
+  if (sh.schedule == SDEH_SCHED_ONE_LAUNCH) {
+    const int64_t g = (d->n_traj + sh.block - 1) / sh.block;
+    if (g > 0x7fffffff) return fail(c, SDEH_E_INVALID, "ensemble too large for one launch");
+    sh.grid = (int)g;
+  }
+  a.persistent = sh.schedule == SDEH_SCHED_PERSISTENT;
+  a.chunk = sh.chunk;
+  pl->shape = sh;
+  return SDEH_OK;
+}
+
+// Bytes of each io buffer for a plan.
+struct Sizes {
+  size_t u0, p, ps, tspan, dt0, order, u_end, t_end, i32, u_saveat, u_steps, t_steps, saveat;
+};
+
+Sizes sizes_of(const sdeh_solve_desc* d, const sdeh_io* io, const Plan& pl) {
+  const size_t n = (size_t)d->n_traj;
+  Sizes s{};
+  s.u0 = (size_t)pl.n_state * n * pl.es;
+  s.p = (size_t)pl.n_param * n * pl.es;
+  s.ps = (size_t)pl.n_shared * pl.es;
+  s.tspan = io->tspan ? 2 * n * 8 : 0;
+  s.dt0 = io->dt0 ? n * 8 : 0;
+  s.order = io->order ? n * 8 : 0;
+  s.u_end = s.u0;
+  s.t_end = n * 8;
+  s.i32 = n * 4;
+  s.u_saveat = pl.save == SAVE_AT ? (size_t)d->n_saveat * pl.n_state * n * pl.es : 0;
+  s.u_steps = pl.save == SAVE_STEPS ? (size_t)d->max_saves * pl.n_state * n * pl.es : 0;
+  s.t_steps = pl.save == SAVE_STEPS ? (size_t)d->max_saves * n * 8 : 0;
+  s.saveat = pl.save == SAVE_AT ? (size_t)d->n_saveat * 8 : 0;
+  return s;
+}
+
+int check_order_host(sdeh_ctx* c, const int64_t* order, int64_t n) {
+  std::vector<unsigned char> seen((size_t)n, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t v = order[i];
+    if (v < 0 || v >= n || seen[(size_t)v])
+      return fail(c, SDEH_E_INVALID, "order is not a permutation of 0..n_traj-1 (entry %lld)",
+                  (long long)i);
+    seen[(size_t)v] = 1;
+  }
+  return SDEH_OK;
+}
+
+// Launch the prepared plan on `s` (kernel bracketed by the context events).
+int run_plan(sdeh_ctx* c, Plan& pl, hipStream_t s, unsigned long long* queue, bool timed) {
+  sdeh_kargs& a = pl.a;
+  if (pl.save == SAVE_AT) {
+    const int64_t cnt = a.n_saveat * pl.n_state * a.n;
+    if (pl.es == 8) HIPCHK(c, sdeh_util_fill_f64((double*)a.u_saveat, __builtin_nan(""), cnt, s));
+    else HIPCHK(c, sdeh_util_fill_f32((float*)a.u_saveat, __builtin_nanf(""), cnt, s));
+  }
+  a.queue = queue;
+  if (a.persistent) HIPCHK(c, sdeh_util_zero_u64(queue, 1, s));
+  if (a.n > 0) {
+    if (timed) HIPCHK(c, hipEventRecord(c->ev0, s));
+    HIPCHK(c, launch(pl.k, pl.shape.grid, pl.shape.block, &a, s));
+    if (timed) HIPCHK(c, hipEventRecord(c->ev1, s));
+  }
+  return SDEH_OK;
+}
+
+int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t memory,
+               hipStream_t s, int32_t sync) {
+  const auto t_start = std::chrono::steady_clock::now();
+  Plan pl;
+  int r = plan_solve(c, d, io, &pl);
+  if (r) return r;
+  const Sizes sz = sizes_of(d, io, pl);
+  const int64_t n = d->n_traj;
+  if (n > 0) {
+    if (!io->u0 || !io->u_end || !io->t_end || !io->retcode)
+      return fail(c, SDEH_E_INVALID, "u0, u_end, t_end and retcode are required");
+    if (pl.n_param > 0 && !io->p) return fail(c, SDEH_E_INVALID, "p is required (n_param > 0)");
+    if (pl.n_shared > 0 && !io->p_shared)
+      return fail(c, SDEH_E_INVALID, "p_shared is required for this rhs");
+    if (pl.save == SAVE_AT && !io->u_saveat)
+      return fail(c, SDEH_E_INVALID, "u_saveat is required with saveat");
+    if (pl.save == SAVE_STEPS && (!io->u_steps || !io->t_steps || !io->n_steps))
+      return fail(c, SDEH_E_INVALID, "u_steps, t_steps and n_steps are required");
+  }
+  sdeh_kargs& a = pl.a;
+  const bool host = memory == SDEH_MEM_HOST;
+  if (!host && memory != SDEH_MEM_DEVICE) return fail(c, SDEH_E_INVALID, "memory mode");
+  if (host && io->order && n > 0) {
+    r = check_order_host(c, io->order, n);
+    if (r) return r;
+  }
+
+  // Device layout: saveat copy + (host mode) all buffers + counters always.
+  size_t need = round256(sz.saveat);
+  const bool own_counts = !io->naccept || !io->nreject || host;
+  if (host) {
+    need += round256(sz.u0) + round256(sz.p) + round256(sz.ps) + round256(sz.tspan) +
+            round256(sz.dt0) + round256(sz.order) + round256(sz.u_end) + round256(sz.t_end) +
+            round256(sz.i32) + round256(sz.u_saveat) + round256(sz.u_steps) +
+            round256(sz.t_steps) + round256(sz.i32);
+  }
+  if (own_counts) need += 2 * round256(sz.i32);
+  if (!host && io->order) need += round256(sz.i32);
+  r = ensure_arena(c, need);
+  if (r) return r;
+  Arena ar{c->arena};
+  double* d_saveat = (double*)ar.take(sz.saveat);
+  if (sz.saveat) HIPCHK(c, hipMemcpyAsync(d_saveat, d->saveat, sz.saveat, hipMemcpyHostToDevice, s));
+  a.saveat = d_saveat;
+  if (host) {
+    auto up = [&](const void* src, size_t b) -> void* {
+      void* p = ar.take(b);
+      if (b && src) (void)hipMemcpyAsync(p, src, b, hipMemcpyHostToDevice, s);
+      return p;
+    };
+    a.u0 = up(io->u0, sz.u0);
+    a.p = up(io->p, sz.p);
+    a.ps = up(io->p_shared, sz.ps);
+    a.tspan = (const double*)up(io->tspan, sz.tspan);
+    a.dt0 = (const double*)up(io->dt0, sz.dt0);
+    a.order = (const int64_t*)up(io->order, sz.order);
+    HIPCHK(c, hipGetLastError());
+    a.u_end = ar.take(sz.u_end);
+    a.t_end = (double*)ar.take(sz.t_end);
+    a.retcode = (int32_t*)ar.take(sz.i32);
+    a.u_saveat = ar.take(sz.u_saveat);
+    a.u_steps = ar.take(sz.u_steps);
+    a.t_steps = (double*)ar.take(sz.t_steps);
+    a.n_steps = (int32_t*)ar.take(pl.save == SAVE_STEPS ? sz.i32 : 0);
+  } else if (n > 0) {
+    r = check_dev(c, io->u0, sz.u0, "u0");
+    if (!r) r = check_dev(c, io->p, sz.p, "p");
+    if (!r) r = check_dev(c, io->p_shared, sz.ps, "p_shared");
+    if (!r && io->tspan) r = check_dev(c, io->tspan, sz.tspan, "tspan");
+    if (!r && io->dt0) r = check_dev(c, io->dt0, sz.dt0, "dt0");
+    if (!r && io->order) r = check_dev(c, io->order, sz.order, "order");
+    if (!r) r = check_dev(c, io->u_end, sz.u_end, "u_end");
+    if (!r) r = check_dev(c, io->t_end, sz.t_end, "t_end");
+    if (!r) r = check_dev(c, io->retcode, sz.i32, "retcode");
+    if (!r && io->naccept) r = check_dev(c, io->naccept, sz.i32, "naccept");
+    if (!r && io->nreject) r = check_dev(c, io->nreject, sz.i32, "nreject");
+    if (!r && pl.save == SAVE_AT) r = check_dev(c, io->u_saveat, sz.u_saveat, "u_saveat");
+    if (!r && pl.save == SAVE_STEPS) {
+      r = check_dev(c, io->u_steps, sz.u_steps, "u_steps");
+      if (!r) r = check_dev(c, io->t_steps, sz.t_steps, "t_steps");
+      if (!r) r = check_dev(c, io->n_steps, sz.i32, "n_steps");
+    }
+    if (r) return r;
+    a.u0 = io->u0;
+    a.p = io->p;
+    a.ps = io->p_shared;
+    a.tspan = io->tspan;
+    a.dt0 = io->dt0;
+    a.order = io->order;
+    a.u_end = io->u_end;
+    a.t_end = io->t_end;
+    a.retcode = io->retcode;
+    a.u_saveat = io->u_saveat;
+    a.u_steps = io->u_steps;
+    a.t_steps = io->t_steps;
+    a.n_steps = io->n_steps;
+  }
+  a.naccept = (!host && io->naccept) ? io->naccept : nullptr;
+  a.nreject = (!host && io->nreject) ? io->nreject : nullptr;
+  if (!a.naccept) a.naccept = (int32_t*)ar.take(sz.i32);
+  if (!a.nreject) a.nreject = (int32_t*)ar.take(sz.i32);
+
+  if (!host && io->order && n > 0) {  // permutation check on the device
+    int32_t* seen = (int32_t*)ar.take(sz.i32);
+    HIPCHK(c, sdeh_util_zero_u64(c->counters + 5, 1, s));
+    HIPCHK(c, sdeh_util_check_order(io->order, n, seen, c->counters + 5, s));
+    unsigned long long bad = 0;
+    HIPCHK(c, hipMemcpyAsync(&bad, c->counters + 5, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (bad) return fail(c, SDEH_E_INVALID, "order is not a permutation of 0..n_traj-1");
+  }
+
+  r = run_plan(c, pl, s, c->counters, true);
+  if (r) return r;
+  const bool do_sync = host || sync;
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  if (do_sync) {  // counters are only read back by synchronous calls
+    HIPCHK(c, sdeh_util_zero_u64(c->counters + 1, 4, s));
+    HIPCHK(c, sdeh_util_count(a.naccept, a.nreject, a.retcode, n, c->counters + 1, s));
+  }
+  if (do_sync) HIPCHK(c, hipMemcpyAsync(cnt, c->counters + 1, sizeof cnt, hipMemcpyDeviceToHost, s));
+  if (host && n > 0) {
+    auto down = [&](void* dst, const void* src, size_t b) {
+      if (dst && b) (void)hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToHost, s);
+    };
+    down(io->u_end, a.u_end, sz.u_end);
+    down(io->t_end, a.t_end, sz.t_end);
+    down(io->retcode, a.retcode, sz.i32);
+    down(io->naccept, a.naccept, sz.i32);
+    down(io->nreject, a.nreject, sz.i32);
+    down(io->u_saveat, a.u_saveat, sz.u_saveat);
+    down(io->u_steps, a.u_steps, sz.u_steps);
+    down(io->t_steps, a.t_steps, sz.t_steps);
+    down(io->n_steps, a.n_steps, pl.save == SAVE_STEPS ? sz.i32 : 0);
+    HIPCHK(c, hipGetLastError());
+  }
+  c->stats = sdeh_stats{};
+  c->stats.block = pl.shape.block;
+  c->stats.grid = pl.shape.grid;
+  c->stats.chunk = pl.shape.chunk;
+  c->stats.schedule = pl.shape.schedule;
+  if (do_sync) {
+    HIPCHK(c, hipStreamSynchronize(s));
+    float ms = 0.f;
+    if (n > 0) HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stats.kernel_ms = ms;
+    c->stats.attempts = (int64_t)cnt[0];
+    c->stats.accepts = (int64_t)cnt[1];
+    c->stats.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
+            .count();
+    if (cnt[2]) return fail(c, SDEH_E_DTMIN, "%llu trajectories stopped at dt < dtmin", cnt[2]);
+    if (cnt[3])
+      return fail(c, SDEH_E_SAVE_OVERFLOW, "%llu trajectories had more than max_saves steps",
+                  cnt[3]);
+  }
+  return SDEH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void sdeh_desc_init(sdeh_solve_desc* d) {
+  if (!d) return;
+  memset(d, 0, sizeof *d);
+  d->alg = SDEH_ALG_ATSIT5;
+  d->dtype = SDEH_F64;
+  d->rhs = SDEH_RHS_LORENZ;
+  d->t0 = 0.0;
+  d->tf = 1.0;
+  d->dt = (double)0.1f;       // gpuatsit5.jl:208 dt = 0.1f0
+  d->abstol = (double)1.0e-6f;  // gpuatsit5.jl:210 abstol = 1.0f-6
+  d->reltol = (double)1.0e-3f;  // reltol = 1.0f-3
+  d->save_everystep = 1;      // gpuatsit5.jl:209
+  d->quirks = 1;
+  d->schedule = SDEH_SCHED_AUTO;
+  d->precision = SDEH_PREC_PLAIN;
+}
+
+int sdeh_abi_version(void) { return SDEH_ABI_VERSION; }
+
+int sdeh_create(sdeh_ctx** out, int device) {
+  if (!out) return SDEH_E_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SDEH_E_HIP;
+  if (device < 0 || device >= ndev) return SDEH_E_INVALID;
+  sdeh_ctx* c = new sdeh_ctx;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipMalloc((void**)&c->counters, 8 * sizeof(unsigned long long)) != hipSuccess) {
+    sdeh_destroy(c);
+    return SDEH_E_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+  if (c->cus <= 0) c->cus = 256;
+  *out = c;
+  return SDEH_OK;
+}
+
+void sdeh_destroy(sdeh_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& g : c->graphs) {
+    if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+    if (g.second.graph) (void)hipGraphDestroy(g.second.graph);
+    if (g.second.mem) (void)hipFree(g.second.mem);
+  }
+  for (auto& m : c->rtc) (void)hipModuleUnload(m.second.first);
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->counters) (void)hipFree(c->counters);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* sdeh_last_error(const sdeh_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int sdeh_rhs_info(sdeh_ctx* c, int64_t rhs, int32_t* n_state, int32_t* n_param,
+                  int64_t* n_shared) {
+  if (!c || !n_state || !n_param || !n_shared) return SDEH_E_INVALID;
+  return rhs_shape(c, rhs, n_state, n_param, n_shared);
+}
+
+int sdeh_compile_rhs(sdeh_ctx* c, const char* src, const char* fn_name, int32_t n_state,
+                     int32_t n_param, int64_t n_shared, int64_t* handle) {
+  if (!c || !src || !fn_name || !handle) return SDEH_E_INVALID;
+  if (n_state < 1 || n_state > 64 || n_param < 0 || n_param > 256 || n_shared < 0)
+    return fail(c, SDEH_E_INVALID, "rhs shape out of range (1 <= n_state <= 64)");
+  for (const char* q = fn_name; *q; ++q)
+    if (!(isalnum((unsigned char)*q) || *q == '_'))
+      return fail(c, SDEH_E_INVALID, "fn_name must be an identifier");
+  UserRhs u;
+  u.src = src;
+  u.fn = fn_name;
+  u.n = n_state;
+  u.p = n_param;
+  u.s = n_shared;
+  c->user.push_back(u);
+  const int64_t h = kUserBase + (int64_t)c->user.size() - 1;
+  // Compile the default kernel now so source errors surface at registration.
+  KernelRef k;
+  const int r = user_kernel(c, h, SDEH_ALG_ATSIT5, SDEH_F64, SAVE_END, 0, &k);
+  if (r) {
+    c->user.pop_back();
+    return r;
+  }
+  *handle = h;
+  return SDEH_OK;
+}
+
+int sdeh_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t memory,
+               void* stream, int32_t sync) {
+  if (!c) return SDEH_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
+  return solve_impl(c, d, io, memory, stream ? (hipStream_t)stream : c->stream, sync);
+}
+
+int sdeh_get_stats(const sdeh_ctx* c, sdeh_stats* out) {
+  if (!c || !out) return SDEH_E_INVALID;
+  *out = c->stats;
+  return SDEH_OK;
+}
+
+int sdeh_synchronize(sdeh_ctx* c) {
+  if (!c) return SDEH_E_INVALID;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SDEH_OK;
+}
+
+int64_t sdeh_fixed_grid(double t0, double dt, double tf, double* elems, int64_t cap) {
+  if (dt == 0.0) return -1;
+  const sdeh_range::Grid g = sdeh_range::make(t0, dt, tf);
+  if (elems)
+    for (int64_t i = 0; i < std::min(cap, g.len); ++i) elems[i] = sdeh_range::elem(g, i);
+  return g.len;
+}
+
+int sdeh_graph_capture(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io,
+                       int64_t* graph_handle) {
+  if (!c || !graph_handle) return SDEH_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
+  Plan pl;
+  int r = plan_solve(c, d, io, &pl);
+  if (r) return r;
+  if (io->order) return fail(c, SDEH_E_UNSUPPORTED, "graph capture takes no order permutation");
+  const Sizes sz = sizes_of(d, io, pl);
+  const int64_t n = d->n_traj;
+  r = check_dev(c, io->u0, sz.u0, "u0");
+  if (!r) r = check_dev(c, io->p, sz.p, "p");
+  if (!r) r = check_dev(c, io->p_shared, sz.ps, "p_shared");
+  if (!r && io->tspan) r = check_dev(c, io->tspan, sz.tspan, "tspan");
+  if (!r && io->dt0) r = check_dev(c, io->dt0, sz.dt0, "dt0");
+  if (!r) r = check_dev(c, io->u_end, sz.u_end, "u_end");
+  if (!r) r = check_dev(c, io->t_end, sz.t_end, "t_end");
+  if (!r) r = check_dev(c, io->retcode, sz.i32, "retcode");
+  if (!r) r = check_dev(c, io->naccept, sz.i32, "naccept");
+  if (!r) r = check_dev(c, io->nreject, sz.i32, "nreject");
+  if (!r && pl.save == SAVE_AT) r = check_dev(c, io->u_saveat, sz.u_saveat, "u_saveat");
+  if (!r && pl.save == SAVE_STEPS) {
+    r = check_dev(c, io->u_steps, sz.u_steps, "u_steps");
+    if (!r) r = check_dev(c, io->t_steps, sz.t_steps, "t_steps");
+    if (!r) r = check_dev(c, io->n_steps, sz.i32, "n_steps");
+  }
+  if (r) return r;
+  if (n == 0) return fail(c, SDEH_E_INVALID, "empty ensemble");
+  GraphRec g;
+  HIPCHK(c, hipMalloc(&g.mem, 256 + round256(sz.saveat)));
+  double* d_saveat = (double*)((char*)g.mem + 256);
+  if (sz.saveat) {
+    HIPCHK(c, hipMemcpy(d_saveat, d->saveat, sz.saveat, hipMemcpyHostToDevice));
+  }
+  sdeh_kargs& a = pl.a;
+  a.saveat = d_saveat;
+  a.u0 = io->u0;
+  a.p = io->p;
+  a.ps = io->p_shared;
+  a.tspan = io->tspan;
+  a.dt0 = io->dt0;
+  a.order = nullptr;
+  a.u_end = io->u_end;
+  a.t_end = io->t_end;
+  a.retcode = io->retcode;
+  a.naccept = io->naccept;
+  a.nreject = io->nreject;
+  a.u_saveat = io->u_saveat;
+  a.u_steps = io->u_steps;
+  a.t_steps = io->t_steps;
+  a.n_steps = io->n_steps;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  r = run_plan(c, pl, c->stream, (unsigned long long*)g.mem, false);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (r) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipFree(g.mem);
+    return r;
+  }
+  if (e != hipSuccess) {
+    (void)hipFree(g.mem);
+    return fail(c, SDEH_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  }
+  g.graph = graph;
+  if (hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    (void)hipFree(g.mem);
+    return fail(c, SDEH_E_HIP, "hipGraphInstantiate failed");
+  }
+  const int64_t h = c->next_graph++;
+  c->graphs[h] = g;
+  *graph_handle = h;
+  return SDEH_OK;
+}
+
+int sdeh_graph_launch(sdeh_ctx* c, int64_t h, void* stream) {
+  if (!c) return SDEH_E_INVALID;
+  auto it = c->graphs.find(h);
+  if (it == c->graphs.end()) return fail(c, SDEH_E_INVALID, "unknown graph %lld", (long long)h);
+  HIPCHK(c, hipGraphLaunch(it->second.exec, stream ? (hipStream_t)stream : c->stream));
+  return SDEH_OK;
+}
+
+int sdeh_graph_destroy(sdeh_ctx* c, int64_t h) {
+  if (!c) return SDEH_E_INVALID;
+  auto it = c->graphs.find(h);
+  if (it == c->graphs.end()) return fail(c, SDEH_E_INVALID, "unknown graph %lld", (long long)h);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)hipGraphExecDestroy(it->second.exec);
+  (void)hipGraphDestroy(it->second.graph);
+  (void)hipFree(it->second.mem);
+  c->graphs.erase(it);
+  return SDEH_OK;
+}
+
+int sdeh_autotune(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, sdeh_stats* best) {
+  if (!c || !d || !io) return SDEH_E_INVALID;
+  if (!is_adaptive(d->alg))
+    return fail(c, SDEH_E_INVALID, "autotune applies to the adaptive solvers");
+  HIPCHK(c, hipSetDevice(c->device));
+  struct Cand {
+    int sched, block, chunk, occ_div;
+  };
+  std::vector<Cand> cands;
+  for (int b : {64, 128, 256}) cands.push_back({SDEH_SCHED_ONE_LAUNCH, b, 0, 1});
+  for (int b : {64, 128, 256})
+    for (int ch : {16, 64, 256}) cands.push_back({SDEH_SCHED_PERSISTENT, b, ch, 1});
+  for (int ch : {64}) cands.push_back({SDEH_SCHED_PERSISTENT, 256, ch, 2});
+  double best_ms = 1e300;
+  Shape best_shape;
+  sdeh_stats best_stats{};
+  for (const Cand& k : cands) {
+    sdeh_solve_desc dd = *d;
+    dd.schedule = k.sched;
+    dd.block = k.block;
+    dd.chunk = k.chunk;
+    dd.grid = 0;
+    if (k.sched == SDEH_SCHED_PERSISTENT) {
+      Plan pl;
+      int r = plan_solve(c, &dd, io, &pl);
+      if (r) return r;
+      dd.grid = std::max(1, pl.shape.grid / k.occ_div);
+    }
+    double ms = 1e300;
+    for (int rep = 0; rep < 2; ++rep) {  // first run warms caches and modules
+      int r = sdeh_solve(c, &dd, io, SDEH_MEM_DEVICE, nullptr, 1);
+      if (r && r != SDEH_E_DTMIN) return r;
+      ms = c->stats.kernel_ms;
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_shape.schedule = k.sched;
+      best_shape.block = k.block;
+      best_shape.chunk = k.chunk ? k.chunk : 64;
+      best_shape.grid = c->stats.grid;
+      best_stats = c->stats;
+    }
+  }
+  Plan pl;
+  int r = plan_solve(c, d, io, &pl);
+  if (r) return r;
+  c->tuned[TuneKey(d->rhs, d->alg, d->dtype, pl.save, pl.kahan)] = best_shape;
+  if (best) *best = best_stats;
+  return SDEH_OK;
+}
+
+int sdeh_mlp_forward(sdeh_ctx* c, const float* params, const float* x, float* y, int64_t n,
+                     void* stream) {
+  if (!c || !params || !x || !y || n < 0) return SDEH_E_INVALID;
+  int r = check_dev(c, params, SDEH_MLP_NPARAM * 4, "params");
+  if (!r) r = check_dev(c, x, (size_t)n * SDEH_MLP_NIN * 4, "x");
+  if (!r) r = check_dev(c, y, (size_t)n * SDEH_MLP_NOUT * 4, "y");
+  if (r) return r;
+  r = sdeh_mlp_forward_impl(params, x, y, n, stream ? (hipStream_t)stream : c->stream);
+  if (r) return fail(c, r, "mlp forward launch failed");
+  return SDEH_OK;
+}
+
+int sdeh_mlp_loss_grad(sdeh_ctx* c, const float* params, const float* x, const float* y,
+                       int64_t n, float* loss, float* grad, void* stream) {
+  if (!c || !params || !x || !y || !loss || !grad || n < 1) return SDEH_E_INVALID;
+  int r = check_dev(c, params, SDEH_MLP_NPARAM * 4, "params");
+  if (!r) r = check_dev(c, x, (size_t)n * SDEH_MLP_NIN * 4, "x");
+  if (!r) r = check_dev(c, y, (size_t)n * SDEH_MLP_NOUT * 4, "y");
+  if (!r) r = check_dev(c, loss, 4, "loss");
+  if (!r) r = check_dev(c, grad, SDEH_MLP_NPARAM * 4, "grad");
+  if (r) return r;
+  // scratch for per-block partial gradients lives in the arena
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+  const size_t need = (size_t)blocks * (SDEH_MLP_NPARAM + 1) * 4;
+  r = ensure_arena(c, need);
+  if (r) return r;
+  r = sdeh_mlp_loss_grad_impl(params, x, y, n, loss, grad, (float*)c->arena,
+                              stream ? (hipStream_t)stream : c->stream);
+  if (r) return fail(c, r, "mlp loss/grad launch failed");
+  return SDEH_OK;
+}
+
+int sdeh_mlp_adam(sdeh_ctx* c, float* params, const float* grad, float* m, float* v, float lr,
+                  float beta1, float beta2, float eps, int32_t t, void* stream) {
+  if (!c || !params || !grad || !m || !v || t < 1) return SDEH_E_INVALID;
+  int r = check_dev(c, params, SDEH_MLP_NPARAM * 4, "params");
+  if (!r) r = check_dev(c, grad, SDEH_MLP_NPARAM * 4, "grad");
+  if (!r) r = check_dev(c, m, SDEH_MLP_NPARAM * 4, "m");
+  if (!r) r = check_dev(c, v, SDEH_MLP_NPARAM * 4, "v");
+  if (r) return r;
+  r = sdeh_mlp_adam_impl(params, grad, m, v, lr, beta1, beta2, eps, t,
+                         stream ? (hipStream_t)stream : c->stream);
+  if (r) return fail(c, r, "mlp adam launch failed");
+  return SDEH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,9 @@
+"""simplediffeq_hip: MI355X ensemble integrator for SimpleDiffEq.jl's
+GPU-compatible out-of-place solvers (GPUSimpleTsit5, GPUSimpleATsit5,
+GPUSimpleVern7, GPUSimpleAVern7, GPUSimpleVern9, GPUSimpleAVern9).
+
+The compute path is libsimplediffeq_hip.so (HIP kernels for gfx950 behind a C
+ABI); this package is its Python host side.
+"""
+from . import _abi  # noqa: F401
+from ._abi import load_library  # noqa: F401

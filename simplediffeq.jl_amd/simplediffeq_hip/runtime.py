@@ -1,0 +1,234 @@
+"""Thin ctypes layer over libsimplediffeq_hip.
+
+``Context`` owns one ``sdeh_ctx``; ``EnsembleBuffers`` lays out the
+structure-of-arrays inputs/outputs of one ensemble solve (numpy arrays on the
+host, or torch tensors on the GPU) and fills the C ``sdeh_io`` struct.
+The same buffer object is accepted by the CPU oracle's loader in tests, so
+both paths see identical bytes.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+
+_NP_DTYPE = {_abi.F64: np.float64, _abi.F32: np.float32}
+
+
+class SolveError(RuntimeError):
+    def __init__(self, status, message):
+        self.status = status
+        super().__init__("%s (%d): %s" % (_abi.STATUS_NAMES.get(status, "?"), status, message))
+
+
+def make_desc(alg, dtype=_abi.F64, rhs=_abi.RHS_LORENZ, n_traj=0, tspan=(0.0, 1.0), dt=None,
+              abstol=None, reltol=None, saveat=None, save_everystep=False, max_saves=0,
+              maxiters=0, quirks=True, schedule=_abi.SCHED_AUTO, precision=_abi.PREC_PLAIN,
+              block=0, grid=0, chunk=0):
+    """Build an sdeh_solve_desc.  Unset dt/abstol/reltol take the reference
+    defaults (0.1f0, 1f-6, 1f-3).  Keeps the saveat array alive on the struct."""
+    d = _abi.default_desc()
+    d.alg = int(alg)
+    d.dtype = int(dtype)
+    d.rhs = int(rhs)
+    d.n_traj = int(n_traj)
+    d.t0, d.tf = float(tspan[0]), float(tspan[1])
+    if dt is not None:
+        d.dt = float(dt)
+    if abstol is not None:
+        d.abstol = float(abstol)
+    if reltol is not None:
+        d.reltol = float(reltol)
+    d.save_everystep = 1 if save_everystep else 0
+    d.max_saves = int(max_saves)
+    d.maxiters = int(maxiters)
+    d.quirks = 1 if quirks else 0
+    d.schedule = int(schedule)
+    d.precision = int(precision)
+    d.block, d.grid, d.chunk = int(block), int(grid), int(chunk)
+    if saveat is not None and len(saveat) > 0:
+        arr = np.ascontiguousarray(np.asarray(saveat, dtype=np.float64))
+        d._saveat_keep = arr
+        d.saveat = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        d.n_saveat = arr.size
+    return d
+
+
+class EnsembleBuffers:
+    """Inputs and outputs of one ensemble solve, structure-of-arrays.
+
+    u0: (n_state, n_traj); p: (n_param, n_traj); p_shared: (n_shared,);
+    tspan: (2, n_traj) or None; dt0: (n_traj,) or None; order: (n_traj,) int64 or None.
+    Backend "numpy" (host) or "torch" (device tensors on `device`)."""
+
+    def __init__(self, desc, n_state, u0, p=None, p_shared=None, tspan=None, dt0=None,
+                 order=None, backend="numpy", device=None):
+        self.desc = desc
+        self.backend = backend
+        n = int(desc.n_traj)
+        dt_np = _NP_DTYPE[desc.dtype]
+        self.n_state = n_state
+        if backend == "numpy":
+            conv = lambda a, dt: None if a is None else np.ascontiguousarray(np.asarray(a, dtype=dt))
+            empty = lambda shape, dt: np.empty(shape, dtype=dt)
+            ptr = lambda a: None if a is None else a.ctypes.data
+        elif backend == "torch":
+            import torch
+            tdt = {np.float64: torch.float64, np.float32: torch.float32, np.int32: torch.int32,
+                   np.int64: torch.int64}
+
+            def conv(a, dt):
+                if a is None:
+                    return None
+                if isinstance(a, torch.Tensor):
+                    return a.to(device=device, dtype=tdt[dt]).contiguous()
+                return torch.as_tensor(np.asarray(a, dtype=dt), device=device).contiguous()
+
+            empty = lambda shape, dt: torch.empty(shape, dtype=tdt[dt], device=device)
+            ptr = lambda a: None if a is None else a.data_ptr()
+        else:
+            raise ValueError("backend must be 'numpy' or 'torch'")
+        self.u0 = conv(u0, dt_np)
+        self.p = conv(p, dt_np)
+        self.p_shared = conv(p_shared, dt_np)
+        self.tspan = conv(tspan, np.float64)
+        self.dt0 = conv(dt0, np.float64)
+        self.order = conv(order, np.int64)
+        self.u_end = empty((n_state, n), dt_np)
+        self.t_end = empty((n,), np.float64)
+        self.retcode = empty((n,), np.int32)
+        self.naccept = empty((n,), np.int32)
+        self.nreject = empty((n,), np.int32)
+        ns = int(desc.n_saveat)
+        self.u_saveat = empty((ns, n_state, n), dt_np) if ns > 0 else None
+        everystep = ns == 0 and desc.save_everystep
+        ms = int(desc.max_saves)
+        self.u_steps = empty((ms, n_state, n), dt_np) if everystep and ms > 0 else None
+        self.t_steps = empty((ms, n), np.float64) if everystep and ms > 0 else None
+        self.n_steps = empty((n,), np.int32) if everystep else None
+        io = _abi.SolveIO()
+        io.u0 = ptr(self.u0)
+        io.p = ptr(self.p)
+        io.p_shared = ptr(self.p_shared)
+        io.n_shared = 0 if self.p_shared is None else int(np.prod(tuple(self.p_shared.shape)))
+        io.tspan = ptr(self.tspan)
+        io.dt0 = ptr(self.dt0)
+        io.order = ptr(self.order)
+        io.u_end = ptr(self.u_end)
+        io.t_end = ptr(self.t_end)
+        io.retcode = ptr(self.retcode)
+        io.naccept = ptr(self.naccept)
+        io.nreject = ptr(self.nreject)
+        io.u_saveat = ptr(self.u_saveat)
+        io.u_steps = ptr(self.u_steps)
+        io.t_steps = ptr(self.t_steps)
+        io.n_steps = ptr(self.n_steps)
+        self.io = io
+
+    def numpy(self, name):
+        a = getattr(self, name)
+        if a is None or self.backend == "numpy":
+            return a
+        return a.cpu().numpy()
+
+
+class Context:
+    """One library context on a HIP device."""
+
+    def __init__(self, device=0):
+        self.lib = _abi.load_library()
+        h = ctypes.c_void_p()
+        st = self.lib.sdeh_create(ctypes.byref(h), int(device))
+        if st != 0:
+            raise SolveError(st, "sdeh_create(device=%d) failed" % device)
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sdeh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, allow=()):
+        if st != 0 and st not in allow:
+            raise SolveError(st, self.lib.sdeh_last_error(self.h).decode())
+        return st
+
+    def last_error(self):
+        return self.lib.sdeh_last_error(self.h).decode()
+
+    def rhs_info(self, rhs):
+        n, p, s = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        self._check(self.lib.sdeh_rhs_info(self.h, int(rhs), ctypes.byref(n), ctypes.byref(p),
+                                           ctypes.byref(s)))
+        return n.value, p.value, s.value
+
+    def compile_rhs(self, src, fn_name, n_state, n_param, n_shared=0):
+        h = ctypes.c_int64()
+        self._check(self.lib.sdeh_compile_rhs(self.h, src.encode(), fn_name.encode(),
+                                              int(n_state), int(n_param), int(n_shared),
+                                              ctypes.byref(h)))
+        return h.value
+
+    def solve(self, desc, bufs, stream=None, sync=True, allow=(_abi.E_DTMIN,)):
+        """Run sdeh_solve on an EnsembleBuffers; returns the status."""
+        mem = _abi.MEM_HOST if bufs.backend == "numpy" else _abi.MEM_DEVICE
+        st = self.lib.sdeh_solve(self.h, ctypes.byref(desc), ctypes.byref(bufs.io), mem,
+                                 ctypes.c_void_p(stream), 1 if sync else 0)
+        return self._check(st, allow)
+
+    def stats(self):
+        s = _abi.Stats()
+        self._check(self.lib.sdeh_get_stats(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _abi.Stats._fields_}
+
+    def autotune(self, desc, bufs):
+        s = _abi.Stats()
+        self._check(self.lib.sdeh_autotune(self.h, ctypes.byref(desc), ctypes.byref(bufs.io),
+                                           ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _abi.Stats._fields_}
+
+    def graph_capture(self, desc, bufs):
+        h = ctypes.c_int64()
+        self._check(self.lib.sdeh_graph_capture(self.h, ctypes.byref(desc), ctypes.byref(bufs.io),
+                                                ctypes.byref(h)))
+        return h.value
+
+    def graph_launch(self, handle, stream=None):
+        self._check(self.lib.sdeh_graph_launch(self.h, int(handle), ctypes.c_void_p(stream)))
+
+    def graph_destroy(self, handle):
+        self._check(self.lib.sdeh_graph_destroy(self.h, int(handle)))
+
+    def synchronize(self):
+        self._check(self.lib.sdeh_synchronize(self.h))
+
+    # MLP cost model (device pointers, e.g. torch tensor data_ptr()).
+    def mlp_forward(self, params, x, y, n, stream=None):
+        self._check(self.lib.sdeh_mlp_forward(self.h, params, x, y, int(n), ctypes.c_void_p(stream)))
+
+    def mlp_loss_grad(self, params, x, y, n, loss, grad, stream=None):
+        self._check(self.lib.sdeh_mlp_loss_grad(self.h, params, x, y, int(n), loss, grad,
+                                                ctypes.c_void_p(stream)))
+
+    def mlp_adam(self, params, grad, m, v, lr, beta1, beta2, eps, t, stream=None):
+        self._check(self.lib.sdeh_mlp_adam(self.h, params, grad, m, v, float(lr), float(beta1),
+                                           float(beta2), float(eps), int(t),
+                                           ctypes.c_void_p(stream)))
+
+
+def fixed_grid(t0, dt, tf, cap=0):
+    """length(t0:dt:tf) and its first `cap` elements (Julia Float64 range rules)."""
+    lib = _abi.load_library()
+    lib.sdeh_fixed_grid.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_void_p, ctypes.c_int64]
+    lib.sdeh_fixed_grid.restype = ctypes.c_int64
+    out = np.empty(max(cap, 1), dtype=np.float64)
+    n = lib.sdeh_fixed_grid(float(t0), float(dt), float(tf), out.ctypes.data, int(cap))
+    return n, out[:min(cap, max(n, 0))]

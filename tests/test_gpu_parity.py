@@ -1,0 +1,102 @@
+"""GPU integrators vs the CPU oracle on identical inputs, through the C ABI.
+
+In binary64 the kernels implement the same operation order as the oracle's
+fused mode and the same pow, so the comparison is bit for bit (endpoint state,
+final time, retcode, accepted and rejected step counts, saveat output).
+"""
+import numpy as np
+import pytest
+
+from helpers import ALG_NAMES, assert_bit_identical, problem, run_pair
+from simplediffeq_hip import _abi, inputs, runtime
+
+pytestmark = pytest.mark.gpu
+
+FIXED = (_abi.ALG_TSIT5, _abi.ALG_VERN7, _abi.ALG_VERN9)
+ADAPT = (_abi.ALG_ATSIT5, _abi.ALG_AVERN7, _abi.ALG_AVERN9)
+
+
+def test_lorenz_small_fixed_tsit5_bitexact(ctx):
+    """Lorenz 1,024 trajectories, rho ~ U(0,21), GPUSimpleTsit5 dt = 0.01 on (0, 1)."""
+    u0, p = inputs.lorenz_small(1024, seed=0)
+    d = runtime.make_desc(_abi.ALG_TSIT5, rhs=_abi.RHS_LORENZ, n_traj=1024, tspan=(0.0, 1.0),
+                          dt=0.01)
+    st_g, g, st_c, c = run_pair(ctx, d, u0, p)
+    assert st_g == 0 and st_c == 0
+    assert (g.naccept == 100).all()
+    assert_bit_identical(g, c)
+
+
+def test_lorenz_small_adaptive_atsit5_bitexact(ctx):
+    """Same ensemble, GPUSimpleATsit5 abstol 1e-6, reltol 1e-3, dt0 = 0.1."""
+    u0, p = inputs.lorenz_small(1024, seed=0)
+    d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=1024, tspan=(0.0, 1.0),
+                          dt=0.1, abstol=1e-6, reltol=1e-3)
+    st_g, g, st_c, c = run_pair(ctx, d, u0, p)
+    assert st_g == 0 and st_c == 0
+    assert_bit_identical(g, c)
+
+
+@pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT])
+@pytest.mark.parametrize("alg", list(range(6)), ids=lambda a: ALG_NAMES[a])
+@pytest.mark.parametrize("rhs", [_abi.RHS_LORENZ, _abi.RHS_EXPDECAY3, _abi.RHS_AFFINE1,
+                                 _abi.RHS_LINEAR2], ids=["lorenz", "expdecay3", "affine1",
+                                                         "linear2"])
+def test_endpoint_bitexact(ctx, rhs, alg, sched):
+    n = 777
+    u0, p, ps = problem(rhs, n, seed=alg + 10 * rhs)
+    adaptive = alg in ADAPT
+    if alg in FIXED and sched == _abi.SCHED_PERSISTENT:
+        pytest.skip("fixed-step solvers always use one lane per trajectory")
+    d = runtime.make_desc(alg, rhs=rhs, n_traj=n, tspan=(0.0, 2.0),
+                          dt=0.1 if adaptive else 0.02, abstol=1e-7, reltol=1e-6,
+                          schedule=sched, chunk=16)
+    st_g, g, st_c, c = run_pair(ctx, d, u0, p, ps)
+    assert st_g == st_c
+    assert_bit_identical(g, c)
+
+
+@pytest.mark.parametrize("quirks", [True, False])
+@pytest.mark.parametrize("alg", list(range(6)), ids=lambda a: ALG_NAMES[a])
+@pytest.mark.parametrize("rhs", [_abi.RHS_LORENZ, _abi.RHS_AFFINE1], ids=["lorenz", "affine1"])
+def test_saveat_bitexact(ctx, rhs, alg, quirks):
+    n = 300
+    u0, p, ps = problem(rhs, n, seed=3 + alg)
+    adaptive = alg in ADAPT
+    saveat = np.concatenate([[0.0, 0.05, 0.4], np.linspace(0.5, 1.5, 23), [1.5, 1.5]])
+    d = runtime.make_desc(alg, rhs=rhs, n_traj=n, tspan=(0.0, 1.5), dt=0.1 if adaptive else 0.01,
+                          abstol=1e-8, reltol=1e-7, saveat=saveat, quirks=quirks)
+    st_g, g, st_c, c = run_pair(ctx, d, u0, p, ps)
+    assert st_g == st_c == 0
+    assert_bit_identical(g, c, fields=("u_end", "t_end", "retcode", "naccept", "nreject",
+                                       "u_saveat"))
+
+
+@pytest.mark.parametrize("alg", list(range(6)), ids=lambda a: ALG_NAMES[a])
+def test_save_everystep_bitexact(ctx, alg):
+    n = 200
+    u0, p, _ = problem(_abi.RHS_LORENZ, n, seed=7)
+    adaptive = alg in ADAPT
+    d = runtime.make_desc(alg, rhs=_abi.RHS_LORENZ, n_traj=n, tspan=(0.0, 1.0),
+                          dt=0.1 if adaptive else 0.05, abstol=1e-6, reltol=1e-4,
+                          save_everystep=True, max_saves=64)
+    st_g, g, st_c, c = run_pair(ctx, d, u0, p)
+    assert st_g == st_c
+    assert_bit_identical(g, c, fields=("u_end", "t_end", "retcode", "naccept", "nreject",
+                                       "n_steps"))
+    ns = g.n_steps
+    for i in range(n):
+        k = ns[i]
+        assert np.array_equal(g.t_steps[:k, i], c.t_steps[:k, i])
+        assert np.array_equal(g.u_steps[:k, :, i], c.u_steps[:k, :, i])
+
+
+def test_device_memory_path_matches_host(ctx):
+    torch = pytest.importorskip("torch")
+    n = 4096
+    u0, p = inputs.lorenz_wide(n, seed=1)
+    d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=n, tspan=(0.0, 10.0),
+                          dt=float(np.float32(0.1)), abstol=1e-6, reltol=1e-3)
+    st_g, g, st_c, c = run_pair(ctx, d, u0, p, backend="torch")
+    assert st_g == 0 and st_c == 0
+    assert_bit_identical(g, c)
