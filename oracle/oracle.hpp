@@ -614,7 +614,8 @@ void atsit5(const F& f, Traj<T>& tr, const Mode& md) {
     }
   }
 done:
-  cpy<T, N>(tr.u_end, u);
+  // A trajectory that stops early reports its last accepted state.
+  cpy<T, N>(tr.u_end, tr.retcode == SDEH_RET_DEFAULT ? u : uprev);
   tr.t_end = s.t;
   if (!tr.saveat && !tr.save_everystep) push_step<T, N>(tr, s.t, u);  // :322-325
 }
@@ -858,7 +859,8 @@ void avern7(const F& f, Traj<T>& tr, const Mode& md) {
     }
   }
 done:
-  cpy<T, N>(tr.u_end, u);
+  // A trajectory that stops early reports its last accepted state.
+  cpy<T, N>(tr.u_end, tr.retcode == SDEH_RET_DEFAULT ? u : uprev);
   tr.t_end = s.t;
   if (!tr.saveat && !tr.save_everystep) push_step<T, N>(tr, s.t, u);
 }
@@ -1142,7 +1144,8 @@ void avern9(const F& f, Traj<T>& tr, const Mode& md) {
     }
   }
 done:
-  cpy<T, N>(tr.u_end, u);
+  // A trajectory that stops early reports its last accepted state.
+  cpy<T, N>(tr.u_end, tr.retcode == SDEH_RET_DEFAULT ? u : uprev);
   tr.t_end = s.t;
   if (!tr.saveat && !tr.save_everystep) push_step<T, N>(tr, s.t, u);
 }
