@@ -7,3 +7,7 @@ ABI); this package is its Python host side.
 """
 from . import _abi  # noqa: F401
 from ._abi import load_library  # noqa: F401
+from .problem import (DtLessThanMin, EnsembleHIP, EnsembleProblem, EnsembleSolution,  # noqa: F401
+                      GPUSimpleATsit5, GPUSimpleAVern7, GPUSimpleAVern9, GPUSimpleTsit5,
+                      GPUSimpleVern7, GPUSimpleVern9, ODEProblem, ODESolution, isadaptive,
+                      remake, rhs, solve)

@@ -1,0 +1,316 @@
+"""The reference's solve surface, for an ensemble on the GPU.
+
+Mirrors what a SimpleDiffEq.jl user writes:
+
+    prob = ODEProblem(f, u0, tspan, p)                    # SciMLBase ODEProblem
+    sol = solve(prob, GPUSimpleATsit5(), dt=..., abstol=..., reltol=...)
+    eprob = EnsembleProblem(prob, prob_func=lambda prob, i, repeat: remake(prob, p=...))
+    esol = solve(eprob, GPUSimpleATsit5(), EnsembleHIP(), trajectories=N, ...)
+
+with the reference's keyword defaults (gpuatsit5.jl:55-61, 205-212:
+dt = 0.1f0, abstol = 1f-6, reltol = 1f-3, save_everystep = true,
+saveat = nothing) and its results: per trajectory an ODESolution with t, u
+and retcode "Default", or the error "dt<dtmin" the reference throws
+(gpuatsit5.jl:256).  The algorithm structs are the six GPUSimple* types.
+
+Every solve here runs on the HIP library (one trajectory per GPU lane); there
+is no CPU fallback.  Right-hand sides: the library's built-ins (rhs.lorenz,
+rhs.glv16(A), ...) use kernels compiled ahead of time; any other Python
+function f(u, p, t) is traced into HIP C (tracer.py) and compiled with hipRTC.
+"""
+import inspect
+
+import numpy as np
+
+from . import _abi, runtime, tracer
+
+
+# ---------------------------------------------------------------------------
+# Algorithms (gpuatsit5.jl:52,200; gpuvern7.jl:52,295; gpuvern9.jl:52,406)
+# ---------------------------------------------------------------------------
+class _Alg:
+    code = None
+    adaptive = False
+
+    def __repr__(self):
+        return type(self).__name__ + "()"
+
+
+class GPUSimpleTsit5(_Alg):
+    code = _abi.ALG_TSIT5
+
+
+class GPUSimpleATsit5(_Alg):
+    code = _abi.ALG_ATSIT5
+    adaptive = True
+
+
+class GPUSimpleVern7(_Alg):
+    code = _abi.ALG_VERN7
+
+
+class GPUSimpleAVern7(_Alg):
+    code = _abi.ALG_AVERN7
+    adaptive = True
+
+
+class GPUSimpleVern9(_Alg):
+    code = _abi.ALG_VERN9
+
+
+class GPUSimpleAVern9(_Alg):
+    code = _abi.ALG_AVERN9
+    adaptive = True
+
+
+def isadaptive(alg):
+    return alg.adaptive
+
+
+class DtLessThanMin(RuntimeError):
+    """The reference's error("dt<dtmin") (gpuatsit5.jl:256)."""
+
+
+# ---------------------------------------------------------------------------
+# Problems and solutions
+# ---------------------------------------------------------------------------
+class ODEProblem:
+    """Out-of-place problem du = f(u, p, t) on tspan (SciMLBase ODEProblem)."""
+
+    def __init__(self, f, u0, tspan, p=None):
+        self.f = f
+        self.u0 = np.atleast_1d(np.asarray(u0))
+        self.tspan = (float(tspan[0]), float(tspan[1]))
+        self.p = None if p is None else np.atleast_1d(np.asarray(p))
+
+    def isinplace(self):
+        if isinstance(self.f, BuiltinRHS):
+            return False
+        try:
+            return len(inspect.signature(self.f).parameters) == 4
+        except (TypeError, ValueError):
+            return False
+
+
+def remake(prob, u0=None, tspan=None, p=None):
+    return ODEProblem(prob.f, prob.u0 if u0 is None else u0,
+                      prob.tspan if tspan is None else tspan, prob.p if p is None else p)
+
+
+class ODESolution:
+    """t, u (list of states) and retcode of one trajectory."""
+
+    def __init__(self, t, u, retcode, naccept, nreject, alg):
+        self.t = t
+        self.u = u
+        self.retcode = retcode
+        self.stats = {"naccept": int(naccept), "nreject": int(nreject)}
+        self.alg = alg
+
+    def __len__(self):
+        return len(self.t)
+
+
+class EnsembleProblem:
+    """SciMLBase EnsembleProblem: prob_func(prob, i, repeat) makes trajectory i
+    (i is 1-based, as in Julia); output_func(sol, i) -> (out, rerun) and
+    reduction(u, batch, I) -> (u, converged) post-process on the host.
+
+    For large ensembles the trajectory data can instead be given directly as
+    arrays: u0s (n_state, N), ps (n_param, N), tspans (2, N)."""
+
+    def __init__(self, prob, prob_func=None, output_func=None, reduction=None, u_init=None,
+                 safetycopy=True, u0s=None, ps=None, tspans=None):
+        self.prob = prob
+        self.prob_func = prob_func
+        self.output_func = output_func
+        self.reduction = reduction
+        self.u_init = u_init
+        self.safetycopy = safetycopy
+        self.u0s, self.ps, self.tspans = u0s, ps, tspans
+
+
+class EnsembleHIP:
+    """Ensemble algorithm: all trajectories on one GPU through libsimplediffeq_hip."""
+
+    def __init__(self, device=0, schedule="auto", precision="plain", memory="host"):
+        self.device = device
+        self.schedule = {"auto": _abi.SCHED_AUTO, "one_launch": _abi.SCHED_ONE_LAUNCH,
+                         "persistent": _abi.SCHED_PERSISTENT}[schedule]
+        self.precision = {"plain": _abi.PREC_PLAIN, "kahan": _abi.PREC_KAHAN}[precision]
+        self.memory = memory
+
+
+class EnsembleSolution:
+    def __init__(self, u, converged, raw):
+        self.u = u
+        self.converged = converged
+        self.raw = raw  # the EnsembleBuffers (arrays in structure-of-arrays layout)
+
+    def __len__(self):
+        return len(self.u)
+
+    def __getitem__(self, i):
+        return self.u[i]
+
+
+# ---------------------------------------------------------------------------
+# Right-hand sides
+# ---------------------------------------------------------------------------
+class BuiltinRHS:
+    """A right-hand side the library compiles ahead of time."""
+
+    def __init__(self, rid, n_state, n_param, shared=None, pyfunc=None):
+        self.rid, self.n_state, self.n_param = rid, n_state, n_param
+        self.shared = shared
+        self.pyfunc = pyfunc
+
+    def __call__(self, u, p, t):
+        return self.pyfunc(u, p, t)
+
+
+class rhs:
+    """Built-in right-hand sides: rhs.lorenz, rhs.glv16(A), rhs.expdecay3,
+    rhs.affine1, rhs.linear2 (definitions in include/simplediffeq_hip.h)."""
+
+    lorenz = BuiltinRHS(_abi.RHS_LORENZ, 3, 3, pyfunc=lambda u, p, t: [
+        p[0] * (u[1] - u[0]), u[0] * (p[1] - u[2]) - u[1], u[0] * u[1] - p[2] * u[2]])
+    expdecay3 = BuiltinRHS(_abi.RHS_EXPDECAY3, 3, 1, pyfunc=lambda u, p, t: [p[0] * x for x in u])
+    affine1 = BuiltinRHS(_abi.RHS_AFFINE1, 1, 2, pyfunc=lambda u, p, t: [p[0] * u[0] + p[1] * t])
+    linear2 = BuiltinRHS(_abi.RHS_LINEAR2, 2, 4, pyfunc=lambda u, p, t: [
+        p[0] * u[0] + p[1] * u[1], p[2] * u[0] + p[3] * u[1]])
+
+    @staticmethod
+    def glv16(A):
+        A = np.asarray(A, dtype=np.float64).reshape(16, 16)
+
+        def f(u, p, t):
+            out = []
+            for i in range(16):
+                acc = p[i]
+                for j in range(16):
+                    acc = acc + A[i, j] * u[j]
+                out.append(u[i] * acc)
+            return out
+        return BuiltinRHS(_abi.RHS_GLV16, 16, 16, shared=A.reshape(-1), pyfunc=f)
+
+
+_CONTEXTS = {}
+_TRACED = {}
+
+
+def _context(device):
+    if device not in _CONTEXTS:
+        _CONTEXTS[device] = runtime.Context(device)
+    return _CONTEXTS[device]
+
+
+def _rhs_handle(ctx, f, n_state, n_param):
+    if isinstance(f, BuiltinRHS):
+        return f.rid, f.shared
+    key = (id(ctx), f, n_state, n_param)
+    if key not in _TRACED:
+        src, fn = tracer.trace(f, n_state, n_param, "sdeh_traced_%d" % len(_TRACED))
+        _TRACED[key] = ctx.compile_rhs(src, fn, n_state, n_param)
+    return _TRACED[key], None
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+# ---------------------------------------------------------------------------
+# solve
+# ---------------------------------------------------------------------------
+def solve(prob, alg, ensemblealg=None, *, trajectories=None, dt=_f32(0.1), saveat=None,
+          save_everystep=True, abstol=_f32(1e-6), reltol=_f32(1e-3), max_saves=None,
+          dtype=None, maxiters=0, quirks=True, dt0s=None, order=None, **kwargs):
+    """solve(ODEProblem, alg; ...) -> ODESolution
+    solve(EnsembleProblem, alg, EnsembleHIP(); trajectories, ...) -> EnsembleSolution
+
+    Unknown keywords are accepted and ignored, as the reference's `kwargs...`."""
+    if not isinstance(alg, _Alg):
+        raise TypeError("alg must be one of the GPUSimple* algorithms")
+    single = isinstance(prob, ODEProblem)
+    if single:
+        eprob = EnsembleProblem(prob)
+        trajectories = 1
+        ens = ensemblealg or EnsembleHIP()
+    else:
+        eprob = prob
+        ens = ensemblealg or EnsembleHIP()
+        if trajectories is None:
+            raise TypeError("trajectories is required for an EnsembleProblem")
+    base = eprob.prob
+    assert not base.isinplace(), "the GPUSimple solvers are out-of-place only"
+    n = int(trajectories)
+
+    # Materialise the ensemble (prob_func is called with 1-based i, repeat = 1).
+    if eprob.u0s is not None:
+        u0s = np.asarray(eprob.u0s)
+        ps = None if eprob.ps is None else np.asarray(eprob.ps)
+        tspans = None if eprob.tspans is None else np.asarray(eprob.tspans, dtype=np.float64)
+    elif eprob.prob_func is None:
+        u0s = np.repeat(base.u0.reshape(-1, 1), n, axis=1)
+        ps = None if base.p is None else np.repeat(base.p.reshape(-1, 1), n, axis=1)
+        tspans = None
+    else:
+        probs = [eprob.prob_func(base, i + 1, 1) for i in range(n)]
+        u0s = np.stack([np.asarray(q.u0).reshape(-1) for q in probs], axis=1)
+        ps = None if base.p is None else np.stack([np.asarray(q.p).reshape(-1) for q in probs],
+                                                  axis=1)
+        spans = np.array([q.tspan for q in probs], dtype=np.float64).T
+        tspans = None if (spans == np.array(base.tspan).reshape(2, 1)).all() else spans
+    n_state = u0s.shape[0]
+    n_param = 0 if ps is None else ps.shape[0]
+    if dtype is None:
+        dtype = _abi.F32 if u0s.dtype == np.float32 else _abi.F64
+    elif dtype in ("f32", np.float32):
+        dtype = _abi.F32
+    elif dtype in ("f64", np.float64):
+        dtype = _abi.F64
+
+    ctx = _context(ens.device)
+    rid, shared = _rhs_handle(ctx, base.f, n_state, n_param)
+    if ps is None:
+        ps = np.zeros((0, n))
+    everystep = saveat is None and bool(save_everystep)
+    if everystep and max_saves is None:
+        max_saves = 4096
+    saveat_arr = None
+    if saveat is not None:
+        saveat_arr = np.atleast_1d(np.asarray(saveat, dtype=np.float64))
+    d = runtime.make_desc(alg.code, dtype=dtype, rhs=rid, n_traj=n, tspan=base.tspan, dt=dt,
+                          abstol=abstol, reltol=reltol, saveat=saveat_arr,
+                          save_everystep=everystep, max_saves=max_saves or 0, maxiters=maxiters,
+                          quirks=quirks, schedule=ens.schedule, precision=ens.precision)
+    bufs = runtime.EnsembleBuffers(d, n_state, u0s, ps if n_param else None, shared, tspans,
+                                   dt0s, order)
+    st = ctx.solve(d, bufs, allow=(_abi.E_DTMIN,))
+    if st == _abi.E_DTMIN:
+        raise DtLessThanMin("dt<dtmin")
+
+    t0s = tspans[0] if tspans is not None else np.full(n, base.tspan[0])
+    sols = []
+    for i in range(n):
+        rc = _abi.RETCODE_NAMES[int(bufs.retcode[i])]
+        if saveat_arr is not None:
+            t = saveat_arr
+            u = [bufs.u_saveat[j, :, i] for j in range(len(t))]
+        elif everystep:
+            k = int(bufs.n_steps[i])
+            t = bufs.t_steps[:k, i]
+            u = [bufs.u_steps[j, :, i] for j in range(k)]
+        else:
+            t = np.array([t0s[i], bufs.t_end[i]])
+            u = [u0s[:, i].astype(bufs.u_end.dtype), bufs.u_end[:, i]]
+        sols.append(ODESolution(t, u, rc, bufs.naccept[i], bufs.nreject[i], alg))
+    if single:
+        return sols[0]
+    outs = sols
+    if eprob.output_func is not None:
+        outs = [eprob.output_func(s, i + 1)[0] for i, s in enumerate(sols)]
+    converged = True
+    if eprob.reduction is not None:
+        outs, converged = eprob.reduction(eprob.u_init, outs, range(1, n + 1))
+    return EnsembleSolution(outs, converged, bufs)

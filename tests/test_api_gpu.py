@@ -1,0 +1,124 @@
+"""The reference's own tests for these solvers, run through this package's
+solve() on the GPU.
+
+test/gpu_ode_regression.jl: u' = -u, Float32 u0 = ones(3), tspan (0, 1), every
+GPUSimple solver pair; bars against an OrdinaryDiffEq Vern9 solution, here
+against the exact solution exp(-t) u0 (what that Vern9 solution approximates).
+test/gpusimpleatsit5_tests.jl: Lorenz (10, 28, 8/3), u0 = 10 ones; fixed and
+adaptive Tsit5 consistency between save modes.  Plus the traced-RHS path
+(hipRTC) against the ahead-of-time kernel, and EnsembleProblem semantics.
+"""
+import numpy as np
+import pytest
+
+import simplediffeq_hip as sd
+from helpers import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+F = np.float32
+
+
+def neg(u, p, t):
+    return [-u[0], -u[1], -u[2]]
+
+
+PAIRS = [(sd.GPUSimpleATsit5(), sd.GPUSimpleTsit5()), (sd.GPUSimpleAVern7(), sd.GPUSimpleVern7()),
+         (sd.GPUSimpleAVern9(), sd.GPUSimpleVern9())]
+
+
+@pytest.mark.parametrize("adaptive_alg,fixed_alg", PAIRS, ids=["Tsit5", "Vern7", "Vern9"])
+def test_gpu_ode_regression(adaptive_alg, fixed_alg):
+    u0 = np.ones(3, dtype=F)
+    prob = sd.ODEProblem(neg, u0, (0.0, 1.0), np.array([10, 28, 8 / 3], dtype=F))
+    exact = lambda t: np.exp(-np.asarray(t, dtype=np.float64))[..., None] * np.ones(3)
+
+    sol = sd.solve(prob, fixed_alg, dt=float(F(0.01)))
+    asol = sd.solve(prob, adaptive_alg, dt=float(F(0.01)), abstol=float(F(1e-7)),
+                    reltol=float(F(1e-7)), save_everystep=False)
+    assert sol.retcode == "Default" and asol.retcode == "Default"
+    assert np.linalg.norm(exact(1.0) - sol.u[-1]) < 5e-3
+    assert np.linalg.norm(exact(asol.t) - np.array(asol.u)) < 5e-4
+
+    saveat = [0.0, 0.4]
+    sol = sd.solve(prob, fixed_alg, dt=float(F(0.01)), saveat=saveat)
+    asol = sd.solve(prob, adaptive_alg, dt=float(F(0.01)), abstol=float(F(1e-7)),
+                    reltol=float(F(1e-7)), saveat=saveat)
+    assert np.linalg.norm(asol.u[-1] - sol.u[-1]) < 5e-3
+    assert np.linalg.norm(exact(saveat) - np.array(sol.u)) < 2e-4
+    assert np.linalg.norm(exact(saveat) - np.array(asol.u)) < 2e-4
+    assert len(sol.u) == len(saveat) and len(asol.u) == len(saveat)
+
+    saveat = np.arange(101, dtype=F) * F(0.01)
+    sol = sd.solve(prob, fixed_alg, dt=float(F(0.01)), saveat=saveat)
+    asol = sd.solve(prob, adaptive_alg, dt=float(F(0.01)), save_everystep=False,
+                    abstol=float(F(1e-7)), reltol=float(F(1e-7)), saveat=saveat)
+    reached = ~np.isnan(np.array(sol.u)[:, 0])
+    assert np.linalg.norm(exact(saveat)[reached] - np.array(sol.u)[reached]) < 2e-3
+    assert np.linalg.norm(exact(saveat) - np.array(asol.u)) < 3e-3
+    assert len(sol.u) == len(saveat) and len(asol.u) == len(saveat)
+
+    sol = sd.solve(prob, fixed_alg, dt=float(F(0.01)), save_everystep=False)
+    assert len(sol.u) == 2 and np.linalg.norm(exact(sol.t) - np.array(sol.u)) < 5e-3
+
+
+def lorenz(u, p, t):
+    return [p[0] * (u[1] - u[0]), u[0] * (p[1] - u[2]) - u[1], u[0] * u[1] - p[2] * u[2]]
+
+
+def test_gpusimpleatsit5_consistency():
+    prob = sd.ODEProblem(sd.rhs.lorenz, 10 * np.ones(3), (0.0, 100.0), [10.0, 28.0, 8 / 3])
+    sol2 = sd.solve(prob, sd.GPUSimpleATsit5(), dt=1e-2, abstol=1e-6, reltol=1e-3,
+                    max_saves=20000)
+    sol3 = sd.solve(prob, sd.GPUSimpleATsit5(), dt=1e-2, abstol=1e-6, reltol=1e-3,
+                    save_everystep=False)
+    assert sol2.t[-1] == sol3.t[-1] == 100.0
+    assert bits_equal(sol2.u[-1], sol3.u[-1])
+    # fixed step: every step / endpoint / saveat [5, 100] agree (gpusimpleatsit5_tests.jl:52-67)
+    s2 = sd.solve(prob, sd.GPUSimpleTsit5(), dt=0.1, max_saves=2000)
+    s3 = sd.solve(prob, sd.GPUSimpleTsit5(), dt=0.1, save_everystep=False)
+    s4 = sd.solve(prob, sd.GPUSimpleTsit5(), dt=0.1, saveat=[5.0, 100.0])
+    assert len(s2.t) == 1001 and bits_equal(s2.u[-1], s3.u[-1])
+    assert np.allclose(s4.u[-1], s3.u[-1], rtol=1e-12, atol=0)
+    i5 = int(np.argmin(np.abs(s2.t - 5.0)))
+    assert np.allclose(s4.u[0], s2.u[i5], rtol=1e-10)
+
+
+def test_traced_rhs_matches_builtin_bitwise():
+    # The hipRTC path: the traced Lorenz function runs the same operations as the
+    # built-in kernel, so results are identical to the bit.
+    from simplediffeq_hip import inputs
+    u0, p = inputs.lorenz_wide(2048, seed=1)
+    for alg in (sd.GPUSimpleATsit5(), sd.GPUSimpleAVern7(), sd.GPUSimpleTsit5()):
+        kw = dict(trajectories=2048, save_everystep=False, dt=0.1 if alg.adaptive else 0.01)
+        a = sd.solve(sd.EnsembleProblem(sd.ODEProblem(lorenz, u0[:, 0], (0.0, 5.0), p[:, 0]),
+                                        u0s=u0, ps=p), alg, sd.EnsembleHIP(), **kw)
+        b = sd.solve(sd.EnsembleProblem(sd.ODEProblem(sd.rhs.lorenz, u0[:, 0], (0.0, 5.0),
+                                                      p[:, 0]), u0s=u0, ps=p),
+                     alg, sd.EnsembleHIP(), **kw)
+        assert bits_equal(a.raw.u_end, b.raw.u_end) and bits_equal(a.raw.naccept, b.raw.naccept)
+
+
+def test_ensemble_problem_prob_func_output_func_reduction():
+    base = sd.ODEProblem(sd.rhs.lorenz, [1.0, 0.0, 0.0], (0.0, 1.0), [10.0, 0.0, 8 / 3])
+    rhos = np.linspace(0.0, 21.0, 64)
+    eprob = sd.EnsembleProblem(
+        base, prob_func=lambda prob, i, repeat: sd.remake(prob, p=[10.0, rhos[i - 1], 8 / 3]),
+        output_func=lambda sol, i: (sol.u[-1][2], False),
+        reduction=lambda u, data, I: ((u or 0.0) + sum(data), True), u_init=0.0)
+    esol = sd.solve(eprob, sd.GPUSimpleATsit5(), sd.EnsembleHIP(), trajectories=64, dt=0.1,
+                    abstol=1e-6, reltol=1e-3, save_everystep=False)
+    single = [sd.solve(sd.remake(base, p=[10.0, r, 8 / 3]), sd.GPUSimpleATsit5(), dt=0.1,
+                       abstol=1e-6, reltol=1e-3, save_everystep=False).u[-1][2] for r in rhos]
+    assert esol.converged and esol.u == sum(single)
+    # per-trajectory tspans from prob_func
+    eprob = sd.EnsembleProblem(base, prob_func=lambda prob, i, r: sd.remake(prob, tspan=(0.0, 0.5 * i)))
+    esol = sd.solve(eprob, sd.GPUSimpleATsit5(), sd.EnsembleHIP(), trajectories=4,
+                    save_everystep=False)
+    assert [s.t[-1] for s in esol.u] == [0.5, 1.0, 1.5, 2.0]
+
+
+def test_dtmin_raises_like_the_reference():
+    prob = sd.ODEProblem(sd.rhs.expdecay3, np.ones(3), (0.0, 1.0), [-1.0])
+    with pytest.raises(sd.DtLessThanMin, match="dt<dtmin"):
+        sd.solve(prob, sd.GPUSimpleATsit5(), dt=1e-15, save_everystep=False)
