@@ -67,6 +67,223 @@ def cpu_baseline(u0, p, desc_kwargs, sample, threads):
     return sample / el, el
 
 
+def _time_solves(ctx, d, bufs, stream, steps, warmup):
+    """Average device time per solve (HIP events on the launch stream) and wall time."""
+    import torch
+    for _ in range(warmup):
+        ctx.solve(d, bufs, stream=stream.cuda_stream, sync=False)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        ctx.solve(d, bufs, stream=stream.cuda_stream, sync=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps, (time.perf_counter() - t0) / steps
+
+
+def workload_small(args):
+    """The 1,024-trajectory Lorenz ensemble (rho ~ U(0,21), seed 0, tspan (0,1)):
+    fixed Tsit5 dt = 0.01 and ATsit5 1e-6/1e-3, GPU vs the CPU restatement."""
+    import numpy as np
+    import torch
+    from oracle import pyoracle
+    from simplediffeq_hip import _abi, inputs, runtime
+    u0, p = inputs.lorenz_small(1024, seed=0)
+    ctx = runtime.Context(0)
+    stream = torch.cuda.Stream()
+    out = {"workload": "lorenz-small"}
+    for name, alg, dt in (("tsit5", _abi.ALG_TSIT5, 0.01), ("atsit5", _abi.ALG_ATSIT5, 0.1)):
+        d = runtime.make_desc(alg, rhs=_abi.RHS_LORENZ, n_traj=1024, tspan=(0.0, 1.0), dt=dt,
+                              abstol=1e-6, reltol=1e-3)
+        g = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
+        ms, wall = _time_solves(ctx, d, g, stream, args.steps, args.warmup)
+        c = runtime.EnsembleBuffers(d, 3, u0, p)
+        threads = min(16, len(os.sched_getaffinity(0)))
+        t0 = time.perf_counter()
+        for _ in range(5):
+            pyoracle.solve(d, c, nthreads=threads)
+        cpu = (time.perf_counter() - t0) / 5
+        same = all(np.array_equal(g.numpy(f), c.numpy(f)) for f in ("u_end", "naccept", "nreject"))
+        out[name] = {"gpu_kernel_ms": ms, "gpu_call_ms": wall * 1e3, "cpu_ms": cpu * 1e3,
+                     "cpu_threads": threads, "bit_identical": bool(same)}
+    return out
+
+
+def workload_uneven(args):
+    """2,097,152 Lorenz trajectories, rho ~ U(0,60), t_end ~ LogUniform(0.5,20),
+    u0 ~ N(0,1)^3 (seed 2), AVern7 binary64 at 1e-8/1e-8: one-launch vs persistent
+    vs autotuned, and K repeats from a HIP graph vs K solve calls."""
+    import torch
+    from simplediffeq_hip import _abi, inputs, runtime
+    n = args.n if args.n != N_TRAJ else 1 << 21
+    u0, p, ts = inputs.lorenz_uneven(n, seed=2)
+    ctx = runtime.Context(0)
+    stream = torch.cuda.Stream()
+    out = {"workload": "lorenz-uneven", "trajectories": n}
+    base = dict(rhs=_abi.RHS_LORENZ, n_traj=n, tspan=(0.0, 1.0), dt=float(__import__("numpy").float32(0.1)),
+                abstol=1e-8, reltol=1e-8)
+    for sched_name, sched in (("one_launch", _abi.SCHED_ONE_LAUNCH), ("persistent", _abi.SCHED_PERSISTENT)):
+        d = runtime.make_desc(_abi.ALG_AVERN7, schedule=sched, **base)
+        g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device="cuda:0")
+        ctx.solve(d, g, sync=True)
+        st = ctx.stats()
+        ms, wall = _time_solves(ctx, d, g, stream, args.steps, args.warmup)
+        flops = algorithmic_flops(_abi.ALG_AVERN7, 3, LORENZ_FLOPS, st["attempts"], st["accepts"], n)
+        out[sched_name] = {"kernel_ms": ms, "traj_per_s": n / (ms * 1e-3),
+                           "tflops": flops / (ms * 1e-3) / 1e12, "attempts": st["attempts"],
+                           "launch": {k: st[k] for k in ("block", "grid", "chunk")}}
+    d = runtime.make_desc(_abi.ALG_AVERN7, **base)
+    g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device="cuda:0")
+    best = ctx.autotune(d, g)
+    ms, _ = _time_solves(ctx, d, g, stream, args.steps, args.warmup)
+    out["autotuned"] = {"kernel_ms": ms, "traj_per_s": n / (ms * 1e-3),
+                        "launch": {k: best[k] for k in ("block", "grid", "chunk", "schedule")}}
+    h = ctx.graph_capture(d, g)
+    torch.cuda.synchronize()
+    reps = max(args.steps, 5)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.graph_launch(h)
+    ctx.synchronize()
+    t_graph = (time.perf_counter() - t0) / reps
+    ctx.graph_destroy(h)
+    out["graph_replay"] = {"ms_per_repeat": t_graph * 1e3, "traj_per_s": n / t_graph}
+    return out
+
+
+def workload_glv(args):
+    """GLV-16, 131,072 trajectories (seed 3), tspan (0,50), saveat 256 evenly spaced
+    points: AVern9 binary64 at 1e-10 and ATsit5 binary32 at 1e-6."""
+    import numpy as np
+    import torch
+    from simplediffeq_hip import _abi, inputs, runtime
+    n = args.n if args.n != N_TRAJ else 1 << 17
+    u0, r, A = inputs.glv16(n, seed=3)
+    saveat = np.linspace(0.0, 50.0, 256)
+    ctx = runtime.Context(0)
+    stream = torch.cuda.Stream()
+    out = {"workload": "glv16-saveat", "trajectories": n}
+    for name, alg, dtype, tol in (("avern9_f64", _abi.ALG_AVERN9, _abi.F64, 1e-10),
+                                  ("atsit5_f32", _abi.ALG_ATSIT5, _abi.F32, 1e-6)):
+        d = runtime.make_desc(alg, dtype=dtype, rhs=_abi.RHS_GLV16, n_traj=n, tspan=(0.0, 50.0),
+                              dt=0.1, abstol=tol, reltol=tol, saveat=saveat)
+        g = runtime.EnsembleBuffers(d, 16, u0, r, A, backend="torch", device="cuda:0")
+        ctx.solve(d, g, sync=True)
+        st = ctx.stats()
+        ms, _ = _time_solves(ctx, d, g, stream, max(1, args.steps // 4), 1)
+        es = 8 if dtype == _abi.F64 else 4
+        out_bytes = 256 * 16 * n * es
+        per_attempt = 16 * STAGE_FLOPS[alg] + RHS_PER_ATTEMPT[alg] * 528 + 12
+        flops = st["attempts"] * per_attempt + (st["accepts"] * 528 if alg == 5 else n * 528)
+        out[name] = {"kernel_ms": ms, "traj_per_s": n / (ms * 1e-3), "attempts": st["attempts"],
+                     "saveat_GBs": out_bytes / (ms * 1e-3) / 1e9,
+                     "tflops_excl_saveat": flops / (ms * 1e-3) / 1e12}
+    return out
+
+
+def workload_costmodel(args):
+    """Learned cost binning + dt0.  Train the 6-64-64-2 MLP on the GPU from the
+    accepted-step counts of 2^20 plain binary64 ATsit5 solves (Lorenz, sigma~U(8,12),
+    rho~U(0,28), beta~U(2,3), u0~U(-1,1)^3, t_end~LogUniform(0.5,10); seed 40) at
+    abstol 1e-7, reltol 1e-5; evaluate on 2^22 fresh trajectories (seed 4):
+    binary32 + compensated update, persistent schedule, binned by predicted cost and
+    dt0-seeded, against the plain unsorted binary64 one-launch kernel.  Endpoint error
+    of both against binary64 AVern9 at 1e-14 on a 65,536-trajectory subset."""
+    import numpy as np
+    import torch
+    from simplediffeq_hip import _abi, costmodel, inputs, runtime
+    n_train = min(args.n, 1 << 20) if args.n != N_TRAJ else 1 << 20
+    n_eval = 1 << 22 if args.n == N_TRAJ else args.n
+    ctx = runtime.Context(0)
+    stream = torch.cuda.Stream()
+    dev = "cuda:0"
+    out = {"workload": "costmodel", "train": n_train, "eval": n_eval}
+    tol = dict(abstol=1e-7, reltol=1e-5)
+    f32_01 = float(np.float32(0.1))
+
+    # 1. step logs of the plain solver
+    u0, p, ts = inputs.lorenz_costmodel(n_train, seed=40)
+    d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=n_train, dt=f32_01, **tol)
+    g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device=dev)
+    ctx.solve(d, g, sync=True)
+    x = costmodel.features(p, u0, ts)
+    y = costmodel.targets(g.numpy("naccept"), ts)
+    del g
+    # 2. train on the GPU (forward/backward/Adam kernels)
+    cm = costmodel.CostModel(ctx, seed=0)
+    t0 = time.perf_counter()
+    hist = cm.fit(x, y, steps=args.train_steps, batch=65536, lr=3e-3, log_every=200)
+    out["train_s"] = time.perf_counter() - t0
+    out["loss"] = [hist[0][1], hist[-1][1]]
+
+    # 3. evaluation ensemble
+    u0, p, ts = inputs.lorenz_costmodel(n_eval, seed=4)
+    xe = costmodel.features(p, u0, ts)
+    t0 = time.perf_counter()
+    pred = cm.predict(xe)
+    order = cm.order_by_cost(pred)
+    dt0 = cm.dt0(pred)
+    out["predict_and_sort_s"] = time.perf_counter() - t0
+
+    def run(dtype, sched, precision, order_=None, dt0_=None):
+        dd = runtime.make_desc(_abi.ALG_ATSIT5, dtype=dtype, rhs=_abi.RHS_LORENZ, n_traj=n_eval,
+                               dt=f32_01, schedule=sched, precision=precision, **tol)
+        gg = runtime.EnsembleBuffers(dd, 3, u0, p, tspan=ts, dt0=dt0_, order=order_,
+                                     backend="torch", device=dev)
+        ctx.solve(dd, gg, sync=True)
+        st = ctx.stats()
+        ms, _ = _time_solves(ctx, dd, gg, stream, args.steps, 1)
+        return gg, ms, st
+
+    base, ms_base, st_base = run(_abi.F64, _abi.SCHED_ONE_LAUNCH, _abi.PREC_PLAIN)
+    out["f64_one_launch_unsorted"] = {"kernel_ms": ms_base, "traj_per_s": n_eval / (ms_base * 1e-3),
+                                      "attempts": st_base["attempts"]}
+    rec = {}
+    for name, dtype, sched, prec, od, dz in (
+            ("f64_persistent_unsorted", _abi.F64, _abi.SCHED_PERSISTENT, _abi.PREC_PLAIN, None, None),
+            ("f64_persistent_binned", _abi.F64, _abi.SCHED_PERSISTENT, _abi.PREC_PLAIN, order, None),
+            ("f32k_persistent_unsorted", _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN, None, None),
+            ("f32k_one_launch_binned", _abi.F32, _abi.SCHED_ONE_LAUNCH, _abi.PREC_KAHAN, order, None),
+            ("f32k_persistent_binned", _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN, order, None),
+            ("f32k_persistent_binned_dt0", _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN, order, dt0)):
+        gg, ms, st = run(dtype, sched, prec, od, dz)
+        rec[name] = gg
+        out[name] = {"kernel_ms": ms, "traj_per_s": n_eval / (ms * 1e-3), "attempts": st["attempts"],
+                     "speedup_vs_f64_one_launch": ms_base / ms}
+    # binning leaves every trajectory's result unchanged
+    out["binned_bit_identical"] = bool(
+        np.array_equal(rec["f64_persistent_binned"].numpy("u_end"), base.numpy("u_end")) and
+        np.array_equal(rec["f32k_persistent_binned"].numpy("u_end"),
+                       rec["f32k_persistent_unsorted"].numpy("u_end")))
+    # 4. error against binary64 AVern9 at 1e-14 on a subset
+    ns = min(65536, n_eval)
+    dref = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_LORENZ, n_traj=ns, dt=f32_01,
+                             abstol=1e-14, reltol=1e-14)
+    gref = runtime.EnsembleBuffers(dref, 3, u0[:, :ns], p[:, :ns], tspan=ts[:, :ns],
+                                   backend="torch", device=dev)
+    ctx.solve(dref, gref, sync=True)
+    uref = gref.numpy("u_end").astype(np.float64)
+    ok = gref.numpy("retcode") == 0
+
+    def err(gg):
+        ue = gg.numpy("u_end")[:, :ns].astype(np.float64)
+        e = np.max(np.abs(ue - uref), axis=0) / np.max(np.abs(uref), axis=0)
+        e = e[ok]
+        return {"median": float(np.median(e)), "p90": float(np.percentile(e, 90))}
+    out["error_vs_avern9_1e-14"] = {"f64_one_launch_unsorted": err(base),
+                                    "f32k_persistent_binned": err(rec["f32k_persistent_binned"]),
+                                    "f32k_persistent_binned_dt0": err(rec["f32k_persistent_binned_dt0"]),
+                                    "reference_ok": int(ok.sum())}
+    a_base = np.asarray(base.numpy("naccept"), dtype=np.int64) + base.numpy("nreject")
+    out["wave_efficiency"] = {"unsorted": costmodel.wave_efficiency(a_base),
+                              "binned": costmodel.wave_efficiency(a_base, order)}
+    out["target_speedup_3x_met"] = out["f32k_persistent_binned_dt0"]["speedup_vs_f64_one_launch"] >= 3.0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,12 +294,22 @@ def main():
     ap.add_argument("--no-f32", action="store_true", help="skip the binary32 run")
     ap.add_argument("--cpu-sample", type=int, default=1 << 17)
     ap.add_argument("--tune", action="store_true", help="autotune the launch shape first")
+    ap.add_argument("--workload", default="lorenz-wide",
+                    choices=["lorenz-wide", "lorenz-small", "lorenz-uneven", "glv16", "costmodel"],
+                    help="lorenz-wide is the headline line; the others report their own JSON")
+    ap.add_argument("--train-steps", type=int, default=1500)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
               "(trajectories are independent, see DESIGN.md for how the ensemble shards)"
               % args.gpus, file=sys.stderr)
         return 2
+
+    extra = {"lorenz-small": workload_small, "lorenz-uneven": workload_uneven,
+             "glv16": workload_glv, "costmodel": workload_costmodel}
+    if args.workload in extra:
+        print(json.dumps(extra[args.workload](args)))
+        return 0
 
     import numpy as np
     import torch

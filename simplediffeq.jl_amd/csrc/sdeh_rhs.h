@@ -28,16 +28,24 @@ struct Lorenz {
 // Generalised Lotka-Volterra, 16 species: du_i = u_i (r_i + sum_j A_ij u_j),
 // the sum accumulated from r_i as fma(A_ij, u_j, acc) for j = 0..15.
 // p = r (16 per trajectory); ps = A, 16x16 row-major, shared.
+// A is read through the constant address space (the same matrix for every lane,
+// at compile-time offsets), so it arrives by scalar loads into SGPRs and feeds
+// the FMAs directly; the empty asm is a compiler memory barrier that keeps those
+// loads inside each evaluation instead of letting the compiler hold all of A
+// (256 values) in vector registers across the whole integration loop.
 struct GLV16 {
   static constexpr int N = 16, P = 16, S = 256;
   template <class T>
   __device__ static inline void f(T* du, const T* u, const T* p, const T* __restrict__ ps,
                                   double t) {
+    asm volatile("" ::: "memory");
+    const __attribute__((address_space(4))) T* A =
+        (const __attribute__((address_space(4))) T*)ps;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       T acc = p[i];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) acc = fma(ps[i * 16 + j], u[j], acc);
+      for (int j = 0; j < 16; ++j) acc = fma(A[i * 16 + j], u[j], acc);
       du[i] = u[i] * acc;
     }
   }
