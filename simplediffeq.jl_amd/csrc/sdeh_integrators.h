@@ -61,20 +61,30 @@ struct Id {
 template <class T>
 struct Ops;
 
+// Ops<T>::Log / pw_from: x^y through one logarithm that several exponents can
+// share; pw_from(plog(x), y) == pw(x, y).
 template <>
 struct Ops<double> {
+  typedef sdeh_math::PowLog Log;
   __device__ static inline double div(double a, double b) { return a / b; }
   __device__ static inline double root(double x) { return __builtin_sqrt(x); }
   __device__ static inline double pw(double x, double y) { return sdeh_math::pow_ctrl(x, y); }
+  __device__ static inline Log plog(double x) { return sdeh_math::pow_log(x); }
+  __device__ static inline double pw_from(const Log& l, double y) {
+    return sdeh_math::pow_from(l, y);
+  }
 };
 
 template <>
 struct Ops<float> {
+  typedef float Log;
   __device__ static inline float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   __device__ static inline float root(float x) { return __builtin_amdgcn_sqrtf(x); }
   __device__ static inline float pw(float x, float y) {
     return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
   }
+  __device__ static inline Log plog(float x) { return __builtin_amdgcn_logf(x); }
+  __device__ static inline float pw_from(Log l, float y) { return __builtin_amdgcn_exp2f(y * l); }
 };
 
 // ---------------------------------------------------------------------------
@@ -672,7 +682,10 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
         if (EEst != EEst) {
           ret = RET_NAN;
         } else {
-          const T q11 = Ops<T>::pw(EEst, beta1);
+          // EEst^beta1 now, and EEst^beta2 as the next step's qold^beta2 when qold
+          // becomes EEst: one shared logarithm (same bits as two pow calls).
+          const typename Ops<T>::Log lg = Ops<T>::plog(EEst);
+          const T q11 = Ops<T>::pw_from(lg, beta1);
           if (EEst > T(1)) {
             dt = Ops<T>::div(dt, sdeh_math::jl_min(inv_qmin, Ops<T>::div(q11, gamma)));
             ++nrej;
@@ -680,7 +693,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
             T q = (EEst == T(0)) ? inv_qmax : Ops<T>::div(q11, qold_pow);
             q = sdeh_math::fast_max(inv_qmax, sdeh_math::fast_min(inv_qmin, Ops<T>::div(q, gamma)));
             qold = sdeh_math::jl_max(EEst, qoldinit);
-            qold_pow = Ops<T>::pw(qold, beta2);
+            qold_pow = (qold == EEst) ? Ops<T>::pw_from(lg, beta2) : qoldinit_pow;
             const T dtold = dt;
             dt = Ops<T>::div(dt, q);
             const double rem = tf - t - (double)dtold;
@@ -730,7 +743,10 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
         if (EEst != EEst) {
           ret = RET_NAN;
         } else {
-          const T q11 = Ops<T>::pw(EEst, beta1);
+          // EEst^beta1 now, and EEst^beta2 as the next step's qold^beta2 when qold
+          // becomes EEst: one shared logarithm (same bits as two pow calls).
+          const typename Ops<T>::Log lg = Ops<T>::plog(EEst);
+          const T q11 = Ops<T>::pw_from(lg, beta1);
           if (EEst > T(1)) {
             dt = Ops<T>::div(dt, sdeh_math::jl_min(inv_qmin, Ops<T>::div(q11, gamma)));
             ++nrej;
@@ -738,7 +754,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
             T q = (EEst == T(0)) ? inv_qmax : Ops<T>::div(q11, qold_pow);
             q = sdeh_math::fast_max(inv_qmax, sdeh_math::fast_min(inv_qmin, Ops<T>::div(q, gamma)));
             qold = sdeh_math::jl_max(EEst, qoldinit);
-            qold_pow = Ops<T>::pw(qold, beta2);
+            qold_pow = (qold == EEst) ? Ops<T>::pw_from(lg, beta2) : qoldinit_pow;
             const T dtold = dt;
             dt = Ops<T>::div(dt, q);
             const double rem = tf - t - (double)dtold;

@@ -153,19 +153,47 @@ SDEH_HD inline double exp_dd(double zh, double zl) {
   return (p * s2a) * s2b;
 }
 
-// x^y for the controller: x >= 0 (an RMS error norm), y > 0 a fixed exponent.
-// NaN x gives NaN, 0 gives 0, +Inf gives +Inf, negative x gives NaN.
-SDEH_HD inline double pow_ctrl(double x, double y) {
-  if (x != x) return x;
-  if (x == 0.0) return 0.0;
-  if (x < 0.0) return __builtin_nan("");
-  if (x == __builtin_inf()) return x;
-  if (x == 1.0) return 1.0;
-  double lh, ll;
-  log_pos_dd(x, &lh, &ll);
-  const double ph = y * lh;
-  const double pl = fma(y, lh, -ph) + y * ll;
+// x^y for the controller: x >= 0 (an RMS error norm), y > 0 a fixed exponent,
+// split as log then exp so one logarithm can serve several exponents (the
+// controller raises the same error estimate to beta1 now and to beta2 for the
+// next step).  pow_from(pow_log(x), y) == pow_ctrl(x, y) bit for bit.
+// Special values: NaN x gives NaN, 0 gives 0, +Inf gives +Inf, negative x
+// gives NaN, 1 gives 1.
+struct PowLog {
+  double hi, lo;
+  double special;  // result for every y when is_special
+  bool is_special;
+};
+
+SDEH_HD inline PowLog pow_log(double x) {
+  PowLog r;
+  r.hi = r.lo = 0.0;
+  r.is_special = true;
+  if (x != x) {
+    r.special = x;
+  } else if (x == 0.0) {
+    r.special = 0.0;
+  } else if (x < 0.0) {
+    r.special = __builtin_nan("");
+  } else if (x == __builtin_inf()) {
+    r.special = x;
+  } else if (x == 1.0) {
+    r.special = 1.0;
+  } else {
+    r.is_special = false;
+    r.special = 0.0;
+    log_pos_dd(x, &r.hi, &r.lo);
+  }
+  return r;
+}
+
+SDEH_HD inline double pow_from(const PowLog& l, double y) {
+  if (l.is_special) return l.special;
+  const double ph = y * l.hi;
+  const double pl = fma(y, l.hi, -ph) + y * l.lo;
   return exp_dd(ph, pl);
 }
+
+SDEH_HD inline double pow_ctrl(double x, double y) { return pow_from(pow_log(x), y); }
 
 }  // namespace sdeh_math

@@ -206,12 +206,23 @@ extern "C" SDEH_HIDDEN int sdeh_mlp_forward_impl(const float* params, const floa
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-// `scratch` holds min(ceil(n/64), 1024) * (NPARAM + 1) floats.
+// Blocks of the gradient pass: one 64-lane block per 64 samples, at most 1024
+// (the blocks then stride over the rest).  The partial sums need
+// sdeh_mlp_grad_scratch_bytes(n) bytes of scratch.
+static int64_t grad_blocks(int64_t n) {
+  int64_t nb = (n + 63) / 64;
+  return nb > 1024 ? 1024 : (nb < 1 ? 1 : nb);
+}
+
+extern "C" SDEH_HIDDEN size_t sdeh_mlp_grad_scratch_bytes(int64_t n) {
+  return (size_t)grad_blocks(n) * (NPARAM + 1) * sizeof(float);
+}
+
+// `scratch` holds sdeh_mlp_grad_scratch_bytes(n) bytes.
 extern "C" SDEH_HIDDEN int sdeh_mlp_loss_grad_impl(const float* params, const float* x,
                                                    const float* y, int64_t n, float* loss,
                                                    float* grad, float* scratch, hipStream_t s) {
-  int64_t nb = (n + 63) / 64;
-  if (nb > 1024) nb = 1024;
+  const int64_t nb = grad_blocks(n);
   hipLaunchKernelGGL(mlp_grad_partial, dim3((unsigned)nb), dim3(64), 0, s, params, x, y, n,
                      scratch);
   if (hipGetLastError() != hipSuccess) return -2;

@@ -45,6 +45,7 @@ SDEH_HIDDEN hipError_t sdeh_util_check_order(const int64_t*, int64_t, int32_t*, 
 SDEH_HIDDEN int sdeh_mlp_forward_impl(const float*, const float*, float*, int64_t, hipStream_t);
 SDEH_HIDDEN int sdeh_mlp_loss_grad_impl(const float*, const float*, const float*, int64_t, float*,
                                         float*, float*, hipStream_t);
+SDEH_HIDDEN size_t sdeh_mlp_grad_scratch_bytes(int64_t);
 SDEH_HIDDEN int sdeh_mlp_adam_impl(float*, const float*, float*, float*, float, float, float, float,
                                    int32_t, hipStream_t);
 }
@@ -963,10 +964,9 @@ int sdeh_mlp_loss_grad(sdeh_ctx* c, const float* params, const float* x, const f
   if (!r) r = check_dev(c, loss, 4, "loss");
   if (!r) r = check_dev(c, grad, SDEH_MLP_NPARAM * 4, "grad");
   if (r) return r;
-  // scratch for per-block partial gradients lives in the arena
-  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
-  const size_t need = (size_t)blocks * (SDEH_MLP_NPARAM + 1) * 4;
-  r = ensure_arena(c, need);
+  // scratch for per-block partial gradients lives in the arena; its size comes
+  // from the MLP unit, which owns the launch shape
+  r = ensure_arena(c, sdeh_mlp_grad_scratch_bytes(n));
   if (r) return r;
   r = sdeh_mlp_loss_grad_impl(params, x, y, n, loss, grad, (float*)c->arena,
                               stream ? (hipStream_t)stream : c->stream);

@@ -22,31 +22,32 @@ __global__ void zero_u64(unsigned long long* p, int n) {
 }
 
 // out[0] += sum(naccept + nreject), out[1] += sum(naccept),
-// out[2] += count(retcode == 1), out[3] += count(retcode == 4)
+// out[2] += count(retcode == 1), out[3] += count(retcode == 4).
+// Grid-stride loop, wave reduction, block reduction through LDS, then one
+// atomic per block per counter (few blocks, so little contention).
 __global__ void __launch_bounds__(256) count_kernel(const int32_t* nacc, const int32_t* nrej,
                                                     const int32_t* ret, int64_t n,
                                                     unsigned long long* out) {
-  unsigned long long att = 0, acc = 0, dtmin = 0, ovf = 0;
+  __shared__ unsigned long long part[4][4];
+  unsigned long long v[4] = {0, 0, 0, 0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const unsigned long long a = (unsigned long long)nacc[i];
-    acc += a;
-    att += a + (unsigned long long)nrej[i];
-    dtmin += ret[i] == 1;
-    ovf += ret[i] == 4;
+    v[1] += a;
+    v[0] += a + (unsigned long long)nrej[i];
+    v[2] += ret[i] == 1;
+    v[3] += ret[i] == 4;
   }
-  // wave reduction, then one atomic per wave per counter
-  for (int off = 32; off > 0; off >>= 1) {
-    att += __shfl_down(att, off);
-    acc += __shfl_down(acc, off);
-    dtmin += __shfl_down(dtmin, off);
-    ovf += __shfl_down(ovf, off);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(out + 0, att);
-    atomicAdd(out + 1, acc);
-    atomicAdd(out + 2, dtmin);
-    atomicAdd(out + 3, ovf);
+  for (int off = 32; off > 0; off >>= 1)
+    for (int k = 0; k < 4; ++k) v[k] += __shfl_down(v[k], off);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 4; ++k) part[w][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long s = 0;
+    for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) s += part[ww][threadIdx.x];
+    atomicAdd(out + threadIdx.x, s);
   }
 }
 
@@ -96,7 +97,9 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_count(
     const int32_t* nacc, const int32_t* nrej, const int32_t* ret, int64_t n,
     unsigned long long* out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(count_kernel, dim3(grid_for(n)), dim3(256), 0, s, nacc, nrej, ret, n, out);
+  int g = grid_for(n);
+  if (g > 512) g = 512;
+  hipLaunchKernelGGL(count_kernel, dim3(g), dim3(256), 0, s, nacc, nrej, ret, n, out);
   return hipGetLastError();
 }
 
