@@ -36,6 +36,13 @@ N_TRAJ = 1 << 20
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the binary64 headline kernel at 2^20 trajectories, from
+# two rocprofv3 --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from
+# a 256 MiB device copy in the same run): 49,307 KiB read + 75,835 KiB written.
+# Counters cannot be read from inside a timed run, so the figure applies only to the
+# configuration it was measured on.
+MEASURED_TRAFFIC = {("f64", 1 << 20): 128.14e6}
+
 # Algorithmic FLOPs (per component per attempt, RHS calls per attempt) by solver.
 STAGE_FLOPS = {0: 48, 1: 67, 2: 83, 3: 117, 4: 167, 5: 211}
 RHS_PER_ATTEMPT = {0: 6, 1: 6, 2: 9, 3: 9, 4: 15, 5: 15}
@@ -311,7 +318,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(trajectories are independent, see DESIGN.md for how the ensemble shards)"
+              "(trajectories are independent: an N-GPU run gives each GPU a contiguous slice)"
               % args.gpus, file=sys.stderr)
         return 2
 
@@ -401,7 +408,7 @@ def main():
                    "u0": "U(-1,1)^3", "launch": r64["shape"]},
         "roofline": {"bound": "valu", "achieved": r64["tflops"], "peak": PEAK_TFLOPS["f64"],
                      "unit": "TFLOP/s", "frac": r64["tflops"] / PEAK_TFLOPS["f64"],
-                     "traffic": None,
+                     "traffic": MEASURED_TRAFFIC.get(("f64", n)),
                      "kernel": "sdeh_adaptive_kernel<double, Lorenz, ATsit5, endpoint>",
                      "kernel_ms": r64["kernel_ms"], "algorithmic_flops_per_launch": r64["flops"],
                      "attempts_per_launch": r64["attempts"],
