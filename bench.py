@@ -311,6 +311,9 @@ def main():
                     help="run one 256 MiB device copy first (reference kernel for the HBM "
                          "byte counters)")
     ap.add_argument("--tune", action="store_true", help="autotune the launch shape first")
+    ap.add_argument("--grid", type=int, default=0,
+                    help="persistent schedule: workgroups of 256 (0 = library default, "
+                         "CUs x occupancy)")
     ap.add_argument("--workload", default="lorenz-wide",
                     choices=["lorenz-wide", "lorenz-small", "lorenz-uneven", "glv16", "costmodel"],
                     help="lorenz-wide is the headline line; the others report their own JSON")
@@ -353,7 +356,7 @@ def main():
     for dtype_name, dtype in (("f64", _abi.F64), ("f32", _abi.F32)):
         if dtype_name == "f32" and args.no_f32:
             continue
-        d = runtime.make_desc(dtype=dtype, n_traj=n, **common)
+        d = runtime.make_desc(dtype=dtype, n_traj=n, grid=args.grid, **common)
         bufs = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device=dev)
         torch.cuda.synchronize()
         if args.tune:

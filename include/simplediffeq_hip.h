@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define SDEH_ABI_VERSION 1
+#define SDEH_ABI_VERSION 2
 
 typedef enum sdeh_status {
   SDEH_OK = 0,
@@ -124,8 +124,14 @@ typedef struct sdeh_solve_desc {
   int32_t precision;        /* sdeh_precision */
   int32_t block;            /* 0 = tuned; else workgroup size (multiple of 64, <= 1024) */
   int32_t grid;             /* 0 = tuned; persistent schedule: number of workgroups */
-  int32_t chunk;            /* 0 = tuned; persistent schedule: trajectories per queue grab */
-  int32_t reserved[4];
+  int32_t chunk;            /* 0 = tuned; persistent schedule: at most this many
+                               trajectories per queue grab (grabs shrink near the end) */
+  int32_t lanes;            /* lanes per trajectory: 0 = for binary64, the right-hand
+                               side's group layout where it has one (GLV16: 16 lanes,
+                               one component each), else 1; 1 = one lane holds a whole
+                               trajectory; or the group size itself.  Results are
+                               identical. */
+  int32_t reserved[3];
 } sdeh_solve_desc;
 
 /* Solve buffers.  With memory == SDEH_MEM_HOST these are host pointers and the
@@ -159,7 +165,7 @@ typedef struct sdeh_stats {
   int64_t attempts;        /* sum over trajectories of naccept + nreject */
   int64_t accepts;
   int32_t block, grid, chunk, schedule; /* launch shape actually used */
-  int32_t vgprs;           /* reserved (0) */
+  int32_t lanes;           /* lanes per trajectory used */
 } sdeh_stats;
 
 typedef struct sdeh_ctx sdeh_ctx;

@@ -140,6 +140,10 @@ int oracle_solve(const sdeh_solve_desc* d, const sdeh_io* io, int arith, int pow
 
 double oracle_pow_ctrl(double x, double y) { return sdeh_math::pow_ctrl(x, y); }
 
+// The pow's two halves, for their own accuracy tests.
+void oracle_log_dd(double x, double* hi_lo) { sdeh_math::log_pos_dd(x, &hi_lo[0], &hi_lo[1]); }
+double oracle_exp_dd(double zh, double zl) { return sdeh_math::exp_dd(zh, zl); }
+
 // Julia range restatement: length and the first min(len, cap) elements.
 int64_t oracle_range(double start, double step, double stop, double* elems, int64_t cap,
                      int32_t* rational) {

@@ -88,6 +88,38 @@ struct Ops<float> {
 };
 
 // ---------------------------------------------------------------------------
+// Lane layout.  A right-hand side type R declares G, its lanes per trajectory.
+// G = 1: a lane holds a whole trajectory (R::N components).  G > 1 (a power of
+// two dividing 64): the G adjacent lanes of a group hold one trajectory, lane c
+// holding component c (R::N == 1 per lane, G components in all), and R::f
+// exchanges the state across the group through LDS.  The stage, error and
+// dense-output arithmetic is the same code in both layouts; only loads, stores,
+// the error norm and work distribution look at G.
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ __forceinline__ int lane_comp() {
+  return R::G == 1 ? 0 : (int)(threadIdx.x & (unsigned)(R::G - 1));
+}
+
+// all[j] = v of lane j of this lane's group (j = 0..G-1), for lanes in uniform
+// control flow within their group.  One LDS slot per thread of the workgroup
+// (blocks of at most 256 threads); a wave's LDS accesses complete in program
+// order, and the wave-scope fences keep the compiler from moving them.
+template <class T, int G>
+__device__ __forceinline__ void group_gather(T v, T (&all)[G]) {
+  __shared__ T xbuf[256];
+  const unsigned base = threadIdx.x & ~(unsigned)(G - 1);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  xbuf[threadIdx.x] = v;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int j = 0; j < G; ++j) all[j] = xbuf[base + j];
+}
+
+// ---------------------------------------------------------------------------
 // Linear combinations over stage vectors (arrays held in registers).
 // ---------------------------------------------------------------------------
 template <class T, int N>
@@ -152,19 +184,32 @@ __device__ __forceinline__ double stage_t(double t, double c, double dt) {
   return fma(c, dt, t);
 }
 
-// tmp <- tmp / (abstol + max(|uprev|, |u|) * reltol); return ODE_DEFAULT_NORM(tmp).
-template <class T, int N>
-__device__ __forceinline__ T err_norm(T (&tmp)[N], const T (&uprev)[N], const T (&u)[N],
+// tmp <- tmp / (abstol + max(|uprev|, |u|) * reltol); return ODE_DEFAULT_NORM(tmp)
+// over all components of the trajectory (gathered from the group when G > 1; the
+// sum of squares runs over components 0, 1, ... in that order either way).
+template <class T, class R>
+__device__ __forceinline__ T err_norm(T (&tmp)[R::N], const T (&uprev)[R::N], const T (&u)[R::N],
                                       T abstol, T reltol) {
+  constexpr int N = R::N;
 #pragma unroll
   for (int c = 0; c < N; ++c) {
     const T m = sdeh_math::jl_max(fabs(uprev[c]), fabs(u[c]));
     tmp[c] = Ops<T>::div(tmp[c], fma(m, reltol, abstol));
   }
-  T s = tmp[0] * tmp[0];
+  if constexpr (R::G > 1) {
+    static_assert(N == 1, "a lane of a group holds one component");
+    T all[R::G];
+    group_gather<T, R::G>(tmp[0], all);
+    T s = all[0] * all[0];
 #pragma unroll
-  for (int c = 1; c < N; ++c) s = s + tmp[c] * tmp[c];
-  return Ops<T>::root(Ops<T>::div(s, T(N)));
+    for (int c = 1; c < R::G; ++c) s = s + all[c] * all[c];
+    return Ops<T>::root(Ops<T>::div(s, T(R::G)));
+  } else {
+    T s = tmp[0] * tmp[0];
+#pragma unroll
+    for (int c = 1; c < N; ++c) s = s + tmp[c] * tmp[c];
+    return Ops<T>::root(Ops<T>::div(s, T(N)));
+  }
 }
 
 // Horner evaluation of evalpoly(θ, (0, r...)) / evalpoly(θ, (0, 0, r...)).
@@ -519,12 +564,35 @@ __device__ __forceinline__ void vern9_dense(T (&out)[N], const T (&uprev)[N], T 
 // ---------------------------------------------------------------------------
 // Output helpers (structure-of-arrays, component major).
 // ---------------------------------------------------------------------------
-template <class T, int N>
+// Row `row` of a [rows][components][n] output: this lane's components of `traj`.
+template <class T, class R>
 __device__ __forceinline__ void store_state(void* base, int64_t row, int64_t n, int64_t traj,
-                                            const T (&u)[N]) {
+                                            const T (&u)[R::N]) {
+  constexpr int FULL = R::N * R::G;
+  const int c0 = lane_comp<R>() * R::N;
   T* o = (T*)base;
 #pragma unroll
-  for (int c = 0; c < N; ++c) o[(row * N + c) * n + traj] = u[c];
+  for (int c = 0; c < R::N; ++c) o[(row * FULL + c0 + c) * n + traj] = u[c];
+}
+
+template <class T, class R>
+__device__ __forceinline__ void load_state(T (&u)[R::N], const void* base, int64_t n,
+                                           int64_t traj) {
+  const int c0 = lane_comp<R>() * R::N;
+#pragma unroll
+  for (int c = 0; c < R::N; ++c) u[c] = ((const T*)base)[(c0 + c) * n + traj];
+}
+
+// Per-lane parameters: p[c][traj] for c < R::P, or the right-hand side's own
+// lane loader in a group layout.
+template <class T, class R, int P>
+__device__ __forceinline__ void load_params(T (&pl)[P], const sdeh_kargs& a, int64_t traj) {
+  if constexpr (R::G > 1) {
+    R::template load_lane<T>(pl, (const T*)a.p, (const T*)a.ps, a.n, traj, lane_comp<R>());
+  } else {
+#pragma unroll
+    for (int c = 0; c < R::P; ++c) pl[c] = ((const T*)a.p)[c * a.n + traj];
+  }
 }
 
 template <class T>
@@ -542,13 +610,16 @@ __device__ __forceinline__ unsigned lane_rank(unsigned long long mask) {
 
 // Claim queue slots for the lanes flagged in `need` (called in uniform control
 // flow by the whole wave).  Returns this lane's slot or -1.  `base`/`lim` is
-// the wave's current claimed range; `drained` becomes true once the queue is
-// exhausted.
+// the wave's current claimed range of `chunk` slots; `drained` becomes true once
+// the queue is exhausted.  (Claims that shrink as the queue empties balance the
+// end of the launch better on paper but measured slower: each extra claim is a
+// device-scope atomic round trip the wave waits for.)
 __device__ __forceinline__ int64_t wave_claim(const sdeh_kargs& a, bool need_me, int64_t& base,
                                               int64_t& lim, bool& drained) {
   int64_t mine = -1;
   unsigned long long need = __ballot(need_me);
   while (need != 0ull && !drained) {
+    const int cnt = __popcll(need);
     if (base >= lim) {
       unsigned long long old = 0;
       if (__lane_id() == 0) old = atomicAdd(a.queue, (unsigned long long)a.chunk);
@@ -561,7 +632,6 @@ __device__ __forceinline__ int64_t wave_claim(const sdeh_kargs& a, bool need_me,
       }
     }
     const int64_t avail = lim - base;
-    const int cnt = __popcll(need);
     const bool in_need = ((need >> __lane_id()) & 1ull) != 0ull;
     const int64_t rank = (int64_t)lane_rank(need);
     if (in_need && rank < avail) mine = base + rank;
@@ -600,8 +670,9 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
 
   int64_t qbase = 0, qlim = 0;
   bool drained = !a.persistent;
+  const bool lead = lane_comp<R>() == 0;  // writes the per-trajectory scalars
   if (!a.persistent) {
-    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / R::G;
     if (slot < n) {
       traj = a.order ? a.order[slot] : slot;
       active = true;
@@ -611,7 +682,9 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
 
   for (;;) {
     if (a.persistent) {
-      const int64_t slot = wave_claim(a, !active, qbase, qlim, drained);
+      int64_t slot = wave_claim(a, !active && lead, qbase, qlim, drained);
+      if constexpr (R::G > 1)  // the group leader claimed; its group follows
+        slot = (int64_t)__shfl((long long)slot, (int)(__lane_id() & ~(unsigned)(R::G - 1)));
       if (slot >= 0) {
         traj = a.order ? a.order[slot] : slot;
         active = true;
@@ -623,13 +696,10 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
 
     if (fresh) {  // start trajectory `traj` (gpuatsit5.jl:221-241)
       fresh = false;
+      load_state<T, R>(uprev, a.u0, n, traj);
 #pragma unroll
-      for (int c = 0; c < N; ++c) {
-        uprev[c] = ((const T*)a.u0)[c * n + traj];
-        comp[c] = T(0);
-      }
-#pragma unroll
-      for (int c = 0; c < R::P; ++c) pl[c] = ((const T*)a.p)[c * n + traj];
+      for (int c = 0; c < N; ++c) comp[c] = T(0);
+      load_params<T, R>(pl, a, traj);
       t = a.tspan ? a.tspan[traj] : a.t0;
       tf = a.tspan ? a.tspan[n + traj] : a.tf;
       dt = (T)(a.dt0 ? a.dt0[traj] : a.dt);
@@ -641,14 +711,14 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
       nsaved = 0;
       if constexpr (SAVE == SAVE_AT) {
         if (a.n_saveat > 0 && t == a.saveat[0]) {
-          store_state<T, N>(a.u_saveat, 0, n, traj, uprev);
+          store_state<T, R>(a.u_saveat, 0, n, traj, uprev);
           cur = 1;
         }
       }
       if constexpr (SAVE == SAVE_STEPS) {
         if (a.max_saves > 0) {
-          store_state<T, N>(a.u_steps, 0, n, traj, uprev);
-          a.t_steps[traj] = t;
+          store_state<T, R>(a.u_steps, 0, n, traj, uprev);
+          if (lead) a.t_steps[traj] = t;
         }
         nsaved = 1;
       }
@@ -665,9 +735,12 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
     } else {
       ++attempts;
       T u[N], err[N], comp_new[N];
-      // ---- one attempt ----
+      // ---- one attempt: stages, trial state u and the embedded error ----
+      constexpr bool V9 = (ALG == ALG_AVERN9);
+      T k2[N], k3[N], k4[N], k5[N], k6[N], k7[N];  // ATsit5 stages (FSAL k1 = fsal)
+      V7K<T, N> K7;                                 // AVern7 stages
+      V9S<T, N> S9;                                 // AVern9 stages
       if constexpr (ALG == ALG_ATSIT5) {
-        T k2[N], k3[N], k4[N], k5[N], k6[N], k7[N];
         tsit5_stages<T, R, KAHAN>(uprev, fsal, k2, k3, k4, k5, k6, k7, u, comp, comp_new, pl,
                                   ps, t, dt);
         // dt * (btilde1*k1 + ... + btilde7*k7), gpuatsit5.jl:272-275
@@ -675,131 +748,97 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
              T(sdeh_tab::tsit5::btilde3), k3, T(sdeh_tab::tsit5::btilde4), k4,
              T(sdeh_tab::tsit5::btilde5), k5, T(sdeh_tab::tsit5::btilde6), k6,
              T(sdeh_tab::tsit5::btilde7), k7);
+      } else if constexpr (V9) {
+        // Verner: k1 is evaluated at the start of every attempt (gpuvern7.jl:360,
+        // gpuvern9.jl:468).
+        vern9_stages<T, R, true, KAHAN, SAVE == SAVE_AT>(uprev, S9, u, err, comp, comp_new, pl,
+                                                          ps, t, dt, false);
+      } else {
+        vern7_stages<T, R, true, KAHAN>(uprev, K7, u, err, comp, comp_new, pl, ps, t, dt);
+      }
 #pragma unroll
-        for (int c = 0; c < N; ++c) err[c] = dt * err[c];
-        const T EEst = err_norm(err, uprev, u, abstol, reltol);
-        // ---- PI controller, gpuatsit5.jl:279-299 ----
-        if (EEst != EEst) {
-          ret = RET_NAN;
+      for (int c = 0; c < N; ++c) err[c] = dt * err[c];
+      const T EEst = err_norm<T, R>(err, uprev, u, abstol, reltol);
+      if (EEst != EEst) {
+        ret = RET_NAN;
+      } else {
+        // ---- PI controller: gpuatsit5.jl:279-299, gpuvern7.jl:406-426,
+        // gpuvern9.jl:575-595 ----
+        // Evaluated for every lane without branching on accept/reject: the reject
+        // step dt / min(inv_qmin, q11/gamma) and the accept step
+        // dt / clamp((q11/qold^beta2)/gamma) share their three divisions, a reject
+        // dividing q11 by 1 (exact), and EEst == 0 dividing inv_qmax by 1.
+        // EEst^beta1 now and EEst^beta2 (the next step's qold^beta2 when qold becomes
+        // EEst) share one logarithm: same bits as two pow calls.
+        const typename Ops<T>::Log lg = Ops<T>::plog(EEst);
+        const T q11 = Ops<T>::pw_from(lg, beta1);
+        const T e_b2 = Ops<T>::pw_from(lg, beta2);
+        const bool accept = !(EEst > T(1));
+        const bool zero = (EEst == T(0));
+        const T num = (accept && zero) ? inv_qmax : q11;
+        const T den = (accept && !zero) ? qold_pow : T(1);
+        const T qg = Ops<T>::div(Ops<T>::div(num, den), gamma);
+        const T q = accept ? sdeh_math::fast_max(inv_qmax, sdeh_math::fast_min(inv_qmin, qg))
+                           : sdeh_math::jl_min(inv_qmin, qg);
+        const T dtq = Ops<T>::div(dt, q);
+        if (!accept) {
+          dt = dtq;
+          ++nrej;
         } else {
-          // EEst^beta1 now, and EEst^beta2 as the next step's qold^beta2 when qold
-          // becomes EEst: one shared logarithm (same bits as two pow calls).
-          const typename Ops<T>::Log lg = Ops<T>::plog(EEst);
-          const T q11 = Ops<T>::pw_from(lg, beta1);
-          if (EEst > T(1)) {
-            dt = Ops<T>::div(dt, sdeh_math::jl_min(inv_qmin, Ops<T>::div(q11, gamma)));
-            ++nrej;
-          } else {
-            T q = (EEst == T(0)) ? inv_qmax : Ops<T>::div(q11, qold_pow);
-            q = sdeh_math::fast_max(inv_qmax, sdeh_math::fast_min(inv_qmin, Ops<T>::div(q, gamma)));
-            qold = sdeh_math::jl_max(EEst, qoldinit);
-            qold_pow = (qold == EEst) ? Ops<T>::pw_from(lg, beta2) : qoldinit_pow;
-            const T dtold = dt;
-            dt = Ops<T>::div(dt, q);
-            const double rem = tf - t - (double)dtold;
-            dt = (T)sdeh_math::jl_min((double)fabs(dt), fabs(rem));
-            const double told = t;
-            t = (rem < SNAP) ? tf : t + (double)dtold;
-            ++nacc;
-            if constexpr (SAVE == SAVE_AT) {  // :305-316
+          qold = sdeh_math::jl_max(EEst, qoldinit);
+          qold_pow = (qold == EEst) ? e_b2 : qoldinit_pow;
+          const T dtold = dt;
+          const double rem = tf - t - (double)dtold;
+          dt = (T)sdeh_math::jl_min((double)fabs(dtq), fabs(rem));
+          const double told = t;
+          t = (rem < SNAP) ? tf : t + (double)dtold;
+          ++nacc;
+          if constexpr (SAVE == SAVE_AT) {
+            if constexpr (ALG == ALG_ATSIT5) {  // gpuatsit5.jl:305-316
               while (cur < a.n_saveat && a.saveat[cur] <= t) {
                 const T th = (T)((a.saveat[cur] - told) / (double)dtold);
                 T o[N];
                 tsit5_dense(o, uprev, dtold, th, fsal, k2, k3, k4, k5, k6, k7);
-                store_state<T, N>(a.u_saveat, cur, n, traj, o);
+                store_state<T, R>(a.u_saveat, cur, n, traj, o);
                 ++cur;
               }
-            }
-            if constexpr (SAVE == SAVE_STEPS) {
-              if (nsaved < a.max_saves) {
-                store_state<T, N>(a.u_steps, nsaved, n, traj, u);
-                a.t_steps[(int64_t)nsaved * n + traj] = t;
-              }
-              ++nsaved;
-            }
-            copyv(uprev, u);
-            copyv(fsal, k7);
-            if constexpr (KAHAN) copyv(comp, comp_new);
-            if (!(t < tf)) ret = RET_DEFAULT;
-          }
-        }
-      } else {
-        // Verner: k1 is evaluated at the start of every attempt (gpuvern7.jl:360,
-        // gpuvern9.jl:468).
-        constexpr bool V9 = (ALG == ALG_AVERN9);
-        constexpr int NS = V9 ? 9 : 7;
-        (void)NS;
-        V7K<T, N> K7;
-        V9S<T, N> S9;
-        if constexpr (V9) {
-          vern9_stages<T, R, true, KAHAN, SAVE == SAVE_AT>(uprev, S9, u, err, comp, comp_new, pl,
-                                                            ps, t, dt, false);
-        } else {
-          vern7_stages<T, R, true, KAHAN>(uprev, K7, u, err, comp, comp_new, pl, ps, t, dt);
-        }
-#pragma unroll
-        for (int c = 0; c < N; ++c) err[c] = dt * err[c];
-        const T EEst = err_norm(err, uprev, u, abstol, reltol);
-        if (EEst != EEst) {
-          ret = RET_NAN;
-        } else {
-          // EEst^beta1 now, and EEst^beta2 as the next step's qold^beta2 when qold
-          // becomes EEst: one shared logarithm (same bits as two pow calls).
-          const typename Ops<T>::Log lg = Ops<T>::plog(EEst);
-          const T q11 = Ops<T>::pw_from(lg, beta1);
-          if (EEst > T(1)) {
-            dt = Ops<T>::div(dt, sdeh_math::jl_min(inv_qmin, Ops<T>::div(q11, gamma)));
-            ++nrej;
-          } else {
-            T q = (EEst == T(0)) ? inv_qmax : Ops<T>::div(q11, qold_pow);
-            q = sdeh_math::fast_max(inv_qmax, sdeh_math::fast_min(inv_qmin, Ops<T>::div(q, gamma)));
-            qold = sdeh_math::jl_max(EEst, qoldinit);
-            qold_pow = (qold == EEst) ? Ops<T>::pw_from(lg, beta2) : qoldinit_pow;
-            const T dtold = dt;
-            dt = Ops<T>::div(dt, q);
-            const double rem = tf - t - (double)dtold;
-            dt = (T)sdeh_math::jl_min((double)fabs(dt), fabs(rem));
-            const double told = t;
-            t = (rem < SNAP) ? tf : t + (double)dtold;
-            ++nacc;
-            if constexpr (SAVE == SAVE_AT) {
-              if (cur < a.n_saveat && a.saveat[cur] <= t) {
-                // Extra stages depend only on the step, not on θ: computed once.
-                if constexpr (V9) {
-                  T x[10][N];
-                  vern9_extra<T, R>(uprev, S9, x, pl, ps, told, dtold);  // told + c*dtold
-                  while (cur < a.n_saveat && a.saveat[cur] <= t) {
-                    const T th = (T)((a.saveat[cur] - told) / (double)dtold);
-                    T o[N];
-                    vern9_dense(o, uprev, dtold, th, S9, x);
-                    store_state<T, N>(a.u_saveat, cur, n, traj, o);
-                    ++cur;
-                  }
-                } else {
-                  T x[6][N];
-                  // Quirk (gpuvern7.jl:449): stage times from the post-step t.
-                  vern7_extra<T, R>(uprev, K7, x, pl, ps, a.quirks ? t : told, dtold);
-                  while (cur < a.n_saveat && a.saveat[cur] <= t) {
-                    const T th = (T)((a.saveat[cur] - told) / (double)dtold);
-                    T o[N];
-                    vern7_dense(o, uprev, dtold, th, K7, x);
-                    store_state<T, N>(a.u_saveat, cur, n, traj, o);
-                    ++cur;
-                  }
+            } else if (cur < a.n_saveat && a.saveat[cur] <= t) {
+              // Extra stages depend only on the step, not on θ: computed once.
+              if constexpr (V9) {
+                T x[10][N];
+                vern9_extra<T, R>(uprev, S9, x, pl, ps, told, dtold);  // told + c*dtold
+                while (cur < a.n_saveat && a.saveat[cur] <= t) {
+                  const T th = (T)((a.saveat[cur] - told) / (double)dtold);
+                  T o[N];
+                  vern9_dense(o, uprev, dtold, th, S9, x);
+                  store_state<T, R>(a.u_saveat, cur, n, traj, o);
+                  ++cur;
+                }
+              } else {
+                T x[6][N];
+                // Quirk (gpuvern7.jl:449): stage times from the post-step t.
+                vern7_extra<T, R>(uprev, K7, x, pl, ps, a.quirks ? t : told, dtold);
+                while (cur < a.n_saveat && a.saveat[cur] <= t) {
+                  const T th = (T)((a.saveat[cur] - told) / (double)dtold);
+                  T o[N];
+                  vern7_dense(o, uprev, dtold, th, K7, x);
+                  store_state<T, R>(a.u_saveat, cur, n, traj, o);
+                  ++cur;
                 }
               }
             }
-            if constexpr (SAVE == SAVE_STEPS) {
-              if (nsaved < a.max_saves) {
-                store_state<T, N>(a.u_steps, nsaved, n, traj, u);
-                a.t_steps[(int64_t)nsaved * n + traj] = t;
-              }
-              ++nsaved;
-            }
-            copyv(uprev, u);
-            if constexpr (KAHAN) copyv(comp, comp_new);
-            if (!(t < tf)) ret = RET_DEFAULT;
           }
+          if constexpr (SAVE == SAVE_STEPS) {
+            if (nsaved < a.max_saves) {
+              store_state<T, R>(a.u_steps, nsaved, n, traj, u);
+              if (lead) a.t_steps[(int64_t)nsaved * n + traj] = t;
+            }
+            ++nsaved;
+          }
+          copyv(uprev, u);
+          if constexpr (ALG == ALG_ATSIT5) copyv(fsal, k7);
+          if constexpr (KAHAN) copyv(comp, comp_new);
+          if (!(t < tf)) ret = RET_DEFAULT;
         }
       }
     }
@@ -807,13 +846,15 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
     if (ret >= 0) {  // trajectory ends: write outputs (gpuatsit5.jl:322-329)
       if (SAVE == SAVE_STEPS && ret == RET_DEFAULT && nsaved > a.max_saves)
         ret = RET_SAVE_OVERFLOW;
-      store_state<T, N>(a.u_end, 0, n, traj, uprev);
-      a.t_end[traj] = t;
-      a.retcode[traj] = ret;
-      if (a.naccept) a.naccept[traj] = nacc;
-      if (a.nreject) a.nreject[traj] = nrej;
-      if constexpr (SAVE == SAVE_STEPS)
-        a.n_steps[traj] = nsaved < a.max_saves ? nsaved : a.max_saves;
+      store_state<T, R>(a.u_end, 0, n, traj, uprev);
+      if (lead) {
+        a.t_end[traj] = t;
+        a.retcode[traj] = ret;
+        if (a.naccept) a.naccept[traj] = nacc;
+        if (a.nreject) a.nreject[traj] = nrej;
+        if constexpr (SAVE == SAVE_STEPS)
+          a.n_steps[traj] = nsaved < a.max_saves ? nsaved : a.max_saves;
+      }
       active = false;
     }
   }
@@ -837,32 +878,30 @@ template <class T, class R, int ALG, int SAVE>
 __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
   constexpr int N = R::N;
   constexpr int P = R::P > 0 ? R::P : 1;
-  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= a.n) return;
+  const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / R::G;
+  if (slot >= a.n) return;  // whole groups leave together
+  const bool lead = lane_comp<R>() == 0;
   const int64_t n = a.n;
   const int64_t traj = a.order ? a.order[slot] : slot;
   const T* ps = (const T*)a.ps;
   T u[N], uprev[N], pl[P], zero[N], dummy[N], err[N];
+  load_state<T, R>(u, a.u0, n, traj);
 #pragma unroll
-  for (int c = 0; c < N; ++c) {
-    u[c] = ((const T*)a.u0)[c * n + traj];
-    zero[c] = T(0);
-  }
-#pragma unroll
-  for (int c = 0; c < R::P; ++c) pl[c] = ((const T*)a.p)[c * n + traj];
+  for (int c = 0; c < N; ++c) zero[c] = T(0);
+  load_params<T, R>(pl, a, traj);
   const T dt = (T)a.dt;
   double t = a.t0;
   int32_t cur = 0, nsaved = 0;
   if constexpr (SAVE == SAVE_AT) {
     if (a.n_saveat > 0 && t == a.saveat[0]) {
-      store_state<T, N>(a.u_saveat, 0, n, traj, u);
+      store_state<T, R>(a.u_saveat, 0, n, traj, u);
       cur = 1;
     }
   }
   if constexpr (SAVE == SAVE_STEPS) {
     if (a.max_saves > 0) {
-      store_state<T, N>(a.u_steps, 0, n, traj, u);
-      a.t_steps[traj] = t;
+      store_state<T, R>(a.u_steps, 0, n, traj, u);
+      if (lead) a.t_steps[traj] = t;
     }
     nsaved = 1;
   }
@@ -881,7 +920,7 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
           const T th = (T)((a.saveat[cur] - (t - (double)dt)) / (double)dt);
           T o[N];
           tsit5_dense(o, uprev, dt, th, k1, k2, k3, k4, k5, k6, k7);
-          store_state<T, N>(a.u_saveat, cur, n, traj, o);
+          store_state<T, R>(a.u_saveat, cur, n, traj, o);
           ++cur;
         }
       }
@@ -898,7 +937,7 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
             const T th = (T)((a.saveat[cur] - (t - (double)dt)) / (double)dt);
             T o[N];
             vern7_dense(o, uprev, dt, th, K, x);
-            store_state<T, N>(a.u_saveat, cur, n, traj, o);
+            store_state<T, R>(a.u_saveat, cur, n, traj, o);
             ++cur;
           }
         }
@@ -917,7 +956,7 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
             const T th = (T)((a.saveat[cur] - (t - (double)dt)) / (double)dt);
             T o[N];
             vern9_dense(o, uprev, dt, th, S, x);
-            store_state<T, N>(a.u_saveat, cur, n, traj, o);
+            store_state<T, R>(a.u_saveat, cur, n, traj, o);
             ++cur;
           }
         }
@@ -925,13 +964,14 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
     }
     if constexpr (SAVE == SAVE_STEPS) {
       if (nsaved < a.max_saves) {
-        store_state<T, N>(a.u_steps, nsaved, n, traj, u);
-        a.t_steps[(int64_t)nsaved * n + traj] = t;
+        store_state<T, R>(a.u_steps, nsaved, n, traj, u);
+        if (lead) a.t_steps[(int64_t)nsaved * n + traj] = t;
       }
       ++nsaved;
     }
   }
-  store_state<T, N>(a.u_end, 0, n, traj, u);
+  store_state<T, R>(a.u_end, 0, n, traj, u);
+  if (!lead) return;
   a.t_end[traj] = t;
   int ret = RET_DEFAULT;
   if (SAVE == SAVE_STEPS && nsaved > a.max_saves) ret = RET_SAVE_OVERFLOW;
@@ -943,10 +983,25 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
 
 }  // namespace sdeh
 
+namespace sdeh {
+// Waves per SIMD the compiler is asked to keep resident.  The binary64 adaptive
+// kernels of small states with endpoint output need a few VGPRs more than the 96
+// that five waves allow when left alone (ATsit5/Lorenz: 101-103); held to 96 they
+// lose nothing and run 6% faster (five waves instead of four).  Larger kernels
+// keep the compiler's choice (1 = no minimum).
+template <class T, class R, int ALG, int SAVE>
+struct WavesHint {
+  static constexpr int value =
+      (sizeof(T) == 8 && R::N * R::G <= 4 && SAVE == SAVE_END && ALG == ALG_ATSIT5) ? 5 : 1;
+};
+}  // namespace sdeh
+
 // Kernel entry templates.  Instantiated with extern "C" names by the
 // ahead-of-time translation unit and by the hipRTC wrapper.
 template <class T, class R, int ALG, int SAVE, bool KAHAN>
-__global__ void __launch_bounds__(256) sdeh_adaptive_kernel(sdeh_kargs a) {
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(sdeh::WavesHint<T, R, ALG, SAVE>::value)))
+    sdeh_adaptive_kernel(sdeh_kargs a) {
   sdeh::adaptive_body<T, R, ALG, SAVE, KAHAN>(a);
 }
 

@@ -37,6 +37,8 @@ using __hip_internal::uint32_t;
 using __hip_internal::uint64_t;
 #endif
 
+#include "sdeh_powtab.h"
+
 namespace sdeh_math {
 
 SDEH_HD inline uint64_t bits_of(double x) { return __builtin_bit_cast(uint64_t, x); }
@@ -44,19 +46,22 @@ SDEH_HD inline double double_of(uint64_t b) { return __builtin_bit_cast(double, 
 
 // Julia Base `min`/`max` on IEEE floats: NaN in either argument gives NaN, and
 // -0.0 orders below +0.0.
+// Written as a chain of selects (in order of precedence from the last one up:
+// NaN x, NaN y, equal values, the comparison) so that device code evaluates it
+// without branches.
 template <class T>
 SDEH_HD inline T jl_min(T x, T y) {
-  if (x != x) return x;
-  if (y != y) return y;
-  if (x == y) return __builtin_signbit(x) ? x : y;
-  return y < x ? y : x;
+  T r = y < x ? y : x;
+  r = x == y ? (__builtin_signbit(x) ? x : y) : r;
+  r = y != y ? y : r;
+  return x != x ? x : r;
 }
 template <class T>
 SDEH_HD inline T jl_max(T x, T y) {
-  if (x != x) return x;
-  if (y != y) return y;
-  if (x == y) return __builtin_signbit(x) ? y : x;
-  return y > x ? y : x;
+  T r = y > x ? y : x;
+  r = x == y ? (__builtin_signbit(x) ? y : x) : r;
+  r = y != y ? y : r;
+  return x != x ? x : r;
 }
 // Julia `@fastmath` min/max (Base.FastMath.min_fast/max_fast): a single
 // comparison select; the result with a NaN argument is not specified by the
@@ -66,99 +71,101 @@ SDEH_HD inline T fast_min(T x, T y) { return y < x ? y : x; }
 template <class T>
 SDEH_HD inline T fast_max(T x, T y) { return y > x ? y : x; }
 
-// Natural log of a finite x > 0 (subnormals included) as an unevaluated sum
-// hi + lo.  x = 2^k * m with m in [sqrt(1/2), sqrt(2)); f = m - 1 (exact);
-// s = f / (2 + f); log(1 + f) = 2 atanh(s) = f + c with
-// c = s (f^2/2 + R(s^2)) - f^2/2 and R(z) = sum_{j>=1} 2 z^j / (2j + 1)
-// (Taylor series, 11 terms, |z| <= 0.0295).  k ln 2 + f is summed exactly.
+// Natural log of a finite x > 0 (normal or subnormal) as an unevaluated sum
+// hi + lo, |error| < 2^-66 |log x| + 2^-90 (table-driven, Tang's construction):
+//   x = 2^k z, z in [0x1.6ap-1, 0x1.6ap0), the subinterval index i taken from
+//   the top 7 bits of the bit pattern of z relative to 0x1.6ap-1;
+//   r = z * invc_i - 1 is exact (invc_i has 8 significant bits, |r| < 2^-7);
+//   log x = k ln2 + lc_i + log1p(r), lc_i = -log(invc_i) = lh_i + ll_i;
+//   log1p(r) = r - r^2/2 + r^3 P(r), P the Taylor series 1/3 - r/4 + ... - r^6/9
+//   (truncation below |r|^10/10 < 2^-70).
+// The large terms are added with error-free transformations (Knuth's two-sum;
+// the fast form where the first operand dominates or is zero): k ln2_hi is exact
+// (ln2_hi has 42 significant bits, |k| < 2^11).
 SDEH_HD inline void log_pos_dd(double x, double* hi, double* lo) {
-  int k = 0;
-  if (x < 0x1p-1022) {  // subnormal: scale into the normal range
-    x *= 0x1p54;
-    k = -54;
-  }
-  uint64_t b = bits_of(x);
-  int e = (int)((b >> 52) & 0x7ff) - 1023;
-  double m = double_of((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
-  if (m > 0x1.6a09e667f3bcdp+0) {  // sqrt(2)
-    m *= 0.5;
-    e += 1;
-  }
-  k += e;
-  const double f = m - 1.0;
-  const double s = f / (2.0 + f);
-  const double z = s * s;
-  double r = 0x1.642c8590b2164p-4;          // 2/23
-  r = fma(r, z, 0x1.8618618618618p-4);      // 2/21
-  r = fma(r, z, 0x1.af286bca1af28p-4);      // 2/19
-  r = fma(r, z, 0x1.e1e1e1e1e1e1ep-4);      // 2/17
-  r = fma(r, z, 0x1.1111111111111p-3);      // 2/15
-  r = fma(r, z, 0x1.3b13b13b13b14p-3);      // 2/13
-  r = fma(r, z, 0x1.745d1745d1746p-3);      // 2/11
-  r = fma(r, z, 0x1.c71c71c71c71cp-3);      // 2/9
-  r = fma(r, z, 0x1.2492492492492p-2);      // 2/7
-  r = fma(r, z, 0x1.999999999999ap-2);      // 2/5
-  r = fma(r, z, 0x1.5555555555555p-1);      // 2/3
-  r = r * z;
-  const double hfsq = 0.5 * f * f;
-  const double c = fma(s, hfsq + r, -hfsq);
+  using namespace sdeh_powtab;
+  const bool sub = x < 0x1p-1022;
+  const double xs = sub ? x * 0x1p54 : x;
+  const uint64_t ix = bits_of(xs);
+  const uint64_t tmp = ix - LOG_OFF;
+  const int i = (int)((tmp >> LOG_SHIFT) & (uint64_t)LOG_MASK);
+  const int k = (int)((int64_t)tmp >> 52) - (sub ? 54 : 0);
+  const double z = double_of(ix - (tmp & 0xfff0000000000000ULL));
+  const double invc = LOG_TAB[i][0], lh = LOG_TAB[i][1], ll = LOG_TAB[i][2];
+  const double r = fma(z, invc, -1.0);  // exact
   const double dk = (double)k;
-  const double ln2_hi = 0x1.62e4200000000p-1;  // ln 2 with the low 32 bits zero
-  const double ln2_lo = 0x1.fdf473de6af28p-22;
-  const double a = dk * ln2_hi;  // exact: |k| < 2^11, ln2_hi has 21 significant bits
-  const double sum = a + f;      // two-sum: a + f == sum + err exactly
-  const double bb = sum - a;
-  const double err = (a - (sum - bb)) + (f - bb);
-  const double tail = err + fma(dk, ln2_lo, c);
-  const double h = sum + tail;
-  *hi = h;
-  *lo = tail - (h - sum);
+  const double a = dk * LN2_HI;  // exact
+  // s1 + e1 = a + lh (|a| >= ln 2 > |lh| unless k == 0)
+  const double s1 = a + lh;
+  const double e1 = (a - s1) + lh;
+  // s2 + e2 = s1 + r (two-sum)
+  const double s2 = s1 + r;
+  const double bb = s2 - s1;
+  const double e2 = (s1 - (s2 - bb)) + (r - bb);
+  // -r^2/2 = ar2 + e3 exactly
+  const double ar = -0.5 * r;
+  const double ar2 = r * ar;
+  const double e3 = fma(ar, r, -ar2);
+  // h + e4 = s2 + ar2 (|s2| >= |ar2|: s2 = r when k = 0 and invc = 1,
+  // otherwise |s2| > 2^-9 > r^2/2)
+  const double h = s2 + ar2;
+  const double e4 = (s2 - h) + ar2;
+  double p = -0x1.c71c71c71c71cp-4;      // -1/9
+  p = fma(p, r, 0x1p-3);                 //  1/8
+  p = fma(p, r, -0x1.2492492492492p-3);  // -1/7
+  p = fma(p, r, 0x1.5555555555555p-3);   //  1/6
+  p = fma(p, r, -0x1.999999999999ap-3);  // -1/5
+  p = fma(p, r, 0x1p-2);                 //  1/4
+  p = fma(p, r, -0x1.5555555555555p-2);  // -1/3
+  const double r3 = (r * r) * r;
+  const double t = fma(-r3, p, fma(dk, LN2_LO, ll)) + ((e1 + e2) + (e3 + e4));
+  const double y = h + t;
+  *hi = y;
+  *lo = (h - y) + t;
 }
 
-// e^(zh + zl) for finite zh, |zl| <= ulp(zh).  n = round(zh / ln 2);
-// r = zh - n ln 2 (exact for the high part of ln 2); e^r by its Taylor
-// polynomial of degree 13 (|r| <= 0.347), with 1 + r summed exactly; the
-// result is scaled by 2^n through the exponent bits in two factors, so a
-// subnormal result is formed by one final rounding multiply.
+// e^(zh + zl) for |zl| <= 2^-40 |zh| + 2^-60, zh in [-745.2, 709.78] (outside:
+// 0 or +Inf); table-driven (Tang):  n = round(zh * 128/ln2) = 128 m + j,
+// u = (zh - n ln2/128) + zl with the first difference exact (Sterbenz),
+// e^u - 1 = u + u^2 Q(u), Q the Taylor series 1/2 + u/6 + ... + u^4/720
+// (|u| < 2^-8.4, truncation below 2^-70); e^z = 2^m (th_j + (th_j (e^u - 1) + tl_j)).
+// 2^m is applied as two exact power-of-two factors so that a subnormal result
+// is rounded by the last multiply only.
 SDEH_HD inline double exp_dd(double zh, double zl) {
-  if (zh > 709.782712893384) return __builtin_inf();
-  if (zh < -745.2) return 0.0;
-  const double n = __builtin_rint(zh * 0x1.71547652b82fep+0);  // 1/ln 2
-  const double r0 = fma(-n, 0x1.62e4200000000p-1, zh);
-  const double rl0 = fma(-n, 0x1.fdf473de6af28p-22, zl);
-  const double rh = r0 + rl0;
-  const double rl = (r0 - rh) + rl0;
-  double q = 0x1.6124613a86d09p-33;         // 1/13!
-  q = fma(q, rh, 0x1.1eed8eff8d898p-29);    // 1/12!
-  q = fma(q, rh, 0x1.ae64567f544e4p-26);    // 1/11!
-  q = fma(q, rh, 0x1.27e4fb7789f5cp-22);    // 1/10!
-  q = fma(q, rh, 0x1.71de3a556c734p-19);    // 1/9!
-  q = fma(q, rh, 0x1.a01a01a01a01ap-16);    // 1/8!
-  q = fma(q, rh, 0x1.a01a01a01a01ap-13);    // 1/7!
-  q = fma(q, rh, 0x1.6c16c16c16c17p-10);    // 1/6!
-  q = fma(q, rh, 0x1.1111111111111p-7);     // 1/5!
-  q = fma(q, rh, 0x1.5555555555555p-5);     // 1/4!
-  q = fma(q, rh, 0x1.5555555555555p-3);     // 1/3!
-  q = fma(q, rh, 0.5);
-  const double t = fma(rh * rh, q, fma(rl, rh, rl));  // e^r - 1 - rh
-  const double s1 = 1.0 + rh;                         // fast two-sum, |rh| < 1
-  const double e1 = rh - (s1 - 1.0);
-  const double p = s1 + (e1 + t);
-  const int ni = (int)n;
-  // 2^ni = 2^h1 * 2^h2 with both halves normal doubles (ni in [-1075, 1024]).
-  const int h1 = ni / 2;
-  const int h2 = ni - h1;
-  const double s2a = double_of((uint64_t)(h1 + 1023) << 52);
-  const double s2b = double_of((uint64_t)(h2 + 1023) << 52);
-  return (p * s2a) * s2b;
+  using namespace sdeh_powtab;
+  const bool over = zh > 709.782712893384;
+  const bool under = zh < -745.2;
+  const double zc = (over || under) ? 0.0 : zh;
+  const double nd = __builtin_rint(zc * INV_LN2N);
+  const int n = (int)nd;
+  const double r0 = fma(-nd, LN2N_HI, zc);  // exact
+  const double u = r0 + fma(-nd, LN2N_LO, zl);
+  double q = 0x1.6c16c16c16c17p-10;   // 1/720
+  q = fma(q, u, 0x1.1111111111111p-7);  // 1/120
+  q = fma(q, u, 0x1.5555555555555p-5);  // 1/24
+  q = fma(q, u, 0x1.5555555555555p-3);  // 1/6
+  q = fma(q, u, 0.5);
+  const double em1 = fma(u * u, q, u);
+  const int j = n & EXP_MASK;
+  const int m = n >> EXP_BITS;  // arithmetic shift: floor(n / 128)
+  const double th = EXP_TAB[j][0], tl = EXP_TAB[j][1];
+  const double res = th + fma(th, em1, tl);
+  // 2^m = 2^m1 * 2^m2 with both halves normal (m in [-1075, 1024]).
+  const int m1 = m / 2;
+  const int m2 = m - m1;
+  const double s2a = double_of((uint64_t)(m1 + 1023) << 52);
+  const double s2b = double_of((uint64_t)(m2 + 1023) << 52);
+  const double v = (res * s2a) * s2b;
+  return over ? __builtin_inf() : (under ? 0.0 : v);
 }
 
-// x^y for the controller: x >= 0 (an RMS error norm), y > 0 a fixed exponent,
-// split as log then exp so one logarithm can serve several exponents (the
-// controller raises the same error estimate to beta1 now and to beta2 for the
-// next step).  pow_from(pow_log(x), y) == pow_ctrl(x, y) bit for bit.
-// Special values: NaN x gives NaN, 0 gives 0, +Inf gives +Inf, negative x
-// gives NaN, 1 gives 1.
+// x^y for the controller: x >= 0 (an RMS error norm), 0 < y <= 1 a fixed
+// exponent, split as log then exp so one logarithm can serve several exponents
+// (the controller raises the same error estimate to beta1 now and to beta2 for
+// the next step).  pow_from(pow_log(x), y) == pow_ctrl(x, y) bit for bit.
+// Error below 0.52 ulp for results in the normal range.  Special values: NaN x
+// gives x, 0 gives 0, +Inf gives +Inf, negative x gives NaN, 1 gives 1 (the
+// table path returns exactly 1 there).  Every path is evaluated with selects.
 struct PowLog {
   double hi, lo;
   double special;  // result for every y when is_special
@@ -166,32 +173,21 @@ struct PowLog {
 };
 
 SDEH_HD inline PowLog pow_log(double x) {
+  const bool nan_or_neg = !(x >= 0.0);
+  const bool zero = x == 0.0;
+  const bool inf = x == __builtin_inf();
   PowLog r;
-  r.hi = r.lo = 0.0;
-  r.is_special = true;
-  if (x != x) {
-    r.special = x;
-  } else if (x == 0.0) {
-    r.special = 0.0;
-  } else if (x < 0.0) {
-    r.special = __builtin_nan("");
-  } else if (x == __builtin_inf()) {
-    r.special = x;
-  } else if (x == 1.0) {
-    r.special = 1.0;
-  } else {
-    r.is_special = false;
-    r.special = 0.0;
-    log_pos_dd(x, &r.hi, &r.lo);
-  }
+  r.is_special = nan_or_neg || zero || inf;
+  r.special = zero ? 0.0 : ((x != x || inf) ? x : __builtin_nan(""));
+  log_pos_dd(r.is_special ? 1.0 : x, &r.hi, &r.lo);
   return r;
 }
 
 SDEH_HD inline double pow_from(const PowLog& l, double y) {
-  if (l.is_special) return l.special;
   const double ph = y * l.hi;
   const double pl = fma(y, l.hi, -ph) + y * l.lo;
-  return exp_dd(ph, pl);
+  const double v = exp_dd(ph, pl);
+  return l.is_special ? l.special : v;
 }
 
 SDEH_HD inline double pow_ctrl(double x, double y) { return pow_from(pow_log(x), y); }

@@ -6,7 +6,7 @@ Only plain ctypes: the structs below must match the C header field for field.
 import ctypes
 import os
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # sdeh_status
 OK = 0
@@ -72,7 +72,8 @@ class SolveDesc(ctypes.Structure):
         ("block", ctypes.c_int32),
         ("grid", ctypes.c_int32),
         ("chunk", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("lanes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -107,7 +108,7 @@ class Stats(ctypes.Structure):
         ("grid", ctypes.c_int32),
         ("chunk", ctypes.c_int32),
         ("schedule", ctypes.c_int32),
-        ("vgprs", ctypes.c_int32),
+        ("lanes", ctypes.c_int32),
     ]
 
 

@@ -1,0 +1,90 @@
+"""GLV-16 (16 species, interaction matrix shared by every trajectory) on the GPU
+against the CPU oracle, in both lane layouts: one lane per trajectory, and the
+group layout (16 adjacent lanes per trajectory, one component each, state
+exchanged through LDS).  binary64 is bit for bit against the oracle (endpoint
+and saveat, both schedules); binary32 is bit for bit between the two layouts,
+which evaluate the same operations in the same order."""
+import numpy as np
+import pytest
+
+from helpers import assert_bit_identical, bits_equal, problem, run_pair
+from simplediffeq_hip import _abi, runtime
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("u_end", "t_end", "retcode", "naccept", "nreject")
+
+
+def _desc(alg, n, saveat, lanes, **kw):
+    adaptive = alg in (_abi.ALG_ATSIT5, _abi.ALG_AVERN9)
+    sv = np.linspace(0.0, 5.0, 17) if saveat else None
+    return runtime.make_desc(alg, rhs=_abi.RHS_GLV16, n_traj=n, tspan=(0.0, 5.0),
+                             dt=0.1 if adaptive else 0.05, abstol=1e-9, reltol=1e-9,
+                             saveat=sv, lanes=lanes, **kw)
+
+
+@pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT])
+@pytest.mark.parametrize("lanes", [1, 16])
+@pytest.mark.parametrize("alg,saveat", [(_abi.ALG_AVERN9, False), (_abi.ALG_AVERN9, True),
+                                        (_abi.ALG_ATSIT5, False), (_abi.ALG_ATSIT5, True),
+                                        (_abi.ALG_TSIT5, False)],
+                         ids=["avern9", "avern9-saveat", "atsit5", "atsit5-saveat", "tsit5"])
+def test_glv16_binary64_bitexact(ctx, alg, saveat, lanes, sched):
+    if alg == _abi.ALG_TSIT5 and sched == _abi.SCHED_PERSISTENT:
+        pytest.skip("fixed-step solvers always run one launch")
+    n = 203  # not a multiple of the four trajectories a wave holds in the group layout
+    u0, r, A = problem(_abi.RHS_GLV16, n, seed=3)
+    d = _desc(alg, n, saveat, lanes, schedule=sched, chunk=3)
+    st_g, g, st_c, c = run_pair(ctx, d, u0, r, A)
+    assert st_g == 0 and st_c == 0
+    assert ctx.stats()["lanes"] == lanes
+    assert_bit_identical(g, c, FIELDS + (("u_saveat",) if saveat else ()))
+    if alg != _abi.ALG_TSIT5:
+        assert (g.numpy("naccept") > 5).all()
+
+
+@pytest.mark.parametrize("kahan,saveat", [(False, False), (False, True), (True, False)],
+                         ids=["plain", "plain-saveat", "kahan"])
+def test_glv16_binary32_layouts_identical(ctx, kahan, saveat):
+    n = 301
+    u0, r, A = problem(_abi.RHS_GLV16, n, seed=4)
+    prec = _abi.PREC_KAHAN if kahan else _abi.PREC_PLAIN
+    out = {}
+    for lanes in (1, 16):
+        d = _desc(_abi.ALG_ATSIT5, n, saveat, lanes, dtype=_abi.F32, precision=prec)
+        d.abstol, d.reltol = 1e-6, 1e-6
+        b = runtime.EnsembleBuffers(d, 16, u0, r, A)
+        ctx.solve(d, b)
+        assert ctx.stats()["lanes"] == lanes
+        out[lanes] = b
+    for f in FIELDS + (("u_saveat",) if saveat else ()):
+        assert bits_equal(out[1].numpy(f), out[16].numpy(f)), f
+    # and close to the binary64 oracle at a tight tolerance
+    from oracle import pyoracle
+    d = _desc(_abi.ALG_AVERN9, n, False, 1)
+    d.abstol, d.reltol = 1e-12, 1e-12
+    ref = runtime.EnsembleBuffers(d, 16, u0, r, A)
+    assert pyoracle.solve(d, ref, nthreads=8) == 0
+    err = np.abs(out[16].numpy("u_end").astype(np.float64) - ref.numpy("u_end"))
+    assert np.max(err / np.abs(ref.numpy("u_end"))) < 1e-3
+
+
+def test_glv16_layout_selection(ctx):
+    n = 64
+    u0, r, A = problem(_abi.RHS_GLV16, n, seed=5)
+    # auto: the group layout where it is built
+    d = _desc(_abi.ALG_AVERN9, n, False, 0)
+    ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0, r, A))
+    assert ctx.stats()["lanes"] == 16
+    # binary32 defaults to one lane per trajectory
+    d = _desc(_abi.ALG_ATSIT5, n, False, 0, dtype=_abi.F32)
+    ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0, r, A))
+    assert ctx.stats()["lanes"] == 1
+    # an unsupported group size, and a group layout for a right-hand side without one
+    d = _desc(_abi.ALG_AVERN9, n, False, 4)
+    with pytest.raises(runtime.SolveError, match="lanes per trajectory"):
+        ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0, r, A))
+    dl = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=8, lanes=16)
+    ul, pl, _ = problem(_abi.RHS_LORENZ, 8)
+    with pytest.raises(runtime.SolveError, match="lane per trajectory"):
+        ctx.solve(dl, runtime.EnsembleBuffers(dl, 3, ul, pl))
