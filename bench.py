@@ -170,7 +170,9 @@ def workload_uneven(args):
 
 def workload_glv(args):
     """GLV-16, 131,072 trajectories (seed 3), tspan (0,50), saveat 256 evenly spaced
-    points: AVern9 binary64 at 1e-10 and ATsit5 binary32 at 1e-6."""
+    points: AVern9 binary64 at 1e-10 and ATsit5 binary32 at 1e-6.  Saved states in
+    the trajectory-major layout ([trajectory][point][component], each trajectory's
+    solution contiguous); the point-major layout is timed beside it."""
     import numpy as np
     import torch
     from simplediffeq_hip import _abi, inputs, runtime
@@ -182,19 +184,23 @@ def workload_glv(args):
     out = {"workload": "glv16-saveat", "trajectories": n}
     for name, alg, dtype, tol in (("avern9_f64", _abi.ALG_AVERN9, _abi.F64, 1e-10),
                                   ("atsit5_f32", _abi.ALG_ATSIT5, _abi.F32, 1e-6)):
-        d = runtime.make_desc(alg, dtype=dtype, rhs=_abi.RHS_GLV16, n_traj=n, tspan=(0.0, 50.0),
-                              dt=0.1, abstol=tol, reltol=tol, saveat=saveat)
-        g = runtime.EnsembleBuffers(d, 16, u0, r, A, backend="torch", device="cuda:0")
-        ctx.solve(d, g, sync=True)
-        st = ctx.stats()
-        ms, _ = _time_solves(ctx, d, g, stream, max(1, args.steps // 4), 1)
-        es = 8 if dtype == _abi.F64 else 4
-        out_bytes = 256 * 16 * n * es
-        per_attempt = 16 * STAGE_FLOPS[alg] + RHS_PER_ATTEMPT[alg] * 528 + 12
-        flops = st["attempts"] * per_attempt + (st["accepts"] * 528 if alg == 5 else n * 528)
-        out[name] = {"kernel_ms": ms, "traj_per_s": n / (ms * 1e-3), "attempts": st["attempts"],
-                     "saveat_GBs": out_bytes / (ms * 1e-3) / 1e9,
-                     "tflops_excl_saveat": flops / (ms * 1e-3) / 1e12}
+        for layout, tag in ((_abi.SAVE_TRAJ_MAJOR, ""), (_abi.SAVE_POINT_MAJOR, "_point_major")):
+            d = runtime.make_desc(alg, dtype=dtype, rhs=_abi.RHS_GLV16, n_traj=n,
+                                  tspan=(0.0, 50.0), dt=0.1, abstol=tol, reltol=tol,
+                                  saveat=saveat, save_layout=layout)
+            g = runtime.EnsembleBuffers(d, 16, u0, r, A, backend="torch", device="cuda:0")
+            ctx.solve(d, g, sync=True)
+            st = ctx.stats()
+            ms, _ = _time_solves(ctx, d, g, stream, max(1, args.steps // 4), 1)
+            es = 8 if dtype == _abi.F64 else 4
+            out_bytes = 256 * 16 * n * es
+            per_attempt = 16 * STAGE_FLOPS[alg] + RHS_PER_ATTEMPT[alg] * 528 + 12
+            flops = st["attempts"] * per_attempt + (st["accepts"] * 528 if alg == 5 else n * 528)
+            out[name + tag] = {"kernel_ms": ms, "traj_per_s": n / (ms * 1e-3),
+                               "attempts": st["attempts"], "lanes": st["lanes"],
+                               "saveat_GBs": out_bytes / (ms * 1e-3) / 1e9,
+                               "tflops_excl_saveat": flops / (ms * 1e-3) / 1e12}
+            del g
     return out
 
 

@@ -85,6 +85,18 @@ typedef enum sdeh_builtin_rhs {
   SDEH_RHS_LINEAR2 = 5    /* n_state 2, n_param 4: du = [p0 p1; p2 p3] * u */
 } sdeh_builtin_rhs;
 
+/* Layout of the saved-state outputs (u_saveat, u_steps, t_steps).  u_end and the
+ * other per-trajectory outputs are always [component][trajectory]. */
+typedef enum sdeh_save_layout {
+  SDEH_SAVE_POINT_MAJOR = 0, /* u_saveat [n_saveat][n_state][n_traj], u_steps
+                                [max_saves][n_state][n_traj], t_steps [max_saves][n_traj] */
+  SDEH_SAVE_TRAJ_MAJOR = 1   /* u_saveat [n_traj][n_saveat][n_state], u_steps
+                                [n_traj][max_saves][n_state], t_steps [n_traj][max_saves]:
+                                each trajectory's saves contiguous, as the reference's
+                                per-trajectory solution holds them; a lane writes whole
+                                rows, which the GPU stores far faster */
+} sdeh_save_layout;
+
 /* Scheduling of trajectories onto lanes. */
 typedef enum sdeh_schedule {
   SDEH_SCHED_AUTO = 0,       /* library choice (autotuned) */
@@ -131,7 +143,8 @@ typedef struct sdeh_solve_desc {
                                one component each), else 1; 1 = one lane holds a whole
                                trajectory; or the group size itself.  Results are
                                identical. */
-  int32_t reserved[3];
+  int32_t save_layout;      /* sdeh_save_layout of u_saveat, u_steps and t_steps */
+  int32_t reserved[2];
 } sdeh_solve_desc;
 
 /* Solve buffers.  With memory == SDEH_MEM_HOST these are host pointers and the
@@ -152,7 +165,8 @@ typedef struct sdeh_io {
   int32_t* retcode;        /* [n_traj] sdeh_retcode */
   int32_t* naccept;        /* [n_traj] accepted steps (optional) */
   int32_t* nreject;        /* [n_traj] rejected attempts (optional) */
-  void* u_saveat;          /* [n_saveat][n_state][n_traj]; unreached points are NaN */
+  void* u_saveat;          /* [n_saveat][n_state][n_traj] (or trajectory-major, see
+                              save_layout); unreached points are NaN */
   void* u_steps;           /* [max_saves][n_state][n_traj] save_everystep states */
   double* t_steps;         /* [max_saves][n_traj] save_everystep times */
   int32_t* n_steps;        /* [n_traj] number of saved steps (u0 included) */

@@ -66,18 +66,23 @@ int run_ensemble(const sdeh_solve_desc* d, const sdeh_io* io, const Mode& md, in
       if (rc == SDEH_RET_DTLESSTHANMIN) any_dtmin = 1;
       if (io->naccept) io->naccept[i] = (int32_t)tr.naccept;
       if (io->nreject) io->nreject[i] = (int32_t)tr.nreject;
+      const bool tm = d->save_layout == SDEH_SAVE_TRAJ_MAJOR;
       if (saveat && io->u_saveat) {
         T* o = (T*)io->u_saveat;
-        for (int64_t j = 0; j < d->n_saveat; ++j)
-          for (int c = 0; c < N; ++c) o[(j * N + c) * n + i] = us[j * N + c];
+        const int64_t ns = d->n_saveat;
+        for (int64_t j = 0; j < ns; ++j)
+          for (int c = 0; c < N; ++c)
+            o[tm ? (i * ns + j) * N + c : (j * N + c) * n + i] = us[j * N + c];
       }
       if (everystep) {
         const int64_t cnt = (int64_t)st.size();
         const int64_t keep = std::min<int64_t>(cnt, d->max_saves);
+        const int64_t ms = d->max_saves;
         for (int64_t s = 0; s < keep; ++s) {
-          if (io->t_steps) io->t_steps[s * n + i] = st[s];
+          if (io->t_steps) io->t_steps[tm ? i * ms + s : s * n + i] = st[s];
           if (io->u_steps)
-            for (int c = 0; c < N; ++c) ((T*)io->u_steps)[(s * N + c) * n + i] = su[s * N + c];
+            for (int c = 0; c < N; ++c)
+              ((T*)io->u_steps)[tm ? (i * ms + s) * N + c : (s * N + c) * n + i] = su[s * N + c];
         }
         if (io->n_steps) io->n_steps[i] = (int32_t)keep;
         if (cnt > d->max_saves) {

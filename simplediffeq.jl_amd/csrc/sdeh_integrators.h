@@ -575,6 +575,29 @@ __device__ __forceinline__ void store_state(void* base, int64_t row, int64_t n, 
   for (int c = 0; c < R::N; ++c) o[(row * FULL + c0 + c) * n + traj] = u[c];
 }
 
+// Row `row` of a saved-state output with `rows` rows, in the layout the kernel
+// arguments select: point-major [rows][components][n] or trajectory-major
+// [n][rows][components] (a lane then writes its components contiguously).
+template <class T, class R>
+__device__ __forceinline__ void store_save(const sdeh_kargs& a, void* base, int64_t row,
+                                           int64_t rows, int64_t traj, const T (&u)[R::N]) {
+  constexpr int FULL = R::N * R::G;
+  if (!a.traj_major) {
+    store_state<T, R>(base, row, a.n, traj, u);
+    return;
+  }
+  const int c0 = lane_comp<R>() * R::N;
+  T* o = (T*)base + (traj * rows + row) * FULL + c0;
+#pragma unroll
+  for (int c = 0; c < R::N; ++c) o[c] = u[c];
+}
+
+__device__ __forceinline__ void store_save_time(const sdeh_kargs& a, int64_t row, int64_t traj,
+                                                double t) {
+  if (a.traj_major) a.t_steps[traj * a.max_saves + row] = t;
+  else a.t_steps[row * a.n + traj] = t;
+}
+
 template <class T, class R>
 __device__ __forceinline__ void load_state(T (&u)[R::N], const void* base, int64_t n,
                                            int64_t traj) {
@@ -642,6 +665,14 @@ __device__ __forceinline__ int64_t wave_claim(const sdeh_kargs& a, bool need_me,
   return mine;
 }
 
+// Save time i, or 0 past the end (callers test i < n_saveat first).  The next save
+// time is kept in a register between steps and the one after it is loaded while
+// the current point's dense output is computed, so the save test on each step
+// waits on no global load.
+__device__ __forceinline__ double save_time(const sdeh_kargs& a, int64_t i) {
+  return i < a.n_saveat ? a.saveat[i] : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // Adaptive solvers: ATsit5, AVern7, AVern9.
 // ---------------------------------------------------------------------------
@@ -665,6 +696,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
   double t = 0.0, tf = 0.0;
   T dt = T(0), qold = T(0), qold_pow = T(0);
   int32_t nacc = 0, nrej = 0, cur = 0, nsaved = 0;
+  double ts_next = 0.0;  // save_time(cur)
   int64_t attempts = 0, traj = -1;
   bool active = false;
 
@@ -711,14 +743,15 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
       nsaved = 0;
       if constexpr (SAVE == SAVE_AT) {
         if (a.n_saveat > 0 && t == a.saveat[0]) {
-          store_state<T, R>(a.u_saveat, 0, n, traj, uprev);
+          store_save<T, R>(a, a.u_saveat, 0, a.n_saveat, traj, uprev);
           cur = 1;
         }
+        ts_next = save_time(a, cur);
       }
       if constexpr (SAVE == SAVE_STEPS) {
         if (a.max_saves > 0) {
-          store_state<T, R>(a.u_steps, 0, n, traj, uprev);
-          if (lead) a.t_steps[traj] = t;
+          store_save<T, R>(a, a.u_steps, 0, a.max_saves, traj, uprev);
+          if (lead) store_save_time(a, 0, traj, t);
         }
         nsaved = 1;
       }
@@ -795,43 +828,49 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
           ++nacc;
           if constexpr (SAVE == SAVE_AT) {
             if constexpr (ALG == ALG_ATSIT5) {  // gpuatsit5.jl:305-316
-              while (cur < a.n_saveat && a.saveat[cur] <= t) {
-                const T th = (T)((a.saveat[cur] - told) / (double)dtold);
+              while (cur < a.n_saveat && ts_next <= t) {
+                const double ts_after = save_time(a, cur + 1);
+                const T th = (T)((ts_next - told) / (double)dtold);
                 T o[N];
                 tsit5_dense(o, uprev, dtold, th, fsal, k2, k3, k4, k5, k6, k7);
-                store_state<T, R>(a.u_saveat, cur, n, traj, o);
+                store_save<T, R>(a, a.u_saveat, cur, a.n_saveat, traj, o);
                 ++cur;
+                ts_next = ts_after;
               }
-            } else if (cur < a.n_saveat && a.saveat[cur] <= t) {
+            } else if (cur < a.n_saveat && ts_next <= t) {
               // Extra stages depend only on the step, not on θ: computed once.
               if constexpr (V9) {
                 T x[10][N];
                 vern9_extra<T, R>(uprev, S9, x, pl, ps, told, dtold);  // told + c*dtold
-                while (cur < a.n_saveat && a.saveat[cur] <= t) {
-                  const T th = (T)((a.saveat[cur] - told) / (double)dtold);
+                while (cur < a.n_saveat && ts_next <= t) {
+                  const double ts_after = save_time(a, cur + 1);
+                  const T th = (T)((ts_next - told) / (double)dtold);
                   T o[N];
                   vern9_dense(o, uprev, dtold, th, S9, x);
-                  store_state<T, R>(a.u_saveat, cur, n, traj, o);
+                  store_save<T, R>(a, a.u_saveat, cur, a.n_saveat, traj, o);
                   ++cur;
+                  ts_next = ts_after;
                 }
               } else {
                 T x[6][N];
                 // Quirk (gpuvern7.jl:449): stage times from the post-step t.
                 vern7_extra<T, R>(uprev, K7, x, pl, ps, a.quirks ? t : told, dtold);
-                while (cur < a.n_saveat && a.saveat[cur] <= t) {
-                  const T th = (T)((a.saveat[cur] - told) / (double)dtold);
+                while (cur < a.n_saveat && ts_next <= t) {
+                  const double ts_after = save_time(a, cur + 1);
+                  const T th = (T)((ts_next - told) / (double)dtold);
                   T o[N];
                   vern7_dense(o, uprev, dtold, th, K7, x);
-                  store_state<T, R>(a.u_saveat, cur, n, traj, o);
+                  store_save<T, R>(a, a.u_saveat, cur, a.n_saveat, traj, o);
                   ++cur;
+                  ts_next = ts_after;
                 }
               }
             }
           }
           if constexpr (SAVE == SAVE_STEPS) {
             if (nsaved < a.max_saves) {
-              store_state<T, R>(a.u_steps, nsaved, n, traj, u);
-              if (lead) a.t_steps[(int64_t)nsaved * n + traj] = t;
+              store_save<T, R>(a, a.u_steps, nsaved, a.max_saves, traj, u);
+              if (lead) store_save_time(a, nsaved, traj, t);
             }
             ++nsaved;
           }
@@ -892,16 +931,18 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
   const T dt = (T)a.dt;
   double t = a.t0;
   int32_t cur = 0, nsaved = 0;
+  double ts_next = 0.0;  // save_time(cur)
   if constexpr (SAVE == SAVE_AT) {
     if (a.n_saveat > 0 && t == a.saveat[0]) {
-      store_state<T, R>(a.u_saveat, 0, n, traj, u);
+      store_save<T, R>(a, a.u_saveat, 0, a.n_saveat, traj, u);
       cur = 1;
     }
+    ts_next = save_time(a, cur);
   }
   if constexpr (SAVE == SAVE_STEPS) {
     if (a.max_saves > 0) {
-      store_state<T, R>(a.u_steps, 0, n, traj, u);
-      if (lead) a.t_steps[traj] = t;
+      store_save<T, R>(a, a.u_steps, 0, a.max_saves, traj, u);
+      if (lead) store_save_time(a, 0, traj, t);
     }
     nsaved = 1;
   }
@@ -916,12 +957,14 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
       tsit5_stages<T, R, false>(uprev, k1, k2, k3, k4, k5, k6, k7, u, zero, dummy, pl, ps, t, dt);
       t += (double)dt;
       if constexpr (SAVE == SAVE_AT) {  // :116-127
-        while (cur < a.n_saveat && a.saveat[cur] <= t) {
-          const T th = (T)((a.saveat[cur] - (t - (double)dt)) / (double)dt);
+        while (cur < a.n_saveat && ts_next <= t) {
+          const double ts_after = save_time(a, cur + 1);
+          const T th = (T)((ts_next - (t - (double)dt)) / (double)dt);
           T o[N];
           tsit5_dense(o, uprev, dt, th, k1, k2, k3, k4, k5, k6, k7);
-          store_state<T, R>(a.u_saveat, cur, n, traj, o);
+          store_save<T, R>(a, a.u_saveat, cur, a.n_saveat, traj, o);
           ++cur;
+          ts_next = ts_after;
         }
       }
     } else if constexpr (ALG == ALG_VERN7) {
@@ -930,15 +973,17 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
       const double told = t;
       t += (double)dt;  // gpuvern7.jl:138
       if constexpr (SAVE == SAVE_AT) {
-        if (cur < a.n_saveat && a.saveat[cur] <= t) {
+        if (cur < a.n_saveat && ts_next <= t) {
           T x[6][N];
           vern7_extra<T, R>(uprev, K, x, pl, ps, a.quirks ? t : told, dt);  // quirk :160
-          while (cur < a.n_saveat && a.saveat[cur] <= t) {
-            const T th = (T)((a.saveat[cur] - (t - (double)dt)) / (double)dt);
+          while (cur < a.n_saveat && ts_next <= t) {
+            const double ts_after = save_time(a, cur + 1);
+            const T th = (T)((ts_next - (t - (double)dt)) / (double)dt);
             T o[N];
             vern7_dense(o, uprev, dt, th, K, x);
-            store_state<T, R>(a.u_saveat, cur, n, traj, o);
+            store_save<T, R>(a, a.u_saveat, cur, a.n_saveat, traj, o);
             ++cur;
+            ts_next = ts_after;
           }
         }
       }
@@ -949,23 +994,25 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
       const double told = t;
       t += (double)dt;  // gpuvern9.jl:185
       if constexpr (SAVE == SAVE_AT) {
-        if (cur < a.n_saveat && a.saveat[cur] <= t) {
+        if (cur < a.n_saveat && ts_next <= t) {
           T x[10][N];
           vern9_extra<T, R>(uprev, S, x, pl, ps, a.quirks ? t : told, dt);  // quirk :222
-          while (cur < a.n_saveat && a.saveat[cur] <= t) {
-            const T th = (T)((a.saveat[cur] - (t - (double)dt)) / (double)dt);
+          while (cur < a.n_saveat && ts_next <= t) {
+            const double ts_after = save_time(a, cur + 1);
+            const T th = (T)((ts_next - (t - (double)dt)) / (double)dt);
             T o[N];
             vern9_dense(o, uprev, dt, th, S, x);
-            store_state<T, R>(a.u_saveat, cur, n, traj, o);
+            store_save<T, R>(a, a.u_saveat, cur, a.n_saveat, traj, o);
             ++cur;
+            ts_next = ts_after;
           }
         }
       }
     }
     if constexpr (SAVE == SAVE_STEPS) {
       if (nsaved < a.max_saves) {
-        store_state<T, R>(a.u_steps, nsaved, n, traj, u);
-        if (lead) a.t_steps[(int64_t)nsaved * n + traj] = t;
+        store_save<T, R>(a, a.u_steps, nsaved, a.max_saves, traj, u);
+        if (lead) store_save_time(a, nsaved, traj, t);
       }
       ++nsaved;
     }

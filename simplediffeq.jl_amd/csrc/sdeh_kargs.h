@@ -44,5 +44,5 @@ struct sdeh_kargs {
   int32_t quirks;
   int32_t persistent;      // 1: lanes refill from *queue in chunks of `chunk`
   int32_t chunk;
-  int32_t pad_;
+  int32_t traj_major;      // saved-state outputs [n][rows][N] instead of [rows][N][n]
 };

@@ -32,17 +32,20 @@ struct Lorenz {
 // p = r (16 per trajectory); ps = A, 16x16 row-major, shared.
 // A is read through the constant address space (the same matrix for every lane,
 // at compile-time offsets), so it arrives by scalar loads into SGPRs and feeds
-// the FMAs directly; the empty asm is a compiler memory barrier that keeps those
-// loads inside each evaluation instead of letting the compiler hold all of A
-// (256 values) in vector registers across the whole integration loop.
+// the FMAs directly.  The pointer passes through an empty asm on each evaluation:
+// the compiler can then no longer prove the loads loop-invariant, so it keeps them
+// inside the evaluation (batched s_load_dwordx16) instead of loading all of A (256
+// values) once and parking it in VGPR lanes, from which every FMA needed a
+// v_readlane first.
 struct GLV16 {
   static constexpr int N = 16, P = 16, S = 256, G = 1;
   template <class T>
   __device__ static inline void f(T* du, const T* u, const T* p, const T* __restrict__ ps,
                                   double t) {
-    asm volatile("" ::: "memory");
+    const T* pa = ps;
+    asm volatile("" : "+s"(pa));
     const __attribute__((address_space(4))) T* A =
-        (const __attribute__((address_space(4))) T*)ps;
+        (const __attribute__((address_space(4))) T*)pa;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       T acc = p[i];

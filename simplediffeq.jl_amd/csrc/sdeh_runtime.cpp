@@ -439,6 +439,8 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
   if (d->block != 0 && (d->block < 64 || d->block > 256 || d->block % 64 != 0))
     return fail(c, SDEH_E_INVALID, "block must be 64, 128, 192 or 256");
   if (d->grid < 0 || d->chunk < 0) return fail(c, SDEH_E_INVALID, "grid/chunk must be >= 0");
+  if (d->save_layout != SDEH_SAVE_POINT_MAJOR && d->save_layout != SDEH_SAVE_TRAJ_MAJOR)
+    return fail(c, SDEH_E_INVALID, "save_layout %d out of range", d->save_layout);
 
   sdeh_kargs& a = pl->a;
   a.n = d->n_traj;
@@ -451,6 +453,7 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
   a.reltol = d->reltol;
   a.max_saves = pl->save == SAVE_STEPS ? d->max_saves : 0;
   a.quirks = d->quirks;
+  a.traj_major = d->save_layout == SDEH_SAVE_TRAJ_MAJOR;
   if (!adaptive) {
     if (d->dt == 0.0) return fail(c, SDEH_E_INVALID, "range step cannot be zero");
     const sdeh_range::Grid g = sdeh_range::make(d->t0, d->dt, d->tf);

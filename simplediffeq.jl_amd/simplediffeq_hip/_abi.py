@@ -42,6 +42,7 @@ BUILTIN_RHS = {
 }
 
 # sdeh_schedule / precision / memory
+SAVE_POINT_MAJOR, SAVE_TRAJ_MAJOR = 0, 1
 SCHED_AUTO, SCHED_ONE_LAUNCH, SCHED_PERSISTENT = 0, 1, 2
 PREC_PLAIN, PREC_KAHAN = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
@@ -73,7 +74,8 @@ class SolveDesc(ctypes.Structure):
         ("grid", ctypes.c_int32),
         ("chunk", ctypes.c_int32),
         ("lanes", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("save_layout", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

@@ -283,7 +283,8 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, dt=_f32(0.1), savea
     d = runtime.make_desc(alg.code, dtype=dtype, rhs=rid, n_traj=n, tspan=base.tspan, dt=dt,
                           abstol=abstol, reltol=reltol, saveat=saveat_arr,
                           save_everystep=everystep, max_saves=max_saves or 0, maxiters=maxiters,
-                          quirks=quirks, schedule=ens.schedule, precision=ens.precision)
+                          quirks=quirks, schedule=ens.schedule, precision=ens.precision,
+                          save_layout=_abi.SAVE_TRAJ_MAJOR)  # each solution contiguous
     bufs = runtime.EnsembleBuffers(d, n_state, u0s, ps if n_param else None, shared, tspans,
                                    dt0s, order)
     st = ctx.solve(d, bufs, allow=(_abi.E_DTMIN,))
@@ -296,11 +297,11 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, dt=_f32(0.1), savea
         rc = _abi.RETCODE_NAMES[int(bufs.retcode[i])]
         if saveat_arr is not None:
             t = saveat_arr
-            u = [bufs.u_saveat[j, :, i] for j in range(len(t))]
+            u = list(bufs.u_saveat[i])
         elif everystep:
             k = int(bufs.n_steps[i])
-            t = bufs.t_steps[:k, i]
-            u = [bufs.u_steps[j, :, i] for j in range(k)]
+            t = bufs.t_steps[i, :k]
+            u = list(bufs.u_steps[i, :k])
         else:
             t = np.array([t0s[i], bufs.t_end[i]])
             u = [u0s[:, i].astype(bufs.u_end.dtype), bufs.u_end[:, i]]

@@ -24,10 +24,11 @@ class SolveError(RuntimeError):
 def make_desc(alg, dtype=_abi.F64, rhs=_abi.RHS_LORENZ, n_traj=0, tspan=(0.0, 1.0), dt=None,
               abstol=None, reltol=None, saveat=None, save_everystep=False, max_saves=0,
               maxiters=0, quirks=True, schedule=_abi.SCHED_AUTO, precision=_abi.PREC_PLAIN,
-              block=0, grid=0, chunk=0, lanes=0):
+              block=0, grid=0, chunk=0, lanes=0, save_layout=_abi.SAVE_POINT_MAJOR):
     """Build an sdeh_solve_desc.  Unset dt/abstol/reltol take the reference
     defaults (0.1f0, 1f-6, 1f-3).  lanes: lanes per trajectory (0 = the
-    right-hand side's group layout where it has one).  Keeps the saveat array
+    right-hand side's group layout where it has one).  save_layout: the layout of
+    u_saveat/u_steps/t_steps (point- or trajectory-major).  Keeps the saveat array
     alive on the struct."""
     d = _abi.default_desc()
     d.alg = int(alg)
@@ -49,6 +50,7 @@ def make_desc(alg, dtype=_abi.F64, rhs=_abi.RHS_LORENZ, n_traj=0, tspan=(0.0, 1.
     d.precision = int(precision)
     d.block, d.grid, d.chunk = int(block), int(grid), int(chunk)
     d.lanes = int(lanes)
+    d.save_layout = int(save_layout)
     if saveat is not None and len(saveat) > 0:
         arr = np.ascontiguousarray(np.asarray(saveat, dtype=np.float64))
         d._saveat_keep = arr
@@ -103,11 +105,17 @@ class EnsembleBuffers:
         self.naccept = empty((n,), np.int32)
         self.nreject = empty((n,), np.int32)
         ns = int(desc.n_saveat)
-        self.u_saveat = empty((ns, n_state, n), dt_np) if ns > 0 else None
-        everystep = ns == 0 and desc.save_everystep
         ms = int(desc.max_saves)
-        self.u_steps = empty((ms, n_state, n), dt_np) if everystep and ms > 0 else None
-        self.t_steps = empty((ms, n), np.float64) if everystep and ms > 0 else None
+        tm = int(desc.save_layout) == _abi.SAVE_TRAJ_MAJOR
+        self.traj_major = tm
+        self.u_saveat = None
+        if ns > 0:
+            self.u_saveat = empty((n, ns, n_state) if tm else (ns, n_state, n), dt_np)
+        everystep = ns == 0 and desc.save_everystep
+        self.u_steps = self.t_steps = None
+        if everystep and ms > 0:
+            self.u_steps = empty((n, ms, n_state) if tm else (ms, n_state, n), dt_np)
+            self.t_steps = empty((n, ms) if tm else (ms, n), np.float64)
         self.n_steps = empty((n,), np.int32) if everystep else None
         io = _abi.SolveIO()
         io.u0 = ptr(self.u0)

@@ -23,18 +23,22 @@ def _desc(alg, n, saveat, lanes, **kw):
                              saveat=sv, lanes=lanes, **kw)
 
 
+@pytest.mark.parametrize("layout", [_abi.SAVE_POINT_MAJOR, _abi.SAVE_TRAJ_MAJOR],
+                         ids=["point-major", "traj-major"])
 @pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT])
 @pytest.mark.parametrize("lanes", [1, 16])
 @pytest.mark.parametrize("alg,saveat", [(_abi.ALG_AVERN9, False), (_abi.ALG_AVERN9, True),
                                         (_abi.ALG_ATSIT5, False), (_abi.ALG_ATSIT5, True),
                                         (_abi.ALG_TSIT5, False)],
                          ids=["avern9", "avern9-saveat", "atsit5", "atsit5-saveat", "tsit5"])
-def test_glv16_binary64_bitexact(ctx, alg, saveat, lanes, sched):
+def test_glv16_binary64_bitexact(ctx, alg, saveat, lanes, sched, layout):
     if alg == _abi.ALG_TSIT5 and sched == _abi.SCHED_PERSISTENT:
         pytest.skip("fixed-step solvers always run one launch")
+    if layout == _abi.SAVE_TRAJ_MAJOR and not saveat:
+        pytest.skip("the save layout only concerns saveat output")
     n = 203  # not a multiple of the four trajectories a wave holds in the group layout
     u0, r, A = problem(_abi.RHS_GLV16, n, seed=3)
-    d = _desc(alg, n, saveat, lanes, schedule=sched, chunk=3)
+    d = _desc(alg, n, saveat, lanes, schedule=sched, chunk=3, save_layout=layout)
     st_g, g, st_c, c = run_pair(ctx, d, u0, r, A)
     assert st_g == 0 and st_c == 0
     assert ctx.stats()["lanes"] == lanes

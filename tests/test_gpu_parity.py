@@ -56,39 +56,50 @@ def test_endpoint_bitexact(ctx, rhs, alg, sched):
     assert_bit_identical(g, c)
 
 
+LAYOUTS = [_abi.SAVE_POINT_MAJOR, _abi.SAVE_TRAJ_MAJOR]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=["point-major", "traj-major"])
 @pytest.mark.parametrize("quirks", [True, False])
 @pytest.mark.parametrize("alg", list(range(6)), ids=lambda a: ALG_NAMES[a])
 @pytest.mark.parametrize("rhs", [_abi.RHS_LORENZ, _abi.RHS_AFFINE1], ids=["lorenz", "affine1"])
-def test_saveat_bitexact(ctx, rhs, alg, quirks):
+def test_saveat_bitexact(ctx, rhs, alg, quirks, layout):
     n = 300
     u0, p, ps = problem(rhs, n, seed=3 + alg)
     adaptive = alg in ADAPT
     saveat = np.concatenate([[0.0, 0.05, 0.4], np.linspace(0.5, 1.5, 23), [1.5, 1.5]])
     d = runtime.make_desc(alg, rhs=rhs, n_traj=n, tspan=(0.0, 1.5), dt=0.1 if adaptive else 0.01,
-                          abstol=1e-8, reltol=1e-7, saveat=saveat, quirks=quirks)
+                          abstol=1e-8, reltol=1e-7, saveat=saveat, quirks=quirks,
+                          save_layout=layout)
     st_g, g, st_c, c = run_pair(ctx, d, u0, p, ps)
     assert st_g == st_c == 0
     assert_bit_identical(g, c, fields=("u_end", "t_end", "retcode", "naccept", "nreject",
                                        "u_saveat"))
 
 
+@pytest.mark.parametrize("layout", LAYOUTS, ids=["point-major", "traj-major"])
 @pytest.mark.parametrize("alg", list(range(6)), ids=lambda a: ALG_NAMES[a])
-def test_save_everystep_bitexact(ctx, alg):
+def test_save_everystep_bitexact(ctx, alg, layout):
     n = 200
     u0, p, _ = problem(_abi.RHS_LORENZ, n, seed=7)
     adaptive = alg in ADAPT
     d = runtime.make_desc(alg, rhs=_abi.RHS_LORENZ, n_traj=n, tspan=(0.0, 1.0),
                           dt=0.1 if adaptive else 0.05, abstol=1e-6, reltol=1e-4,
-                          save_everystep=True, max_saves=64)
+                          save_everystep=True, max_saves=64, save_layout=layout)
     st_g, g, st_c, c = run_pair(ctx, d, u0, p)
     assert st_g == st_c
     assert_bit_identical(g, c, fields=("u_end", "t_end", "retcode", "naccept", "nreject",
                                        "n_steps"))
     ns = g.n_steps
+    tm = layout == _abi.SAVE_TRAJ_MAJOR
     for i in range(n):
         k = ns[i]
-        assert np.array_equal(g.t_steps[:k, i], c.t_steps[:k, i])
-        assert np.array_equal(g.u_steps[:k, :, i], c.u_steps[:k, :, i])
+        if tm:
+            assert np.array_equal(g.t_steps[i, :k], c.t_steps[i, :k])
+            assert np.array_equal(g.u_steps[i, :k], c.u_steps[i, :k])
+        else:
+            assert np.array_equal(g.t_steps[:k, i], c.t_steps[:k, i])
+            assert np.array_equal(g.u_steps[:k, :, i], c.u_steps[:k, :, i])
 
 
 def test_device_memory_path_matches_host(ctx):
