@@ -178,7 +178,8 @@ int rhs_shape(sdeh_ctx* c, int64_t rhs, int32_t* n, int32_t* p, int64_t* s) {
 const std::vector<std::string>& rtc_options() {
   static const std::vector<std::string> o = {
       "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-mllvm",
-      "-disable-machine-licm", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+      "-disable-machine-licm", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+      "-Xclang", "-target-feature", "-Xclang", "-fmacf64-inst"};
   return o;
 }
 
