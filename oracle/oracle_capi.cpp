@@ -148,6 +148,10 @@ double oracle_pow_ctrl(double x, double y) { return sdeh_math::pow_ctrl(x, y); }
 // The pow's two halves, for their own accuracy tests.
 void oracle_log_dd(double x, double* hi_lo) { sdeh_math::log_pos_dd(x, &hi_lo[0], &hi_lo[1]); }
 double oracle_exp_dd(double zh, double zl) { return sdeh_math::exp_dd(zh, zl); }
+// The kernels' controller-domain pow (pow_log_ctrl + pow_from_ctrl).
+double oracle_pow_ctrl_domain(double x, double y) {
+  return sdeh_math::pow_from_ctrl(sdeh_math::pow_log_ctrl(x), y);
+}
 
 // Julia range restatement: length and the first min(len, cap) elements.
 int64_t oracle_range(double start, double step, double stop, double* elems, int64_t cap,

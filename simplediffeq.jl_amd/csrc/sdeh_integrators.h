@@ -68,10 +68,12 @@ struct Ops<double> {
   typedef sdeh_math::PowLog Log;
   __device__ static inline double div(double a, double b) { return a / b; }
   __device__ static inline double root(double x) { return __builtin_sqrt(x); }
+  // The controller's arguments stay in pow_log_ctrl's domain (normal, 0, Inf or
+  // NaN; exponents 7/50 and 2/25), where it returns pow_ctrl's bits.
   __device__ static inline double pw(double x, double y) { return sdeh_math::pow_ctrl(x, y); }
-  __device__ static inline Log plog(double x) { return sdeh_math::pow_log(x); }
+  __device__ static inline Log plog(double x) { return sdeh_math::pow_log_ctrl(x); }
   __device__ static inline double pw_from(const Log& l, double y) {
-    return sdeh_math::pow_from(l, y);
+    return sdeh_math::pow_from_ctrl(l, y);
   }
 };
 

@@ -82,14 +82,14 @@ SDEH_HD inline T fast_max(T x, T y) { return y > x ? y : x; }
 // The large terms are added with error-free transformations (Knuth's two-sum;
 // the fast form where the first operand dominates or is zero): k ln2_hi is exact
 // (ln2_hi has 42 significant bits, |k| < 2^11).
-SDEH_HD inline void log_pos_dd(double x, double* hi, double* lo) {
+// log_norm_dd is the core for a normal x scaled by 2^-kadj (log of x * 2^kadj);
+// log_pos_dd scales subnormal inputs into the normal range first.
+SDEH_HD inline void log_norm_dd(double xs, int kadj, double* hi, double* lo) {
   using namespace sdeh_powtab;
-  const bool sub = x < 0x1p-1022;
-  const double xs = sub ? x * 0x1p54 : x;
   const uint64_t ix = bits_of(xs);
   const uint64_t tmp = ix - LOG_OFF;
   const int i = (int)((tmp >> LOG_SHIFT) & (uint64_t)LOG_MASK);
-  const int k = (int)((int64_t)tmp >> 52) - (sub ? 54 : 0);
+  const int k = (int)((int64_t)tmp >> 52) - kadj;
   const double z = double_of(ix - (tmp & 0xfff0000000000000ULL));
   const double invc = LOG_TAB[i][0], lh = LOG_TAB[i][1], ll = LOG_TAB[i][2];
   const double r = fma(z, invc, -1.0);  // exact
@@ -124,18 +124,21 @@ SDEH_HD inline void log_pos_dd(double x, double* hi, double* lo) {
   *lo = (h - y) + t;
 }
 
+SDEH_HD inline void log_pos_dd(double x, double* hi, double* lo) {
+  const bool sub = x < 0x1p-1022;
+  log_norm_dd(sub ? x * 0x1p54 : x, sub ? 54 : 0, hi, lo);
+}
+
 // e^(zh + zl) for |zl| <= 2^-40 |zh| + 2^-60, zh in [-745.2, 709.78] (outside:
 // 0 or +Inf); table-driven (Tang):  n = round(zh * 128/ln2) = 128 m + j,
 // u = (zh - n ln2/128) + zl with the first difference exact (Sterbenz),
 // e^u - 1 = u + u^2 Q(u), Q the Taylor series 1/2 + u/6 + ... + u^4/720
 // (|u| < 2^-8.4, truncation below 2^-70); e^z = 2^m (th_j + (th_j (e^u - 1) + tl_j)).
-// 2^m is applied as two exact power-of-two factors so that a subnormal result
-// is rounded by the last multiply only.
-SDEH_HD inline double exp_dd(double zh, double zl) {
+// exp_core returns res and m with e^(zh + zl) = res * 2^m; exp_dd applies 2^m as
+// two exact power-of-two factors so that a subnormal result is rounded by the
+// last multiply only.
+SDEH_HD inline double exp_core(double zc, double zl, int* m_out) {
   using namespace sdeh_powtab;
-  const bool over = zh > 709.782712893384;
-  const bool under = zh < -745.2;
-  const double zc = (over || under) ? 0.0 : zh;
   const double nd = __builtin_rint(zc * INV_LN2N);
   const int n = (int)nd;
   const double r0 = fma(-nd, LN2N_HI, zc);  // exact
@@ -149,7 +152,15 @@ SDEH_HD inline double exp_dd(double zh, double zl) {
   const int j = n & EXP_MASK;
   const int m = n >> EXP_BITS;  // arithmetic shift: floor(n / 128)
   const double th = EXP_TAB[j][0], tl = EXP_TAB[j][1];
-  const double res = th + fma(th, em1, tl);
+  *m_out = m;
+  return th + fma(th, em1, tl);
+}
+
+SDEH_HD inline double exp_dd(double zh, double zl) {
+  const bool over = zh > 709.782712893384;
+  const bool under = zh < -745.2;
+  int m;
+  const double res = exp_core((over || under) ? 0.0 : zh, zl, &m);
   // 2^m = 2^m1 * 2^m2 with both halves normal (m in [-1075, 1024]).
   const int m1 = m / 2;
   const int m2 = m - m1;
@@ -191,5 +202,31 @@ SDEH_HD inline double pow_from(const PowLog& l, double y) {
 }
 
 SDEH_HD inline double pow_ctrl(double x, double y) { return pow_from(pow_log(x), y); }
+
+// The same pow on the controller's own arguments, with less work: x is an RMS
+// error estimate or qold (0, +Inf, NaN, or a normal number: the square root of a
+// binary64 is never subnormal) and 0 < y <= 0.9, so |y log x| < 700 and x^y is a
+// normal number.  No subnormal input scaling, no range saturation, and 2^m applied
+// as one factor (exact for a normal result).  On that domain the results are
+// pow_ctrl's bit for bit.
+SDEH_HD inline PowLog pow_log_ctrl(double x) {
+  const bool nan_or_neg = !(x >= 0.0);
+  const bool zero = x == 0.0;
+  const bool inf = x == __builtin_inf();
+  PowLog r;
+  r.is_special = nan_or_neg || zero || inf;
+  r.special = zero ? 0.0 : ((x != x || inf) ? x : __builtin_nan(""));
+  log_norm_dd(r.is_special ? 1.0 : x, 0, &r.hi, &r.lo);
+  return r;
+}
+
+SDEH_HD inline double pow_from_ctrl(const PowLog& l, double y) {
+  const double ph = y * l.hi;
+  const double pl = fma(y, l.hi, -ph) + y * l.lo;
+  int m;
+  const double res = exp_core(ph, pl, &m);
+  const double v = res * double_of((uint64_t)(m + 1023) << 52);
+  return l.is_special ? l.special : v;
+}
 
 }  // namespace sdeh_math

@@ -135,3 +135,23 @@ def test_pow_ctrl_special_values():
     xs = 10.0 ** rng.uniform(-10, 3, 20000)
     same = np.mean([lib.oracle_pow_ctrl(float(x), 0.14) == math.pow(float(x), 0.14) for x in xs])
     assert same > 0.98
+
+
+def test_controller_domain_pow_equals_general():
+    """The kernels evaluate the controller's pow with the cheaper pow_log_ctrl /
+    pow_from_ctrl, valid for x in {0, +Inf, NaN} or normal and 0 < y <= 0.9; there
+    it must return pow_ctrl's bits."""
+    lib = pyoracle.load()
+    rng = np.random.default_rng(5)
+    # normal x across the whole exponent range, plus the controller's typical range
+    bits = rng.integers(0x0010000000000000, 0x7FF0000000000000, 200000, dtype=np.int64)
+    xs = np.concatenate([bits.view(np.float64), 10.0 ** rng.uniform(-20, 3, 100000),
+                         [2.2250738585072014e-308, 1.7976931348623157e308, 1.0, 1e-4, 0.5,
+                          2.0, 0.0, math.inf]])
+    for y in (7 / 50, 2 / 25, 0.5, 0.9, float(np.float32(7 / 50))):
+        for x in xs:
+            a = lib.oracle_pow_ctrl_domain(float(x), y)
+            b = lib.oracle_pow_ctrl(float(x), y)
+            assert a == b or (math.isnan(a) and math.isnan(b)), (x, y, a, b)
+    assert math.isnan(lib.oracle_pow_ctrl_domain(math.nan, 0.14))
+    assert math.isnan(lib.oracle_pow_ctrl_domain(-1.0, 0.14))
