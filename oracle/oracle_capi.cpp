@@ -148,6 +148,19 @@ double oracle_pow_ctrl(double x, double y) { return sdeh_math::pow_ctrl(x, y); }
 // The pow's two halves, for their own accuracy tests.
 void oracle_log_dd(double x, double* hi_lo) { sdeh_math::log_pos_dd(x, &hi_lo[0], &hi_lo[1]); }
 double oracle_exp_dd(double zh, double zl) { return sdeh_math::exp_dd(zh, zl); }
+// Julia min/max and the kernels' reduced forms: op 0 jl_max, 1 jl_min,
+// 2 jl_max_nonneg, 3 jl_min_nonneg, 4 fast_max, 5 fast_min.
+double oracle_minmax(int op, double x, double y) {
+  switch (op) {
+    case 0: return sdeh_math::jl_max(x, y);
+    case 1: return sdeh_math::jl_min(x, y);
+    case 2: return sdeh_math::jl_max_nonneg(x, y);
+    case 3: return sdeh_math::jl_min_nonneg(x, y);
+    case 4: return sdeh_math::fast_max(x, y);
+    case 5: return sdeh_math::fast_min(x, y);
+  }
+  return 0.0;
+}
 // The kernels' controller-domain pow (pow_log_ctrl + pow_from_ctrl).
 double oracle_pow_ctrl_domain(double x, double y) {
   return sdeh_math::pow_from_ctrl(sdeh_math::pow_log_ctrl(x), y);

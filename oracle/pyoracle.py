@@ -40,6 +40,8 @@ def load():
     lib.oracle_exp_dd.restype = ctypes.c_double
     lib.oracle_pow_ctrl_domain.argtypes = [ctypes.c_double, ctypes.c_double]
     lib.oracle_pow_ctrl_domain.restype = ctypes.c_double
+    lib.oracle_minmax.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    lib.oracle_minmax.restype = ctypes.c_double
     lib.oracle_range.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)]
     lib.oracle_range.restype = ctypes.c_int64

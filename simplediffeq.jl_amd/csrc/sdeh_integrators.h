@@ -195,7 +195,7 @@ __device__ __forceinline__ T err_norm(T (&tmp)[R::N], const T (&uprev)[R::N], co
   constexpr int N = R::N;
 #pragma unroll
   for (int c = 0; c < N; ++c) {
-    const T m = sdeh_math::jl_max(fabs(uprev[c]), fabs(u[c]));
+    const T m = sdeh_math::jl_max_nonneg(fabs(uprev[c]), fabs(u[c]));
     tmp[c] = Ops<T>::div(tmp[c], fma(m, reltol, abstol));
   }
   if constexpr (R::G > 1) {
@@ -813,18 +813,20 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
         const T num = (accept && zero) ? inv_qmax : q11;
         const T den = (accept && !zero) ? qold_pow : T(1);
         const T qg = Ops<T>::div(Ops<T>::div(num, den), gamma);
+        // EEst is not NaN here, so q11 and qold^beta2 are finite or +Inf and qg is not
+        // NaN: Julia's min (gpuatsit5.jl:287) reduces to one comparison select.
         const T q = accept ? sdeh_math::fast_max(inv_qmax, sdeh_math::fast_min(inv_qmin, qg))
-                           : sdeh_math::jl_min(inv_qmin, qg);
+                           : sdeh_math::fast_min(inv_qmin, qg);
         const T dtq = Ops<T>::div(dt, q);
         if (!accept) {
           dt = dtq;
           ++nrej;
         } else {
-          qold = sdeh_math::jl_max(EEst, qoldinit);
+          qold = sdeh_math::fast_max(EEst, qoldinit);  // jl_max: EEst >= 0, not NaN
           qold_pow = (qold == EEst) ? e_b2 : qoldinit_pow;
           const T dtold = dt;
           const double rem = tf - t - (double)dtold;
-          dt = (T)sdeh_math::jl_min((double)fabs(dtq), fabs(rem));
+          dt = (T)sdeh_math::jl_min_nonneg((double)fabs(dtq), fabs(rem));
           const double told = t;
           t = (rem < SNAP) ? tf : t + (double)dtold;
           ++nacc;

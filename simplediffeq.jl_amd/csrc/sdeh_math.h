@@ -63,6 +63,21 @@ SDEH_HD inline T jl_max(T x, T y) {
   r = y != y ? y : r;
   return x != x ? x : r;
 }
+// jl_max / jl_min for arguments that are never negative (absolute values): the
+// -0.0/+0.0 rule cannot apply, so a comparison select plus NaN propagation gives
+// the same result (x's NaN first, as above).
+template <class T>
+SDEH_HD inline T jl_max_nonneg(T x, T y) {
+  T r = x < y ? y : x;
+  r = y != y ? y : r;
+  return x != x ? x : r;
+}
+template <class T>
+SDEH_HD inline T jl_min_nonneg(T x, T y) {
+  T r = y < x ? y : x;
+  r = y != y ? y : r;
+  return x != x ? x : r;
+}
 // Julia `@fastmath` min/max (Base.FastMath.min_fast/max_fast): a single
 // comparison select; the result with a NaN argument is not specified by the
 // reference.  Callers reach these only with non-NaN arguments.

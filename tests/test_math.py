@@ -155,3 +155,27 @@ def test_controller_domain_pow_equals_general():
             assert a == b or (math.isnan(a) and math.isnan(b)), (x, y, a, b)
     assert math.isnan(lib.oracle_pow_ctrl_domain(math.nan, 0.14))
     assert math.isnan(lib.oracle_pow_ctrl_domain(-1.0, 0.14))
+
+
+def _same(a, b):
+    return (math.isnan(a) and math.isnan(b)) or (a == b and math.copysign(1, a) == math.copysign(1, b))
+
+
+def test_reduced_min_max_equal_julia_on_their_domains():
+    """The kernels replace Julia's NaN-propagating min/max by cheaper forms where
+    the arguments allow: *_nonneg for absolute values (no -0.0), a plain
+    comparison select (fast_*) where neither argument can be NaN."""
+    lib = pyoracle.load()
+    rng = np.random.default_rng(6)
+    special = [0.0, math.inf, math.nan, 1e-300, 5e-324, 1.0, 5.0, 1e-4]
+    vals = list(np.abs(rng.standard_normal(2000))) + list(10.0 ** rng.uniform(-20, 5, 2000)) + special
+    for x in vals:
+        for y in special + [float(v) for v in rng.choice(vals, 20)]:
+            x, y = float(x), float(y)
+            assert _same(lib.oracle_minmax(2, x, y), lib.oracle_minmax(0, x, y)), (x, y)
+            assert _same(lib.oracle_minmax(3, x, y), lib.oracle_minmax(1, x, y)), (x, y)
+            if not (math.isnan(x) or math.isnan(y)):
+                assert _same(lib.oracle_minmax(4, x, y), lib.oracle_minmax(0, x, y)), (x, y)
+                assert _same(lib.oracle_minmax(5, x, y), lib.oracle_minmax(1, x, y)), (x, y)
+    # where the -0.0 rule matters, the reduced forms are not used
+    assert math.copysign(1, lib.oracle_minmax(1, 0.0, -0.0)) == -1.0
