@@ -8,7 +8,10 @@ default dt0 = 0.1f0, endpoint output only.  Inputs are resident in HBM before
 the timed region; each step runs the integrator kernel over the whole ensemble.
 
 value = trajectories/s over the K timed steps (binary64 state, the headline);
-the binary32 run of the same ensemble is reported beside it.
+the binary32 run of the same ensemble is reported beside it.  Before the warmup,
+the library's autotuner picks each variant's launch shape (schedule, block, grid,
+queue chunk) by timing candidate launches of the same solve; the chosen shape is
+in config.launch.
 
 roofline: the integrator kernel is bound by vector (VALU) binary64 throughput:
 achieved = algorithmic FLOPs per launch / average launch time (HIP events on
@@ -316,7 +319,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for the HBM "
                          "byte counters)")
-    ap.add_argument("--tune", action="store_true", help="autotune the launch shape first")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="skip the autotuner (by default sdeh_autotune picks the launch shape "
+                         "once per variant, before the warmup and the timed steps)")
     ap.add_argument("--grid", type=int, default=0,
                     help="persistent schedule: workgroups of 256 (0 = library default, "
                          "CUs x occupancy)")

@@ -79,6 +79,10 @@ class ODEProblem:
 
     def __init__(self, f, u0, tspan, p=None):
         self.f = f
+        # A scalar u0 / p (the reference's Float64-scalar problems) is carried as one
+        # component; f sees a scalar and solutions hold scalar states.
+        self.scalar_u = np.ndim(u0) == 0
+        self.scalar_p = p is not None and np.ndim(p) == 0
         self.u0 = np.atleast_1d(np.asarray(u0))
         self.tspan = (float(tspan[0]), float(tspan[1]))
         self.p = None if p is None else np.atleast_1d(np.asarray(p))
@@ -205,12 +209,13 @@ def _context(device):
     return _CONTEXTS[device]
 
 
-def _rhs_handle(ctx, f, n_state, n_param):
+def _rhs_handle(ctx, f, n_state, n_param, scalar_u=False, scalar_p=False):
     if isinstance(f, BuiltinRHS):
         return f.rid, f.shared
-    key = (id(ctx), f, n_state, n_param)
+    key = (id(ctx), f, n_state, n_param, scalar_u, scalar_p)
     if key not in _TRACED:
-        src, fn = tracer.trace(f, n_state, n_param, "sdeh_traced_%d" % len(_TRACED))
+        src, fn = tracer.trace(f, n_state, n_param, "sdeh_traced_%d" % len(_TRACED),
+                               scalar_u=scalar_u, scalar_p=scalar_p)
         _TRACED[key] = ctx.compile_rhs(src, fn, n_state, n_param)
     return _TRACED[key], None
 
@@ -271,7 +276,8 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, dt=_f32(0.1), savea
         dtype = _abi.F64
 
     ctx = _context(ens.device)
-    rid, shared = _rhs_handle(ctx, base.f, n_state, n_param)
+    rid, shared = _rhs_handle(ctx, base.f, n_state, n_param, base.scalar_u,
+                              base.scalar_p and n_param == 1)
     if ps is None:
         ps = np.zeros((0, n))
     everystep = saveat is None and bool(save_everystep)
@@ -305,6 +311,8 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, dt=_f32(0.1), savea
         else:
             t = np.array([t0s[i], bufs.t_end[i]])
             u = [u0s[:, i].astype(bufs.u_end.dtype), bufs.u_end[:, i]]
+        if base.scalar_u:
+            u = [x[0] for x in u]
         sols.append(ODESolution(t, u, rc, bufs.naccept[i], bufs.nreject[i], alg))
     if single:
         return sols[0]

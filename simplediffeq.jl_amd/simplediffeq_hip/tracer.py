@@ -197,8 +197,10 @@ class _Vec:
         return (self[i] for i in range(self.n))
 
 
-def trace(f, n_state, n_param, fn_name="sdeh_user_f"):
-    """Trace f(u, p, t) and return (hip_c_source, fn_name).
+def trace(f, n_state, n_param, fn_name="sdeh_user_f", scalar_u=False, scalar_p=False):
+    """Trace f(u, p, t) and return (hip_c_source, fn_name).  With scalar_u /
+    scalar_p (a problem whose u0 / p is a scalar, as the reference allows), f
+    receives u / p as a single value instead of a vector.
 
     The source defines
         template <class T> __device__ inline void fn_name(T* du, const T* u,
@@ -207,6 +209,14 @@ def trace(f, n_state, n_param, fn_name="sdeh_user_f"):
     tape = _Tape()
     u = _Vec(tape, "u", n_state, "u")
     p = _Vec(tape, "p", n_param, "p")
+    if scalar_u:
+        if n_state != 1:
+            raise TraceError("a scalar state has one component")
+        u = u[0]
+    if scalar_p:
+        if n_param != 1:
+            raise TraceError("a scalar parameter has one component")
+        p = p[0]
     t = Sym(tape, "(T)t")
     out = f(u, p, t)
     out = list(out) if not isinstance(out, Sym) else [out]

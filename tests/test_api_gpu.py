@@ -122,3 +122,14 @@ def test_dtmin_raises_like_the_reference():
     prob = sd.ODEProblem(sd.rhs.expdecay3, np.ones(3), (0.0, 1.0), [-1.0])
     with pytest.raises(sd.DtLessThanMin, match="dt<dtmin"):
         sd.solve(prob, sd.GPUSimpleATsit5(), dt=1e-15, save_everystep=False)
+
+
+def test_scalar_problem_keeps_scalar_states():
+    """interface_tests.jl:128-132: a Float64-scalar problem keeps scalar states."""
+    prob = sd.ODEProblem(lambda u, p, t: p * u, 1.0, (0.0, 1.0), -1.0)
+    sol = sd.solve(prob, sd.GPUSimpleATsit5(), dt=0.01, abstol=1e-10, reltol=1e-10)
+    assert all(np.ndim(x) == 0 for x in sol.u)
+    assert abs(float(sol.u[-1]) - np.exp(-1.0)) < 1e-8
+    sol = sd.solve(prob, sd.GPUSimpleTsit5(), dt=0.01, save_everystep=False)
+    assert len(sol.u) == 2 and np.ndim(sol.u[-1]) == 0
+    assert abs(float(sol.u[-1]) - np.exp(-1.0)) < 1e-9

@@ -94,3 +94,13 @@ def test_untraceable_functions_raise():
         tracer.trace(lambda u, p, t: [p[3]], 1, 2)
     with pytest.raises(tracer.TraceError):
         tracer.trace(lambda u, p, t: [u[0] ** 0.5], 1, 0)
+
+
+def test_scalar_state_and_parameter():
+    """A scalar problem (u0 and p numbers, as the reference's Float64-scalar tests)
+    traces to the same code as its one-component vector form."""
+    src_s, _ = tracer.trace(lambda u, p, t: p * u + t, 1, 1, "g", scalar_u=True, scalar_p=True)
+    src_v, _ = tracer.trace(lambda u, p, t: [p[0] * u[0] + t], 1, 1, "g")
+    assert src_s == src_v
+    with pytest.raises(tracer.TraceError):
+        tracer.trace(lambda u, p, t: u, 2, 0, "g", scalar_u=True)
