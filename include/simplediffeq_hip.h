@@ -60,7 +60,9 @@ typedef enum sdeh_alg {
   SDEH_ALG_VERN7 = 2,   /* GPUSimpleVern7   gpuvern7.jl:52   */
   SDEH_ALG_AVERN7 = 3,  /* GPUSimpleAVern7  gpuvern7.jl:295  */
   SDEH_ALG_VERN9 = 4,   /* GPUSimpleVern9   gpuvern9.jl:52   */
-  SDEH_ALG_AVERN9 = 5   /* GPUSimpleAVern9  gpuvern9.jl:406  */
+  SDEH_ALG_AVERN9 = 5,  /* GPUSimpleAVern9  gpuvern9.jl:406  */
+  SDEH_ALG_RK4 = 6,     /* GPUSimpleRK4     gpurk4.jl:50     (fixed step, no saveat) */
+  SDEH_ALG_EULER = 7    /* GPUSimpleEuler   gpueuler.jl:50   (fixed step, no saveat) */
 } sdeh_alg;
 
 /* Element type of the state u (the reference's eltype(u0)). */
@@ -129,9 +131,10 @@ typedef struct sdeh_solve_desc {
   const double* saveat;     /* host array of n_saveat save times, ascending, or NULL */
   int64_t n_saveat;
   int64_t maxiters;         /* attempt cap per trajectory (0 = library default 10^7) */
-  int32_t quirks;           /* 1: reproduce the reference's saveat stage quirks
-                               (fixed Vern9 stage aliasing, post-step stage times);
-                               0: corrected dense output */
+  int32_t quirks;           /* 1: reproduce the reference's time quirks (fixed Vern9
+                               saveat stage aliasing, post-step saveat stage times,
+                               RK4/Euler stages from the END of each step);
+                               0: corrected */
   int32_t schedule;         /* sdeh_schedule */
   int32_t precision;        /* sdeh_precision */
   int32_t block;            /* 0 = tuned; else workgroup size (multiple of 64, <= 1024) */

@@ -12,6 +12,8 @@
 //   src/verner/gpuvern7.jl:300-536  GPUSimpleAVern7  -> avern7()
 //   src/verner/gpuvern9.jl:55-353   GPUSimpleVern9   -> vern9_fixed()
 //   src/verner/gpuvern9.jl:411-779  GPUSimpleAVern9  -> avern9()
+//   src/rk4/gpurk4.jl:53-98         GPUSimpleRK4     -> rk4_fixed()
+//   src/euler/gpueuler.jl:53-90     GPUSimpleEuler   -> euler_fixed()
 //   src/SimpleDiffEq.jl:67-77       controller constants -> Ctrl
 //   src/tsit5/tsit5.jl:385-399, src/verner/verner_tableaus.jl:1337-1398  bθs
 // and restates the external semantics they rely on:
@@ -1150,6 +1152,55 @@ done:
   if (!tr.saveat && !tr.save_everystep) push_step<T, N>(tr, s.t, u);
 }
 
+// ===========================================================================
+// GPUSimpleRK4 — gpurk4.jl:53-98, GPUSimpleEuler — gpueuler.jl:53-90
+// Fixed step over ts = tspan[1]:dt:tspan[2]; every grid point is saved (the
+// reference returns us[1..length(ts)] whatever the keywords); no dense output.
+// Each step evaluates at t = ts[i], the END of the step (gpurk4.jl:75-76,
+// gpueuler.jl:73-74): with quirks off, t = ts[i-1] instead.
+// ===========================================================================
+template <class T, class F, bool RK4>
+void rk4_euler_fixed(const F& f, Traj<T>& tr, const Mode& md) {
+  constexpr int N = F::N;
+  const int ar = md.arith;
+  const T half = T(1) / T(2), sixth = T(1) / T(6);                 // gpurk4.jl:70-71
+  const T* p = tr.p;
+  const T dt = (T)tr.dt;
+  const JRange rg = jl_range(tr.t0, tr.dt, tr.tf);                  // :66
+  double t = tr.t0;
+  if (tr.save_everystep && !tr.saveat) push_step<T, N>(tr, rg.len > 0 ? rg.elem(0) : t, tr.u0);
+  T u[N], uprev[N], k1[N], k2[N], k3[N], k4[N], tmp[N];
+  cpy<T, N>(u, tr.u0);
+  for (int64_t i = 1; i < rg.len; ++i) {                            // :73
+    cpy<T, N>(uprev, u);                                            // :74
+    t = rg.elem(i);                                                 // :75, ts[i]
+    const double te = md.quirks ? t : rg.elem(i - 1);               // evaluation time
+    f(k1, u, p, te);                                                // :76
+    if constexpr (RK4) {
+      comb1<T, N>(ar, tmp, uprev, dt * half, k1);                   // :77
+      f(k2, tmp, p, stage_t(te, half, dt));                         // :78
+      comb1<T, N>(ar, tmp, uprev, dt * half, k2);                   // :79
+      f(k3, tmp, p, stage_t(te, half, dt));                         // :80
+      comb1<T, N>(ar, tmp, uprev, dt, k3);                          // :81
+      f(k4, tmp, p, te + (double)dt);                               // :82
+      // u = uprev + dt*sixth*(k1 + 2k2 + 2k3 + k4)  (:83); the doublings are exact,
+      // so the fused and plain readings of the inner sum agree.
+      for (int c = 0; c < N; ++c) {
+        const T s4 = ((k1[c] + T(2) * k2[c]) + T(2) * k3[c]) + k4[c];
+        u[c] = (ar == ARITH_FUSED) ? std::fma(dt * sixth, s4, uprev[c])
+                                   : uprev[c] + (dt * sixth) * s4;
+      }
+    } else {
+      comb1<T, N>(ar, u, uprev, dt, k1);                            // gpueuler.jl:75
+    }
+    tr.naccept++;
+    if (tr.save_everystep && !tr.saveat) push_step<T, N>(tr, t, u);  // us[i] = u
+  }
+  cpy<T, N>(tr.u_end, u);
+  tr.t_end = rg.len > 0 ? rg.elem(rg.len - 1) : t;
+  if (!tr.save_everystep && !tr.saveat) push_step<T, N>(tr, tr.t_end, u);
+}
+
 template <class T, class F>
 inline void solve_one(int alg, const F& f, Traj<T>& tr, const Mode& md) {
   switch (alg) {
@@ -1159,6 +1210,8 @@ inline void solve_one(int alg, const F& f, Traj<T>& tr, const Mode& md) {
     case SDEH_ALG_AVERN7: avern7<T, F>(f, tr, md); break;
     case SDEH_ALG_VERN9: vern9_fixed<T, F>(f, tr, md); break;
     case SDEH_ALG_AVERN9: avern9<T, F>(f, tr, md); break;
+    case SDEH_ALG_RK4: rk4_euler_fixed<T, F, true>(f, tr, md); break;
+    case SDEH_ALG_EULER: rk4_euler_fixed<T, F, false>(f, tr, md); break;
     default: break;
   }
 }

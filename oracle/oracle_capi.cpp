@@ -131,7 +131,7 @@ extern "C" {
 // pow_kind: POW_PORTABLE (0) or POW_LIBM (1); nthreads: worker threads.
 int oracle_solve(const sdeh_solve_desc* d, const sdeh_io* io, int arith, int pow_kind,
                  int nthreads) {
-  if (!d || !io || d->n_traj < 0 || d->alg < 0 || d->alg > 5) return SDEH_E_INVALID;
+  if (!d || !io || d->n_traj < 0 || d->alg < 0 || d->alg > SDEH_ALG_EULER) return SDEH_E_INVALID;
   if (d->rhs == SDEH_RHS_GLV16 && !io->p_shared) return SDEH_E_INVALID;
   Mode md;
   md.arith = arith;

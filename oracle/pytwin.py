@@ -11,6 +11,8 @@ C++ oracle (ARITH_PLAIN, POW_LIBM) and this module must agree bit for bit.
   GPUSimpleATsit5 src/tsit5/gpuatsit5.jl:205-336  atsit5
   GPUSimpleAVern7 src/verner/gpuvern7.jl:300-536  avern7   (endpoint)
   GPUSimpleAVern9 src/verner/gpuvern9.jl:411-779  avern9   (endpoint)
+  GPUSimpleRK4    src/rk4/gpurk4.jl:53-98         rk4      (every grid point)
+  GPUSimpleEuler  src/euler/gpueuler.jl:53-90     euler    (every grid point)
   Julia Float64 ranges (base/twiceprecision.jl)    julia_range
 
 Coefficients are read from the product's coefficient header, which the
@@ -322,3 +324,49 @@ def avern7(f, u0, p, t0, tf, dt, abstol, reltol, maxiters=10 ** 7):
 def avern9(f, u0, p, t0, tf, dt, abstol, reltol, maxiters=10 ** 7):
     f7 = float(np.float32(1.0e-7))  # gpuvern9.jl:466,591 use the Float32 literal 1f-7
     return _vern_adaptive(f, u0, p, t0, tf, dt, abstol, reltol, "vern9", f7, f7, maxiters)
+
+
+def _fma(a, b, c):
+    """a*b + c rounded once (exact rational arithmetic, then one rounding)."""
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def affine(u, p, t):
+    """u' = p0 u + p1 t (scalar, non-autonomous)."""
+    return [p[0] * u[0] + p[1] * t]
+
+
+def _fixed_grid_solve(step, f, u0, p, t0, tf, dt, quirks):
+    """Iterate `step` over julia_range(t0, dt, tf); f is evaluated from ts[i]
+    (the end of step i, gpurk4.jl:75 / gpueuler.jl:73) or, with quirks off,
+    from ts[i-1].  Returns the grid and the state at every grid point."""
+    grid = julia_range(t0, dt, tf)
+    us = [list(u0)]
+    u = list(u0)
+    for i in range(1, len(grid)):
+        te = grid[i] if quirks else grid[i - 1]
+        u = step(f, u, p, te, dt)
+        us.append(u)
+    return grid, us
+
+
+def _rk4_step(f, uprev, p, t, dt):
+    half, sixth = 1.0 / 2.0, 1.0 / 6.0
+    k1 = f(uprev, p, t)
+    k2 = f(_one(uprev, dt * half, k1), p, _fma(half, dt, t))
+    k3 = f(_one(uprev, dt * half, k2), p, _fma(half, dt, t))
+    k4 = f(_one(uprev, dt, k3), p, t + dt)
+    return [uprev[c] + (dt * sixth) * (((k1[c] + 2.0 * k2[c]) + 2.0 * k3[c]) + k4[c])
+            for c in range(len(uprev))]
+
+
+def _euler_step(f, uprev, p, t, dt):
+    return _one(uprev, dt, f(uprev, p, t))
+
+
+def rk4(f, u0, p, t0, tf, dt, quirks=True):
+    return _fixed_grid_solve(_rk4_step, f, u0, p, t0, tf, dt, quirks)
+
+
+def euler(f, u0, p, t0, tf, dt, quirks=True):
+    return _fixed_grid_solve(_euler_step, f, u0, p, t0, tf, dt, quirks)

@@ -54,12 +54,17 @@ const void* adaptive_or_null() {
 template <class T, int ALG>
 const void* by_save(int save, int kahan) {
   constexpr bool fixed = (ALG == sdeh::ALG_TSIT5 || ALG == sdeh::ALG_VERN7 ||
-                          ALG == sdeh::ALG_VERN9);
+                          ALG == sdeh::ALG_VERN9 || ALG == sdeh::ALG_RK4 ||
+                          ALG == sdeh::ALG_EULER);
+  // GPUSimpleRK4 and GPUSimpleEuler have no interpolant: no saveat kernel.
+  constexpr bool dense = !(ALG == sdeh::ALG_RK4 || ALG == sdeh::ALG_EULER);
   if constexpr (fixed) {
     if (kahan) return nullptr;
     switch (save) {
       case sdeh::SAVE_END: return fixed_or_null<T, ALG, sdeh::SAVE_END>();
-      case sdeh::SAVE_AT: return fixed_or_null<T, ALG, sdeh::SAVE_AT>();
+      case sdeh::SAVE_AT:
+        if constexpr (dense) return fixed_or_null<T, ALG, sdeh::SAVE_AT>();
+        return nullptr;
       case sdeh::SAVE_STEPS: return fixed_or_null<T, ALG, sdeh::SAVE_STEPS>();
     }
     return nullptr;
@@ -89,6 +94,8 @@ const void* by_alg(int alg, int save, int kahan) {
     case sdeh::ALG_AVERN7: return by_save<T, sdeh::ALG_AVERN7>(save, kahan);
     case sdeh::ALG_VERN9: return by_save<T, sdeh::ALG_VERN9>(save, kahan);
     case sdeh::ALG_AVERN9: return by_save<T, sdeh::ALG_AVERN9>(save, kahan);
+    case sdeh::ALG_RK4: return by_save<T, sdeh::ALG_RK4>(save, kahan);
+    case sdeh::ALG_EULER: return by_save<T, sdeh::ALG_EULER>(save, kahan);
   }
   return nullptr;
 }

@@ -7,6 +7,8 @@
 //   GPUSimpleAVern7  src/verner/gpuvern7.jl:300-536   sdeh_adaptive<..., ALG_AVERN7>
 //   GPUSimpleVern9   src/verner/gpuvern9.jl:55-353    sdeh_fixed<..., ALG_VERN9>
 //   GPUSimpleAVern9  src/verner/gpuvern9.jl:411-779   sdeh_adaptive<..., ALG_AVERN9>
+//   GPUSimpleRK4     src/rk4/gpurk4.jl:53-98          sdeh_fixed<..., ALG_RK4>
+//   GPUSimpleEuler   src/euler/gpueuler.jl:53-90      sdeh_fixed<..., ALG_EULER>
 //
 // Execution model.  The reference runs one sequential `while t < tf / while
 // EEst > 1` nest per trajectory.  Here a lane owns a trajectory and the nest is
@@ -46,7 +48,7 @@
 namespace sdeh {
 
 enum { ALG_TSIT5 = 0, ALG_ATSIT5 = 1, ALG_VERN7 = 2, ALG_AVERN7 = 3, ALG_VERN9 = 4,
-       ALG_AVERN9 = 5 };
+       ALG_AVERN9 = 5, ALG_RK4 = 6, ALG_EULER = 7 };
 enum { SAVE_END = 0, SAVE_AT = 1, SAVE_STEPS = 2 };
 enum { RET_DEFAULT = 0, RET_DTMIN = 1, RET_NAN = 2, RET_MAXITERS = 3, RET_SAVE_OVERFLOW = 4 };
 
@@ -955,7 +957,34 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
   for (int64_t i = 1; i < a.g_len; ++i) {
     copyv(uprev, u);
     t = grid_elem(a, i - 1);  // gpuatsit5.jl:98
-    if constexpr (ALG == ALG_TSIT5) {
+    if constexpr (ALG == ALG_RK4 || ALG == ALG_EULER) {
+      // gpurk4.jl:75-82 / gpueuler.jl:73-75 evaluate f from t = ts[i], the END of
+      // the step (quirk); with quirks off, from its start ts[i-1].
+      const double t_next = grid_elem(a, i);
+      const double te = a.quirks ? t_next : t;
+      T k1[N];
+      R::f(k1, u, pl, ps, te);
+      if constexpr (ALG == ALG_RK4) {
+        const T half = T(1) / T(2), sixth = T(1) / T(6);  // gpurk4.jl:70-71
+        T tmp[N], k2[N], k3[N], k4[N];
+        comb1(tmp, uprev, dt * half, k1);
+        R::f(k2, tmp, pl, ps, stage_t(te, (double)half, (double)dt));
+        comb1(tmp, uprev, dt * half, k2);
+        R::f(k3, tmp, pl, ps, stage_t(te, (double)half, (double)dt));
+        comb1(tmp, uprev, dt, k3);
+        R::f(k4, tmp, pl, ps, te + (double)dt);
+        // u = uprev + dt*sixth*(k1 + 2k2 + 2k3 + k4), the sum left to right
+        // (the doublings are exact)
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+          const T s4 = ((k1[c] + T(2) * k2[c]) + T(2) * k3[c]) + k4[c];
+          u[c] = fma(dt * sixth, s4, uprev[c]);
+        }
+      } else {
+        comb1(u, uprev, dt, k1);
+      }
+      t = t_next;  // us[i] belongs to ts[i]
+    } else if constexpr (ALG == ALG_TSIT5) {
       T k1[N], k2[N], k3[N], k4[N], k5[N], k6[N];
       copyv(k1, k7);
       tsit5_stages<T, R, false>(uprev, k1, k2, k3, k4, k5, k6, k7, u, zero, dummy, pl, ps, t, dt);

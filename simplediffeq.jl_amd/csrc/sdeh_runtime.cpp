@@ -402,7 +402,7 @@ struct Plan {
 // Validate the descriptor and fill everything except the device pointers.
 int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* pl) {
   if (!d || !io) return fail(c, SDEH_E_INVALID, "desc and io must not be NULL");
-  if (d->alg < SDEH_ALG_TSIT5 || d->alg > SDEH_ALG_AVERN9)
+  if (d->alg < SDEH_ALG_TSIT5 || d->alg > SDEH_ALG_EULER)
     return fail(c, SDEH_E_INVALID, "alg %d out of range", d->alg);
   if (d->dtype != SDEH_F64 && d->dtype != SDEH_F32)
     return fail(c, SDEH_E_INVALID, "dtype %d out of range", d->dtype);
@@ -414,6 +414,10 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
                 (long long)pl->n_shared, (long long)io->n_shared);
   pl->es = d->dtype == SDEH_F64 ? 8 : 4;
   const bool adaptive = is_adaptive(d->alg);
+  if ((d->alg == SDEH_ALG_RK4 || d->alg == SDEH_ALG_EULER) && d->saveat && d->n_saveat > 0)
+    return fail(c, SDEH_E_UNSUPPORTED,
+                "GPUSimpleRK4 and GPUSimpleEuler have no dense output (saveat); the reference "
+                "returns every grid point (gpurk4.jl:68-86): use save_everystep");
   if (d->saveat && d->n_saveat > 0) {
     pl->save = SAVE_AT;
     for (int64_t j = 1; j < d->n_saveat; ++j)

@@ -21,7 +21,8 @@ STATUS_NAMES = {0: "OK", -1: "INVALID", -2: "HIP", -3: "COMPILE", -4: "UNSUPPORT
                 -5: "NOMEM", -6: "DTMIN", -7: "SAVE_OVERFLOW"}
 
 # sdeh_alg
-ALG_TSIT5, ALG_ATSIT5, ALG_VERN7, ALG_AVERN7, ALG_VERN9, ALG_AVERN9 = range(6)
+ALG_TSIT5, ALG_ATSIT5, ALG_VERN7, ALG_AVERN7, ALG_VERN9, ALG_AVERN9, ALG_RK4, ALG_EULER = range(8)
+FIXED_GRID_ONLY_ALGS = (ALG_RK4, ALG_EULER)  # no interpolant: every grid point is returned
 ADAPTIVE_ALGS = (ALG_ATSIT5, ALG_AVERN7, ALG_AVERN9)
 
 # sdeh_dtype

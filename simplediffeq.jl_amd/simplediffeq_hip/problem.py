@@ -11,7 +11,7 @@ with the reference's keyword defaults (gpuatsit5.jl:55-61, 205-212:
 dt = 0.1f0, abstol = 1f-6, reltol = 1f-3, save_everystep = true,
 saveat = nothing) and its results: per trajectory an ODESolution with t, u
 and retcode "Default", or the error "dt<dtmin" the reference throws
-(gpuatsit5.jl:256).  The algorithm structs are the six GPUSimple* types.
+(gpuatsit5.jl:256).  The algorithm structs are the eight GPUSimple* types.
 
 Every solve here runs on the HIP library (one trajectory per GPU lane); there
 is no CPU fallback.  Right-hand sides: the library's built-ins (rhs.lorenz,
@@ -26,11 +26,13 @@ from . import _abi, runtime, tracer
 
 
 # ---------------------------------------------------------------------------
-# Algorithms (gpuatsit5.jl:52,200; gpuvern7.jl:52,295; gpuvern9.jl:52,406)
+# Algorithms (gpuatsit5.jl:52,200; gpuvern7.jl:52,295; gpuvern9.jl:52,406;
+# gpurk4.jl:50; gpueuler.jl:50)
 # ---------------------------------------------------------------------------
 class _Alg:
     code = None
     adaptive = False
+    dt_required = False
 
     def __repr__(self):
         return type(self).__name__ + "()"
@@ -61,6 +63,21 @@ class GPUSimpleVern9(_Alg):
 class GPUSimpleAVern9(_Alg):
     code = _abi.ALG_AVERN9
     adaptive = True
+
+
+class GPUSimpleRK4(_Alg):
+    """Classic fourth-order Runge-Kutta on the grid tspan[0]:dt:tspan[1]
+    (gpurk4.jl:53-98).  dt is required; the solution holds every grid point,
+    whatever save_everystep or saveat say (the reference ignores them)."""
+    code = _abi.ALG_RK4
+    dt_required = True
+
+
+class GPUSimpleEuler(_Alg):
+    """Explicit Euler on the grid tspan[0]:dt:tspan[1] (gpueuler.jl:53-90); dt
+    is required and every grid point is returned, as GPUSimpleRK4."""
+    code = _abi.ALG_EULER
+    dt_required = True
 
 
 def isadaptive(alg):
@@ -227,15 +244,21 @@ def _f32(x):
 # ---------------------------------------------------------------------------
 # solve
 # ---------------------------------------------------------------------------
-def solve(prob, alg, ensemblealg=None, *, trajectories=None, dt=_f32(0.1), saveat=None,
+def solve(prob, alg, ensemblealg=None, *, trajectories=None, dt=None, saveat=None,
           save_everystep=True, abstol=_f32(1e-6), reltol=_f32(1e-3), max_saves=None,
           dtype=None, maxiters=0, quirks=True, dt0s=None, order=None, **kwargs):
     """solve(ODEProblem, alg; ...) -> ODESolution
     solve(EnsembleProblem, alg, EnsembleHIP(); trajectories, ...) -> EnsembleSolution
 
-    Unknown keywords are accepted and ignored, as the reference's `kwargs...`."""
+    Unknown keywords are accepted and ignored, as the reference's `kwargs...`.
+    dt defaults to 0.1f0, except for GPUSimpleRK4 and GPUSimpleEuler, which
+    raise "dt is required for this algorithm" without it (gpurk4.jl:56)."""
     if not isinstance(alg, _Alg):
         raise TypeError("alg must be one of the GPUSimple* algorithms")
+    if dt is None:
+        if alg.dt_required:
+            raise ValueError("dt is required for this algorithm")
+        dt = _f32(0.1)
     single = isinstance(prob, ODEProblem)
     if single:
         eprob = EnsembleProblem(prob)
@@ -281,6 +304,11 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, dt=_f32(0.1), savea
     if ps is None:
         ps = np.zeros((0, n))
     everystep = saveat is None and bool(save_everystep)
+    if alg.code in _abi.FIXED_GRID_ONLY_ALGS:
+        # gpurk4.jl:66-88 / gpueuler.jl: the solution is (ts, us) over the whole
+        # grid; save_everystep and saveat arrive in kwargs... and are ignored.
+        saveat, everystep = None, True
+        max_saves = max(int(runtime.fixed_grid(base.tspan[0], float(dt), base.tspan[1])[0]), 1)
     if everystep and max_saves is None:
         max_saves = 4096
     saveat_arr = None
