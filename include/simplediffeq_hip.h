@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define SDEH_ABI_VERSION 2
+#define SDEH_ABI_VERSION 3
 
 typedef enum sdeh_status {
   SDEH_OK = 0,
@@ -74,7 +74,8 @@ typedef enum sdeh_retcode {
   SDEH_RET_DTLESSTHANMIN = 1,  /* the reference's error("dt<dtmin") */
   SDEH_RET_NAN = 2,            /* error estimate or time became NaN (reference: unspecified) */
   SDEH_RET_MAXITERS = 3,       /* attempt cap reached (the reference has no cap) */
-  SDEH_RET_SAVE_OVERFLOW = 4   /* save_everystep output exceeded max_saves; integration completed */
+  SDEH_RET_SAVE_OVERFLOW = 4   /* save_everystep output exceeded max_saves (or, packed,
+                                  steps_capacity); integration completed */
 } sdeh_retcode;
 
 /* Built-in right-hand sides (compiled ahead of time into the library).
@@ -92,11 +93,23 @@ typedef enum sdeh_builtin_rhs {
 typedef enum sdeh_save_layout {
   SDEH_SAVE_POINT_MAJOR = 0, /* u_saveat [n_saveat][n_state][n_traj], u_steps
                                 [max_saves][n_state][n_traj], t_steps [max_saves][n_traj] */
-  SDEH_SAVE_TRAJ_MAJOR = 1   /* u_saveat [n_traj][n_saveat][n_state], u_steps
+  SDEH_SAVE_TRAJ_MAJOR = 1,  /* u_saveat [n_traj][n_saveat][n_state], u_steps
                                 [n_traj][max_saves][n_state], t_steps [n_traj][max_saves]:
                                 each trajectory's saves contiguous, as the reference's
                                 per-trajectory solution holds them; a lane writes whole
                                 rows, which the GPU stores far faster */
+  SDEH_SAVE_PACKED = 2       /* save_everystep only, no per-trajectory cap (max_saves is
+                                not used): trajectory i's points are rows
+                                steps_offset[i] .. steps_offset[i+1]-1 of u_steps
+                                [steps_capacity][n_state] and t_steps [steps_capacity];
+                                n_steps[i] = their count.  The reference's per-trajectory
+                                `us`/`ts` vectors grow with push! (gpuatsit5.jl:113-114,
+                                302-303); here one pass saves into chunks claimed from a device
+                                pool and a compaction lays them out contiguously.  When
+                                the points do not fit steps_capacity, trajectories keep a
+                                prefix of their points (retcode SAVE_OVERFLOW; which ones
+                                lose points depends on the order chunks were claimed)
+                                and stats.steps_needed gives the capacity that fits. */
 } sdeh_save_layout;
 
 /* Scheduling of trajectories onto lanes. */
@@ -173,6 +186,9 @@ typedef struct sdeh_io {
   void* u_steps;           /* [max_saves][n_state][n_traj] save_everystep states */
   double* t_steps;         /* [max_saves][n_traj] save_everystep times */
   int32_t* n_steps;        /* [n_traj] number of saved steps (u0 included) */
+  int64_t* steps_offset;   /* SDEH_SAVE_PACKED: [n_traj + 1] first row of each
+                              trajectory; steps_offset[n_traj] = rows used */
+  int64_t steps_capacity;  /* SDEH_SAVE_PACKED: rows of u_steps and t_steps */
 } sdeh_io;
 
 /* Timing and work counters of the last sdeh_solve() on a context. */
@@ -183,6 +199,9 @@ typedef struct sdeh_stats {
   int64_t accepts;
   int32_t block, grid, chunk, schedule; /* launch shape actually used */
   int32_t lanes;           /* lanes per trajectory used */
+  int32_t reserved0;
+  int64_t steps_needed;    /* SDEH_SAVE_PACKED: total points offered by all
+                              trajectories (the steps_capacity that holds them all) */
 } sdeh_stats;
 
 typedef struct sdeh_ctx sdeh_ctx;

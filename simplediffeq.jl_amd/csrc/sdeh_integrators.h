@@ -602,6 +602,71 @@ __device__ __forceinline__ void store_save_time(const sdeh_kargs& a, int64_t row
   else a.t_steps[row * a.n + traj] = t;
 }
 
+// save_everystep output of one trajectory.  Fixed rows: point k goes to row k of
+// the trajectory's max_saves rows.  Packed (a.pk_chunk > 0): points fill chunks of
+// the pool claimed one at a time (one device atomic per chunk, by the group
+// leader); once the pool is exhausted the trajectory stores no more points.
+struct StepSink {
+  int64_t chunk;  // current chunk; -1 before the first claim, -2 once the pool ran out
+  int32_t fill;   // rows used in the current chunk
+  int32_t seq;    // chunks claimed
+  int32_t n;      // points offered
+  int32_t kept;   // points stored
+};
+
+__device__ __forceinline__ void sink_reset(StepSink& s) {
+  s.chunk = -1;
+  s.fill = s.seq = s.n = s.kept = 0;
+}
+
+template <class T, class R>
+__device__ __forceinline__ void save_step(const sdeh_kargs& a, StepSink& s, int64_t traj,
+                                          double t, const T (&u)[R::N]) {
+  const bool lead = lane_comp<R>() == 0;
+  ++s.n;
+  if (a.pk_chunk == 0) {
+    if (s.kept < a.max_saves) {
+      store_save<T, R>(a, a.u_steps, s.kept, a.max_saves, traj, u);
+      if (lead) store_save_time(a, s.kept, traj, t);
+      ++s.kept;
+    }
+    return;
+  }
+  if (s.chunk == -2) return;
+  if (s.chunk < 0 || s.fill == a.pk_chunk) {
+    long long c = 0;
+    if (lead) c = (long long)atomicAdd(a.pk_next, 1ull);
+    if constexpr (R::G > 1)  // the group follows its leader's claim
+      c = __shfl(c, (int)(__lane_id() & ~(unsigned)(R::G - 1)));
+    if (c >= a.pk_chunks) {
+      s.chunk = -2;
+      return;
+    }
+    s.chunk = c;
+    s.fill = 0;
+    if (lead) {
+      a.pk_traj[c] = traj;
+      a.pk_seq[c] = s.seq;
+    }
+    ++s.seq;
+  }
+  constexpr int FULL = R::N * R::G;
+  const int64_t row = s.chunk * a.pk_chunk + s.fill;
+  T* o = (T*)a.u_steps + row * FULL + lane_comp<R>() * R::N;
+#pragma unroll
+  for (int c = 0; c < R::N; ++c) o[c] = u[c];
+  if (lead) a.t_steps[row] = t;
+  ++s.fill;
+  ++s.kept;
+}
+
+// Leader's end-of-trajectory writes of the save_everystep counts.
+__device__ __forceinline__ void finish_steps(const sdeh_kargs& a, const StepSink& s,
+                                             int64_t traj) {
+  a.n_steps[traj] = s.kept;
+  if (a.pk_chunk) a.pk_need[traj] = s.n;
+}
+
 template <class T, class R>
 __device__ __forceinline__ void load_state(T (&u)[R::N], const void* base, int64_t n,
                                            int64_t traj) {
@@ -699,7 +764,9 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
   T uprev[N], comp[N], fsal[N], pl[P];
   double t = 0.0, tf = 0.0;
   T dt = T(0), qold = T(0), qold_pow = T(0);
-  int32_t nacc = 0, nrej = 0, cur = 0, nsaved = 0;
+  int32_t nacc = 0, nrej = 0, cur = 0;
+  StepSink sk;
+  sink_reset(sk);
   double ts_next = 0.0;  // save_time(cur)
   int64_t attempts = 0, traj = -1;
   bool active = false;
@@ -744,7 +811,6 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
       nacc = nrej = 0;
       attempts = 0;
       cur = 0;
-      nsaved = 0;
       if constexpr (SAVE == SAVE_AT) {
         if (a.n_saveat > 0 && t == a.saveat[0]) {
           store_save<T, R>(a, a.u_saveat, 0, a.n_saveat, traj, uprev);
@@ -753,11 +819,8 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
         ts_next = save_time(a, cur);
       }
       if constexpr (SAVE == SAVE_STEPS) {
-        if (a.max_saves > 0) {
-          store_save<T, R>(a, a.u_steps, 0, a.max_saves, traj, uprev);
-          if (lead) store_save_time(a, 0, traj, t);
-        }
-        nsaved = 1;
+        sink_reset(sk);
+        save_step<T, R>(a, sk, traj, t, uprev);
       }
       if constexpr (ALG == ALG_ATSIT5) R::f(fsal, uprev, pl, ps, t);  // FSAL seed, :241
     }
@@ -873,13 +936,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
               }
             }
           }
-          if constexpr (SAVE == SAVE_STEPS) {
-            if (nsaved < a.max_saves) {
-              store_save<T, R>(a, a.u_steps, nsaved, a.max_saves, traj, u);
-              if (lead) store_save_time(a, nsaved, traj, t);
-            }
-            ++nsaved;
-          }
+          if constexpr (SAVE == SAVE_STEPS) save_step<T, R>(a, sk, traj, t, u);
           copyv(uprev, u);
           if constexpr (ALG == ALG_ATSIT5) copyv(fsal, k7);
           if constexpr (KAHAN) copyv(comp, comp_new);
@@ -889,16 +946,14 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
     }
 
     if (ret >= 0) {  // trajectory ends: write outputs (gpuatsit5.jl:322-329)
-      if (SAVE == SAVE_STEPS && ret == RET_DEFAULT && nsaved > a.max_saves)
-        ret = RET_SAVE_OVERFLOW;
+      if (SAVE == SAVE_STEPS && ret == RET_DEFAULT && sk.kept < sk.n) ret = RET_SAVE_OVERFLOW;
       store_state<T, R>(a.u_end, 0, n, traj, uprev);
       if (lead) {
         a.t_end[traj] = t;
         a.retcode[traj] = ret;
         if (a.naccept) a.naccept[traj] = nacc;
         if (a.nreject) a.nreject[traj] = nrej;
-        if constexpr (SAVE == SAVE_STEPS)
-          a.n_steps[traj] = nsaved < a.max_saves ? nsaved : a.max_saves;
+        if constexpr (SAVE == SAVE_STEPS) finish_steps(a, sk, traj);
       }
       active = false;
     }
@@ -936,7 +991,9 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
   load_params<T, R>(pl, a, traj);
   const T dt = (T)a.dt;
   double t = a.t0;
-  int32_t cur = 0, nsaved = 0;
+  int32_t cur = 0;
+  StepSink sk;
+  sink_reset(sk);
   double ts_next = 0.0;  // save_time(cur)
   if constexpr (SAVE == SAVE_AT) {
     if (a.n_saveat > 0 && t == a.saveat[0]) {
@@ -945,13 +1002,7 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
     }
     ts_next = save_time(a, cur);
   }
-  if constexpr (SAVE == SAVE_STEPS) {
-    if (a.max_saves > 0) {
-      store_save<T, R>(a, a.u_steps, 0, a.max_saves, traj, u);
-      if (lead) store_save_time(a, 0, traj, t);
-    }
-    nsaved = 1;
-  }
+  if constexpr (SAVE == SAVE_STEPS) save_step<T, R>(a, sk, traj, t, u);
   T k7[N];
   if constexpr (ALG == ALG_TSIT5) R::f(k7, u, pl, ps, t);  // gpuatsit5.jl:86
   for (int64_t i = 1; i < a.g_len; ++i) {
@@ -1042,23 +1093,17 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
         }
       }
     }
-    if constexpr (SAVE == SAVE_STEPS) {
-      if (nsaved < a.max_saves) {
-        store_save<T, R>(a, a.u_steps, nsaved, a.max_saves, traj, u);
-        if (lead) store_save_time(a, nsaved, traj, t);
-      }
-      ++nsaved;
-    }
+    if constexpr (SAVE == SAVE_STEPS) save_step<T, R>(a, sk, traj, t, u);
   }
   store_state<T, R>(a.u_end, 0, n, traj, u);
   if (!lead) return;
   a.t_end[traj] = t;
   int ret = RET_DEFAULT;
-  if (SAVE == SAVE_STEPS && nsaved > a.max_saves) ret = RET_SAVE_OVERFLOW;
+  if (SAVE == SAVE_STEPS && sk.kept < sk.n) ret = RET_SAVE_OVERFLOW;
   a.retcode[traj] = ret;
   if (a.naccept) a.naccept[traj] = (int32_t)(a.g_len > 0 ? a.g_len - 1 : 0);
   if (a.nreject) a.nreject[traj] = 0;
-  if constexpr (SAVE == SAVE_STEPS) a.n_steps[traj] = nsaved < a.max_saves ? nsaved : a.max_saves;
+  if constexpr (SAVE == SAVE_STEPS) finish_steps(a, sk, traj);
 }
 
 }  // namespace sdeh

@@ -45,4 +45,16 @@ struct sdeh_kargs {
   int32_t persistent;      // 1: lanes refill from *queue in chunks of `chunk`
   int32_t chunk;
   int32_t traj_major;      // saved-state outputs [n][rows][N] instead of [rows][N][n]
+  // Packed save_everystep (pk_chunk > 0): u_steps/t_steps are a pool of pk_chunks
+  // chunks of pk_chunk rows ([row][N] states, [row] times).  A trajectory claims its
+  // chunks one at a time from *pk_next; chunk c holds rows pk_seq[c]*pk_chunk ...
+  // of trajectory pk_traj[c].  pk_need[traj] receives the number of points the
+  // trajectory offered (stored or not).
+  int32_t pk_chunk;
+  int32_t pk_pad_;
+  int64_t pk_chunks;
+  unsigned long long* pk_next;
+  int64_t* pk_traj;
+  int32_t* pk_seq;
+  int32_t* pk_need;
 };

@@ -43,6 +43,12 @@ SDEH_HIDDEN hipError_t sdeh_util_count(const int32_t*, const int32_t*, const int
                                        unsigned long long*, hipStream_t);
 SDEH_HIDDEN hipError_t sdeh_util_check_order(const int64_t*, int64_t, int32_t*, unsigned long long*,
                                              hipStream_t);
+SDEH_HIDDEN hipError_t sdeh_util_pack(int32_t*, int32_t*, int64_t, int64_t*, int64_t, int64_t*,
+                                      const void*, const double*, const int64_t*, const int32_t*,
+                                      const unsigned long long*, int64_t, int, int, int, void*,
+                                      double*, hipStream_t);
+SDEH_HIDDEN hipError_t sdeh_util_sum_i32(const int32_t*, int64_t, unsigned long long*,
+                                         hipStream_t);
 SDEH_HIDDEN int sdeh_mlp_forward_impl(const float*, const float*, float*, int64_t, hipStream_t);
 SDEH_HIDDEN int sdeh_mlp_loss_grad_impl(const float*, const float*, const float*, int64_t, float*,
                                         float*, float*, hipStream_t);
@@ -58,6 +64,9 @@ namespace {
 enum { SAVE_END = 0, SAVE_AT = 1, SAVE_STEPS = 2 };
 constexpr int64_t kUserBase = 1000;
 constexpr int64_t kDefaultMaxiters = 10000000;
+// Rows per chunk of the packed save_everystep pool: one device atomic per this
+// many saved points, and at most this many rows of slack per trajectory.
+constexpr int kPackChunk = 8;
 
 struct Builtin {
   int64_t id;
@@ -131,7 +140,9 @@ struct sdeh_ctx {
   int cus = 0;
   char* arena = nullptr;
   size_t arena_cap = 0;
-  unsigned long long* counters = nullptr;  // [0] queue, [1..4] count_kernel outputs
+  // [0] queue, [1..4] count_kernel outputs, [5] order check, [6] packed points
+  // offered, [7] packed chunk claims
+  unsigned long long* counters = nullptr;
   std::vector<UserRhs> user;
   std::map<std::string, std::pair<hipModule_t, hipFunction_t>> rtc;
   std::map<TuneKey, Shape> tuned;
@@ -394,6 +405,14 @@ struct Plan {
   int save = SAVE_END;
   int kahan = 0;
   size_t es = 8;  // element size of the state type
+  // packed save_everystep: the kernel fills the pool (a.u_steps/a.t_steps); the
+  // finishing passes write these
+  bool packed = false;
+  int64_t pk_cap = 0;
+  int64_t* pk_off = nullptr;
+  void* pk_out_u = nullptr;
+  double* pk_out_t = nullptr;
+  int64_t* pk_tiles = nullptr;
   KernelRef k;
   Shape shape;
   sdeh_kargs a{};
@@ -425,7 +444,7 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
         return fail(c, SDEH_E_INVALID, "saveat must be ascending (entry %lld)", (long long)j);
   } else if (d->save_everystep) {
     pl->save = SAVE_STEPS;
-    if (d->max_saves < 1)
+    if (d->save_layout != SDEH_SAVE_PACKED && d->max_saves < 1)
       return fail(c, SDEH_E_INVALID,
                   "save_everystep needs max_saves >= 1 and u_steps/t_steps/n_steps buffers "
                   "(set save_everystep = 0 for endpoint-only output)");
@@ -444,8 +463,15 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
   if (d->block != 0 && (d->block < 64 || d->block > 256 || d->block % 64 != 0))
     return fail(c, SDEH_E_INVALID, "block must be 64, 128, 192 or 256");
   if (d->grid < 0 || d->chunk < 0) return fail(c, SDEH_E_INVALID, "grid/chunk must be >= 0");
-  if (d->save_layout != SDEH_SAVE_POINT_MAJOR && d->save_layout != SDEH_SAVE_TRAJ_MAJOR)
+  if (d->save_layout != SDEH_SAVE_POINT_MAJOR && d->save_layout != SDEH_SAVE_TRAJ_MAJOR &&
+      d->save_layout != SDEH_SAVE_PACKED)
     return fail(c, SDEH_E_INVALID, "save_layout %d out of range", d->save_layout);
+  pl->packed = d->save_layout == SDEH_SAVE_PACKED;
+  if (pl->packed && pl->save != SAVE_STEPS)
+    return fail(c, SDEH_E_INVALID,
+                "the packed layout holds save_everystep output (save_everystep = 1, no saveat)");
+  if (pl->packed && io->steps_capacity < 0)
+    return fail(c, SDEH_E_INVALID, "steps_capacity < 0");
 
   sdeh_kargs& a = pl->a;
   a.n = d->n_traj;
@@ -456,9 +482,16 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
   a.dt = d->dt;
   a.abstol = d->abstol;
   a.reltol = d->reltol;
-  a.max_saves = pl->save == SAVE_STEPS ? d->max_saves : 0;
+  a.max_saves = pl->save == SAVE_STEPS && !pl->packed ? d->max_saves : 0;
   a.quirks = d->quirks;
   a.traj_major = d->save_layout == SDEH_SAVE_TRAJ_MAJOR;
+  if (pl->packed) {
+    pl->pk_cap = io->steps_capacity;
+    a.pk_chunk = kPackChunk;
+    // every trajectory's last chunk may be partly empty: capacity / chunk chunks
+    // plus one per trajectory hold any ensemble whose points fit the capacity
+    a.pk_chunks = (pl->pk_cap + kPackChunk - 1) / kPackChunk + d->n_traj;
+  }
   if (!adaptive) {
     if (d->dt == 0.0) return fail(c, SDEH_E_INVALID, "range step cannot be zero");
     const sdeh_range::Grid g = sdeh_range::make(d->t0, d->dt, d->tf);
@@ -505,6 +538,8 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
 // Bytes of each io buffer for a plan.
 struct Sizes {
   size_t u0, p, ps, tspan, dt0, order, u_end, t_end, i32, u_saveat, u_steps, t_steps, saveat;
+  // packed layout: offsets output, and the pool the kernel fills
+  size_t pk_off, pool_u, pool_t, pk_traj, pk_seq, pk_tiles;
 };
 
 Sizes sizes_of(const sdeh_solve_desc* d, const sdeh_io* io, const Plan& pl) {
@@ -523,6 +558,17 @@ Sizes sizes_of(const sdeh_solve_desc* d, const sdeh_io* io, const Plan& pl) {
   s.u_steps = pl.save == SAVE_STEPS ? (size_t)d->max_saves * pl.n_state * n * pl.es : 0;
   s.t_steps = pl.save == SAVE_STEPS ? (size_t)d->max_saves * n * 8 : 0;
   s.saveat = pl.save == SAVE_AT ? (size_t)d->n_saveat * 8 : 0;
+  if (pl.packed) {
+    const size_t cap = (size_t)pl.pk_cap, rows = (size_t)pl.a.pk_chunks * kPackChunk;
+    s.u_steps = cap * pl.n_state * pl.es;
+    s.t_steps = cap * 8;
+    s.pk_off = (n + 1) * 8;
+    s.pool_u = rows * pl.n_state * pl.es;
+    s.pool_t = rows * 8;
+    s.pk_traj = (size_t)pl.a.pk_chunks * 8;
+    s.pk_seq = (size_t)pl.a.pk_chunks * 4;
+    s.pk_tiles = ((n + 1023) / 1024 + 1) * 8;
+  }
   return s;
 }
 
@@ -548,11 +594,16 @@ int run_plan(sdeh_ctx* c, Plan& pl, hipStream_t s, unsigned long long* queue, bo
   }
   a.queue = queue;
   if (a.persistent) HIPCHK(c, sdeh_util_zero_u64(queue, 1, s));
+  if (pl.packed && a.n > 0) HIPCHK(c, sdeh_util_zero_u64(a.pk_next, 1, s));
   if (a.n > 0) {
     if (timed) HIPCHK(c, hipEventRecord(c->ev0, s));
     HIPCHK(c, launch(pl.k, pl.shape.grid, pl.shape.block, &a, s));
     if (timed) HIPCHK(c, hipEventRecord(c->ev1, s));
   }
+  if (pl.packed && a.n > 0)
+    HIPCHK(c, sdeh_util_pack(a.n_steps, a.retcode, a.n, pl.pk_tiles, pl.pk_cap, pl.pk_off,
+                             a.u_steps, a.t_steps, a.pk_traj, a.pk_seq, a.pk_next, a.pk_chunks,
+                             a.pk_chunk, pl.n_state, (int)pl.es, pl.pk_out_u, pl.pk_out_t, s));
   return SDEH_OK;
 }
 
@@ -574,6 +625,8 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
       return fail(c, SDEH_E_INVALID, "u_saveat is required with saveat");
     if (pl.save == SAVE_STEPS && (!io->u_steps || !io->t_steps || !io->n_steps))
       return fail(c, SDEH_E_INVALID, "u_steps, t_steps and n_steps are required");
+    if (pl.packed && !io->steps_offset)
+      return fail(c, SDEH_E_INVALID, "steps_offset is required with the packed layout");
   }
   sdeh_kargs& a = pl.a;
   const bool host = memory == SDEH_MEM_HOST;
@@ -594,6 +647,11 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   }
   if (own_counts) need += 2 * round256(sz.i32);
   if (!host && io->order) need += round256(sz.i32);
+  if (pl.packed) {
+    need += round256(sz.pool_u) + round256(sz.pool_t) + round256(sz.pk_traj) +
+            round256(sz.pk_seq) + round256(sz.i32) + round256(sz.pk_tiles);
+    if (host) need += round256(sz.pk_off);
+  }
   r = ensure_arena(c, need);
   if (r) return r;
   Arena ar{c->arena};
@@ -637,6 +695,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
       r = check_dev(c, io->u_steps, sz.u_steps, "u_steps");
       if (!r) r = check_dev(c, io->t_steps, sz.t_steps, "t_steps");
       if (!r) r = check_dev(c, io->n_steps, sz.i32, "n_steps");
+      if (!r && pl.packed) r = check_dev(c, io->steps_offset, sz.pk_off, "steps_offset");
     }
     if (r) return r;
     a.u0 = io->u0;
@@ -657,6 +716,27 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   a.nreject = (!host && io->nreject) ? io->nreject : nullptr;
   if (!a.naccept) a.naccept = (int32_t*)ar.take(sz.i32);
   if (!a.nreject) a.nreject = (int32_t*)ar.take(sz.i32);
+  if (pl.packed) {  // the kernel saves into the pool; the finishing passes fill the outputs
+    pl.pk_out_u = a.u_steps;
+    pl.pk_out_t = a.t_steps;
+    pl.pk_off = host ? (int64_t*)ar.take(sz.pk_off) : io->steps_offset;
+    a.u_steps = ar.take(sz.pool_u);
+    a.t_steps = (double*)ar.take(sz.pool_t);
+    a.pk_traj = (int64_t*)ar.take(sz.pk_traj);
+    a.pk_seq = (int32_t*)ar.take(sz.pk_seq);
+    a.pk_need = (int32_t*)ar.take(sz.i32);
+    pl.pk_tiles = (int64_t*)ar.take(sz.pk_tiles);
+    a.pk_next = c->counters + 7;
+    if (n == 0 && io->steps_offset) {
+      if (host) {
+        io->steps_offset[0] = 0;
+      } else {
+        r = check_dev(c, io->steps_offset, 8, "steps_offset");
+        if (r) return r;
+        HIPCHK(c, hipMemsetAsync(io->steps_offset, 0, 8, s));
+      }
+    }
+  }
 
   if (!host && io->order && n > 0) {  // permutation check on the device
     int32_t* seen = (int32_t*)ar.take(sz.i32);
@@ -671,10 +751,11 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   r = run_plan(c, pl, s, c->counters, true);
   if (r) return r;
   const bool do_sync = host || sync;
-  unsigned long long cnt[4] = {0, 0, 0, 0};
+  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};  // counters [1..6]
   if (do_sync) {  // counters are only read back by synchronous calls
-    HIPCHK(c, sdeh_util_zero_u64(c->counters + 1, 4, s));
+    HIPCHK(c, sdeh_util_zero_u64(c->counters + 1, 6, s));
     HIPCHK(c, sdeh_util_count(a.naccept, a.nreject, a.retcode, n, c->counters + 1, s));
+    if (pl.packed) HIPCHK(c, sdeh_util_sum_i32(a.pk_need, n, c->counters + 6, s));
   }
   if (do_sync) HIPCHK(c, hipMemcpyAsync(cnt, c->counters + 1, sizeof cnt, hipMemcpyDeviceToHost, s));
   if (host && n > 0) {
@@ -687,8 +768,17 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
     down(io->naccept, a.naccept, sz.i32);
     down(io->nreject, a.nreject, sz.i32);
     down(io->u_saveat, a.u_saveat, sz.u_saveat);
-    down(io->u_steps, a.u_steps, sz.u_steps);
-    down(io->t_steps, a.t_steps, sz.t_steps);
+    if (pl.packed) {  // only the rows used travel back
+      int64_t used = 0;
+      HIPCHK(c, hipMemcpyAsync(&used, pl.pk_off + n, 8, hipMemcpyDeviceToHost, s));
+      HIPCHK(c, hipStreamSynchronize(s));
+      down(io->u_steps, pl.pk_out_u, (size_t)used * pl.n_state * pl.es);
+      down(io->t_steps, pl.pk_out_t, (size_t)used * 8);
+      down(io->steps_offset, pl.pk_off, sz.pk_off);
+    } else {
+      down(io->u_steps, a.u_steps, sz.u_steps);
+      down(io->t_steps, a.t_steps, sz.t_steps);
+    }
     down(io->n_steps, a.n_steps, pl.save == SAVE_STEPS ? sz.i32 : 0);
     HIPCHK(c, hipGetLastError());
   }
@@ -705,10 +795,15 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
     c->stats.kernel_ms = ms;
     c->stats.attempts = (int64_t)cnt[0];
     c->stats.accepts = (int64_t)cnt[1];
+    c->stats.steps_needed = pl.packed ? (int64_t)cnt[5] : 0;
     c->stats.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
             .count();
     if (cnt[2]) return fail(c, SDEH_E_DTMIN, "%llu trajectories stopped at dt < dtmin", cnt[2]);
+    if (cnt[3] && pl.packed)
+      return fail(c, SDEH_E_SAVE_OVERFLOW,
+                  "%llu trajectories lost points: %llu points offered, steps_capacity %lld",
+                  cnt[3], cnt[5], (long long)pl.pk_cap);
     if (cnt[3])
       return fail(c, SDEH_E_SAVE_OVERFLOW, "%llu trajectories had more than max_saves steps",
                   cnt[3]);
@@ -750,7 +845,7 @@ int sdeh_create(sdeh_ctx** out, int device) {
   if (hipSetDevice(device) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-      hipMalloc((void**)&c->counters, 8 * sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc((void**)&c->counters, 16 * sizeof(unsigned long long)) != hipSuccess) {
     sdeh_destroy(c);
     return SDEH_E_HIP;
   }
@@ -848,6 +943,8 @@ int sdeh_graph_capture(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io,
   int r = plan_solve(c, d, io, &pl);
   if (r) return r;
   if (io->order) return fail(c, SDEH_E_UNSUPPORTED, "graph capture takes no order permutation");
+  if (pl.packed)
+    return fail(c, SDEH_E_UNSUPPORTED, "graph capture of the packed save layout is not built");
   const Sizes sz = sizes_of(d, io, pl);
   const int64_t n = d->n_traj;
   r = check_dev(c, io->u0, sz.u0, "u0");

@@ -1,5 +1,6 @@
 // Small device utilities of the runtime: fills (so no hipMemsetAsync lands in
-// a captured graph), the queue reset, and the per-solve counters.
+// a captured graph), the queue reset, the per-solve counters, and the finishing
+// passes of the packed save_everystep layout (offsets scan and compaction).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -62,6 +63,122 @@ __global__ void check_order_kernel(const int64_t* order, int64_t n, int32_t* see
   }
 }
 
+// ---- packed save_everystep ------------------------------------------------
+// Counts n_steps[0..n) are scanned in tiles of PK_TILE entries (256 threads x 4):
+// pk_tile_sums -> pk_scan_tiles (one block) -> pk_offsets; pk_compact then moves
+// each stored pool row to its place in the packed output.
+constexpr int PK_TILE = 1024;
+
+// Exclusive scan over the 256 threads of a block; *total gets the block sum.
+__device__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
+  __shared__ int64_t sh[256];
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int64_t x = tid >= off ? sh[tid - off] : 0;
+    __syncthreads();
+    sh[tid] += x;
+    __syncthreads();
+  }
+  const int64_t incl = sh[tid];
+  *total = sh[255];
+  __syncthreads();  // sh is reused by the next call
+  return incl - v;
+}
+
+__global__ void __launch_bounds__(256) pk_tile_sums(const int32_t* cnt, int64_t n,
+                                                    int64_t* tile_sum) {
+  const int64_t i0 = (int64_t)blockIdx.x * PK_TILE + 4 * threadIdx.x;
+  int64_t v = 0;
+  for (int j = 0; j < 4; ++j)
+    if (i0 + j < n) v += cnt[i0 + j];
+  int64_t total = 0;
+  (void)block_exclusive_scan(v, &total);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+// One block: tile_sum[0..n_tiles) -> exclusive prefix sums, in place.
+__global__ void __launch_bounds__(256) pk_scan_tiles(int64_t* tile_sum, int64_t n_tiles) {
+  int64_t carry = 0;
+  for (int64_t base = 0; base < n_tiles; base += 256) {
+    const int64_t i = base + threadIdx.x;
+    const int64_t v = i < n_tiles ? tile_sum[i] : 0;
+    int64_t total = 0;
+    const int64_t ex = block_exclusive_scan(v, &total);
+    if (i < n_tiles) tile_sum[i] = carry + ex;
+    carry += total;
+  }
+}
+
+// off[i] = first row of trajectory i, clamped to the capacity; a trajectory whose
+// rows would pass the capacity keeps the rows that fit (cnt[i] reduced) and, if it
+// had integrated normally, gets retcode 4 (save overflow).  off[n] = rows used.
+__global__ void __launch_bounds__(256) pk_offsets(int32_t* cnt, int32_t* ret, int64_t n,
+                                                  const int64_t* tile_off, int64_t cap,
+                                                  int64_t* off) {
+  const int64_t i0 = (int64_t)blockIdx.x * PK_TILE + 4 * threadIdx.x;
+  int64_t c[4];
+  int64_t v = 0;
+  for (int j = 0; j < 4; ++j) {
+    c[j] = i0 + j < n ? cnt[i0 + j] : 0;
+    v += c[j];
+  }
+  int64_t total = 0;
+  int64_t o = tile_off[blockIdx.x] + block_exclusive_scan(v, &total);
+  for (int j = 0; j < 4; ++j) {
+    const int64_t i = i0 + j;
+    if (i >= n) break;
+    if (o + c[j] > cap) {
+      const int64_t keep = cap > o ? cap - o : 0;
+      cnt[i] = (int32_t)keep;
+      if (ret[i] == 0) ret[i] = 4;
+    }
+    off[i] = o < cap ? o : cap;
+    o += c[j];
+    if (i == n - 1) off[n] = o < cap ? o : cap;
+  }
+}
+
+// Rows [0, min(*next, n_chunks) * chunk) of the pool: row r of chunk r / chunk is
+// point pk_seq * chunk + r % chunk of trajectory pk_traj; copied when that point
+// is among the trajectory's kept ones.
+template <class T>
+__global__ void __launch_bounds__(256) pk_compact(const T* pool_u, const double* pool_t,
+                                                  const int64_t* pk_traj, const int32_t* pk_seq,
+                                                  const unsigned long long* next,
+                                                  int64_t n_chunks, int chunk, int n_state,
+                                                  const int32_t* cnt, const int64_t* off,
+                                                  T* out_u, double* out_t) {
+  const unsigned long long claimed = *next;
+  const int64_t used = (int64_t)claimed < n_chunks ? (int64_t)claimed : n_chunks;
+  const int64_t rows = used * chunk;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ch = r / chunk;
+    const int64_t traj = pk_traj[ch];
+    const int64_t k = (int64_t)pk_seq[ch] * chunk + (r - ch * chunk);
+    if (k >= cnt[traj]) continue;
+    const int64_t dst = off[traj] + k;
+    out_t[dst] = pool_t[r];
+    for (int e = 0; e < n_state; ++e) out_u[dst * n_state + e] = pool_u[r * n_state + e];
+  }
+}
+
+__global__ void __launch_bounds__(256) sum_i32_kernel(const int32_t* v, int64_t n,
+                                                      unsigned long long* out) {
+  unsigned long long s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += (unsigned long long)(uint32_t)v[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
+__global__ void write_i64(int64_t* p, int64_t v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
 inline int grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
   if (g > 4096) g = 4096;
@@ -109,5 +226,43 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_check_orde
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(fill_f32, dim3(grid_for(n)), dim3(256), 0, s, (float*)seen, 0.0f, n);
   hipLaunchKernelGGL(check_order_kernel, dim3(grid_for(n)), dim3(256), 0, s, order, n, seen, bad);
+  return hipGetLastError();
+}
+
+// Finishing passes of the packed layout (see pk_* above).  tiles: scratch of
+// ceil(n / 1024) int64.  es: element size of the state type (8 or 4).
+extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_pack(
+    int32_t* cnt, int32_t* ret, int64_t n, int64_t* tiles, int64_t cap, int64_t* off,
+    const void* pool_u, const double* pool_t, const int64_t* pk_traj, const int32_t* pk_seq,
+    const unsigned long long* next, int64_t n_chunks, int chunk, int n_state, int es,
+    void* out_u, double* out_t, hipStream_t s) {
+  if (n <= 0) {
+    hipLaunchKernelGGL(write_i64, dim3(1), dim3(64), 0, s, off, (int64_t)0);
+    return hipGetLastError();
+  }
+  const int64_t n_tiles = (n + PK_TILE - 1) / PK_TILE;
+  hipLaunchKernelGGL(pk_tile_sums, dim3((unsigned)n_tiles), dim3(256), 0, s, cnt, n, tiles);
+  hipLaunchKernelGGL(pk_scan_tiles, dim3(1), dim3(256), 0, s, tiles, n_tiles);
+  hipLaunchKernelGGL(pk_offsets, dim3((unsigned)n_tiles), dim3(256), 0, s, cnt, ret, n, tiles,
+                     cap, off);
+  const int g = 2048;  // grid-stride: the used row count is only known on the device
+  if (es == 8)
+    hipLaunchKernelGGL(pk_compact<double>, dim3(g), dim3(256), 0, s, (const double*)pool_u,
+                       pool_t, pk_traj, pk_seq, next, n_chunks, chunk, n_state, cnt, off,
+                       (double*)out_u, out_t);
+  else
+    hipLaunchKernelGGL(pk_compact<float>, dim3(g), dim3(256), 0, s, (const float*)pool_u, pool_t,
+                       pk_traj, pk_seq, next, n_chunks, chunk, n_state, cnt, off,
+                       (float*)out_u, out_t);
+  return hipGetLastError();
+}
+
+// *out += sum(v[0..n)) (out must be cleared by the caller).
+extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_sum_i32(
+    const int32_t* v, int64_t n, unsigned long long* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int g = grid_for(n);
+  if (g > 512) g = 512;
+  hipLaunchKernelGGL(sum_i32_kernel, dim3(g), dim3(256), 0, s, v, n, out);
   return hipGetLastError();
 }

@@ -6,7 +6,7 @@ Only plain ctypes: the structs below must match the C header field for field.
 import ctypes
 import os
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # sdeh_status
 OK = 0
@@ -43,7 +43,7 @@ BUILTIN_RHS = {
 }
 
 # sdeh_schedule / precision / memory
-SAVE_POINT_MAJOR, SAVE_TRAJ_MAJOR = 0, 1
+SAVE_POINT_MAJOR, SAVE_TRAJ_MAJOR, SAVE_PACKED = 0, 1, 2
 SCHED_AUTO, SCHED_ONE_LAUNCH, SCHED_PERSISTENT = 0, 1, 2
 PREC_PLAIN, PREC_KAHAN = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
@@ -98,6 +98,8 @@ class SolveIO(ctypes.Structure):
         ("u_steps", ctypes.c_void_p),
         ("t_steps", ctypes.c_void_p),
         ("n_steps", ctypes.c_void_p),
+        ("steps_offset", ctypes.c_void_p),
+        ("steps_capacity", ctypes.c_int64),
     ]
 
 
@@ -112,6 +114,8 @@ class Stats(ctypes.Structure):
         ("chunk", ctypes.c_int32),
         ("schedule", ctypes.c_int32),
         ("lanes", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
+        ("steps_needed", ctypes.c_int64),
     ]
 
 

@@ -18,7 +18,9 @@ is no CPU fallback.  Right-hand sides: the library's built-ins (rhs.lorenz,
 rhs.glv16(A), ...) use kernels compiled ahead of time; any other Python
 function f(u, p, t) is traced into HIP C (tracer.py) and compiled with hipRTC.
 """
+import copy
 import inspect
+import time
 
 import numpy as np
 
@@ -163,10 +165,15 @@ class EnsembleHIP:
 
 
 class EnsembleSolution:
-    def __init__(self, u, converged, raw):
+    """u (the reduced outputs), converged, elapsedTime (seconds) as SciMLBase's;
+    raw: the EnsembleBuffers of the solve (a list of them, one per batch, when
+    batch_size split the ensemble), arrays in structure-of-arrays layout."""
+
+    def __init__(self, u, converged, raw, elapsedTime=0.0):
         self.u = u
         self.converged = converged
-        self.raw = raw  # the EnsembleBuffers (arrays in structure-of-arrays layout)
+        self.raw = raw
+        self.elapsedTime = elapsedTime
 
     def __len__(self):
         return len(self.u)
@@ -244,84 +251,178 @@ def _f32(x):
 # ---------------------------------------------------------------------------
 # solve
 # ---------------------------------------------------------------------------
-def solve(prob, alg, ensemblealg=None, *, trajectories=None, dt=None, saveat=None,
-          save_everystep=True, abstol=_f32(1e-6), reltol=_f32(1e-3), max_saves=None,
-          dtype=None, maxiters=0, quirks=True, dt0s=None, order=None, **kwargs):
+def solve(prob, alg, ensemblealg=None, *, trajectories=None, batch_size=None, dt=None,
+          saveat=None, save_everystep=True, abstol=_f32(1e-6), reltol=_f32(1e-3),
+          max_saves=None, dtype=None, maxiters=0, quirks=True, dt0s=None, order=None,
+          **kwargs):
     """solve(ODEProblem, alg; ...) -> ODESolution
-    solve(EnsembleProblem, alg, EnsembleHIP(); trajectories, ...) -> EnsembleSolution
+    solve(EnsembleProblem, alg, EnsembleHIP(); trajectories, batch_size, ...)
+        -> EnsembleSolution
 
     Unknown keywords are accepted and ignored, as the reference's `kwargs...`.
     dt defaults to 0.1f0, except for GPUSimpleRK4 and GPUSimpleEuler, which
-    raise "dt is required for this algorithm" without it (gpurk4.jl:56)."""
+    raise "dt is required for this algorithm" without it (gpurk4.jl:56).
+
+    Ensembles follow SciMLBase's loop: trajectories run in batches of
+    batch_size (default: all), one GPU solve per batch; each solution goes
+    through output_func(sol, i) -> (out, rerun), and trajectories asking for a
+    rerun are solved again from prob_func(prob, i, repeat + 1) (together, in one
+    GPU solve per round) until none asks; then reduction(u, batch_outs, I) ->
+    (u, converged) folds the batch in, and converged stops the loop early."""
     if not isinstance(alg, _Alg):
         raise TypeError("alg must be one of the GPUSimple* algorithms")
     if dt is None:
         if alg.dt_required:
             raise ValueError("dt is required for this algorithm")
         dt = _f32(0.1)
-    single = isinstance(prob, ODEProblem)
-    if single:
-        eprob = EnsembleProblem(prob)
-        trajectories = 1
-        ens = ensemblealg or EnsembleHIP()
-    else:
-        eprob = prob
-        ens = ensemblealg or EnsembleHIP()
-        if trajectories is None:
-            raise TypeError("trajectories is required for an EnsembleProblem")
-    base = eprob.prob
-    assert not base.isinplace(), "the GPUSimple solvers are out-of-place only"
+    kw = dict(dt=dt, saveat=saveat, save_everystep=save_everystep, abstol=abstol,
+              reltol=reltol, max_saves=max_saves, dtype=dtype, maxiters=maxiters,
+              quirks=quirks)
+    if isinstance(prob, ODEProblem):
+        sols, _ = _solve_many(EnsembleProblem(prob), alg, ensemblealg or EnsembleHIP(), [0],
+                              [1], dt0s, order, kw)
+        return sols[0]
+    eprob = prob
+    ens = ensemblealg or EnsembleHIP()
+    if trajectories is None:
+        raise TypeError("trajectories is required for an EnsembleProblem")
     n = int(trajectories)
+    bs = n if batch_size is None else int(batch_size)
+    if bs < 1:
+        raise ValueError("batch_size must be >= 1")
+    t_start = time.perf_counter()
+    u = [] if eprob.u_init is None else eprob.u_init
+    converged = False
+    raws = []
+    for start in range(0, n, bs):
+        idx = list(range(start, min(start + bs, n)))
+        batch, raw = _solve_batch(eprob, alg, ens, idx, dt0s, order, kw)
+        raws.append(raw)
+        I = range(idx[0] + 1, idx[-1] + 2)  # 1-based, as Julia's
+        if eprob.reduction is None:
+            u = list(u) + batch  # SciMLBase's default: append!(u, data), false
+            converged = False
+        else:
+            u, converged = eprob.reduction(u, batch, I)
+        if converged:
+            break
+    return EnsembleSolution(u, converged, raws[0] if len(raws) == 1 else raws,
+                            time.perf_counter() - t_start)
 
-    # Materialise the ensemble (prob_func is called with 1-based i, repeat = 1).
+
+def _solve_batch(eprob, alg, ens, idx, dt0s, order, kw):
+    """One batch: solve, output_func, and the reruns it asks for."""
+    repeats = [1] * len(idx)
+    sols, raw = _solve_many(eprob, alg, ens, idx, repeats, dt0s, order, kw)
+    outs = [None] * len(idx)
+    pending = list(range(len(idx)))
+    while pending:
+        again = []
+        for j, s in zip(pending, sols):
+            if eprob.output_func is None:
+                outs[j] = s
+                continue
+            x = eprob.output_func(s, idx[j] + 1)
+            if not isinstance(x, tuple) or len(x) != 2:
+                raise TypeError("output_func must return (output, rerun)")
+            outs[j] = x[0]
+            if x[1]:
+                again.append(j)
+        pending = again
+        if pending:
+            for j in pending:
+                repeats[j] += 1
+            sols, _ = _solve_many(eprob, alg, ens, [idx[j] for j in pending],
+                                  [repeats[j] for j in pending], dt0s, order, kw)
+    return outs, raw
+
+
+def _materialise(eprob, idx, repeats):
+    """(u0s, ps, tspans) of trajectories idx (0-based) from prob_func(prob, i, repeat)
+    (i 1-based; the prototype deep-copied first when safetycopy is set), or from
+    the array form of the problem."""
+    base = eprob.prob
+    n = len(idx)
     if eprob.u0s is not None:
-        u0s = np.asarray(eprob.u0s)
-        ps = None if eprob.ps is None else np.asarray(eprob.ps)
-        tspans = None if eprob.tspans is None else np.asarray(eprob.tspans, dtype=np.float64)
-    elif eprob.prob_func is None:
+        cols = np.asarray(idx)
+        u0s = np.asarray(eprob.u0s)[:, cols]
+        ps = None if eprob.ps is None else np.asarray(eprob.ps)[:, cols]
+        tspans = (None if eprob.tspans is None
+                  else np.asarray(eprob.tspans, dtype=np.float64)[:, cols])
+        return u0s, ps, tspans
+    if eprob.prob_func is None:
         u0s = np.repeat(base.u0.reshape(-1, 1), n, axis=1)
         ps = None if base.p is None else np.repeat(base.p.reshape(-1, 1), n, axis=1)
-        tspans = None
-    else:
-        probs = [eprob.prob_func(base, i + 1, 1) for i in range(n)]
-        u0s = np.stack([np.asarray(q.u0).reshape(-1) for q in probs], axis=1)
-        ps = None if base.p is None else np.stack([np.asarray(q.p).reshape(-1) for q in probs],
-                                                  axis=1)
-        spans = np.array([q.tspan for q in probs], dtype=np.float64).T
-        tspans = None if (spans == np.array(base.tspan).reshape(2, 1)).all() else spans
+        return u0s, ps, None
+    probs = [eprob.prob_func(copy.deepcopy(base) if eprob.safetycopy else base, i + 1, r)
+             for i, r in zip(idx, repeats)]
+    u0s = np.stack([np.asarray(q.u0).reshape(-1) for q in probs], axis=1)
+    ps = None if base.p is None else np.stack([np.asarray(q.p).reshape(-1) for q in probs],
+                                              axis=1)
+    spans = np.array([q.tspan for q in probs], dtype=np.float64).T
+    tspans = None if (spans == np.array(base.tspan).reshape(2, 1)).all() else spans
+    return u0s, ps, tspans
+
+
+def _solve_many(eprob, alg, ens, idx, repeats, dt0s, order, kw):
+    """One GPU solve of trajectories idx; returns their ODESolutions and buffers."""
+    base = eprob.prob
+    assert not base.isinplace(), "the GPUSimple solvers are out-of-place only"
+    n = len(idx)
+    u0s, ps, tspans = _materialise(eprob, idx, repeats)
     n_state = u0s.shape[0]
     n_param = 0 if ps is None else ps.shape[0]
+    dtype = kw["dtype"]
     if dtype is None:
         dtype = _abi.F32 if u0s.dtype == np.float32 else _abi.F64
     elif dtype in ("f32", np.float32):
         dtype = _abi.F32
     elif dtype in ("f64", np.float64):
         dtype = _abi.F64
+    cols = np.asarray(idx)
+    if dt0s is not None:
+        dt0s = np.asarray(dt0s)[cols]
+    if order is not None:
+        # a caller's trajectory order applies to a solve of the whole ensemble
+        order = np.asarray(order) if n == len(order) else None
 
     ctx = _context(ens.device)
     rid, shared = _rhs_handle(ctx, base.f, n_state, n_param, base.scalar_u,
                               base.scalar_p and n_param == 1)
     if ps is None:
         ps = np.zeros((0, n))
-    everystep = saveat is None and bool(save_everystep)
+    dt, saveat, max_saves = kw["dt"], kw["saveat"], kw["max_saves"]
+    everystep = saveat is None and bool(kw["save_everystep"])
+    # save_everystep rows use the packed layout: no per-trajectory cap, as the
+    # reference's push! (gpuatsit5.jl:113-114, 302-303).  The first capacity is a
+    # guess (max_saves rows per trajectory if given, else 64); when it is short,
+    # the library reports the capacity that fits and the batch is solved again.
+    rows_each = max_saves if max_saves else 64
     if alg.code in _abi.FIXED_GRID_ONLY_ALGS:
-        # gpurk4.jl:66-88 / gpueuler.jl: the solution is (ts, us) over the whole
-        # grid; save_everystep and saveat arrive in kwargs... and are ignored.
+        # gpurk4.jl:66-88 / gpueuler.jl:66-82: the solution is (ts, us) over the
+        # whole grid; save_everystep and saveat arrive in kwargs... and are ignored.
         saveat, everystep = None, True
-        max_saves = max(int(runtime.fixed_grid(base.tspan[0], float(dt), base.tspan[1])[0]), 1)
-    if everystep and max_saves is None:
-        max_saves = 4096
+        rows_each = max(int(runtime.fixed_grid(base.tspan[0], float(dt), base.tspan[1])[0]), 1)
     saveat_arr = None
     if saveat is not None:
         saveat_arr = np.atleast_1d(np.asarray(saveat, dtype=np.float64))
     d = runtime.make_desc(alg.code, dtype=dtype, rhs=rid, n_traj=n, tspan=base.tspan, dt=dt,
-                          abstol=abstol, reltol=reltol, saveat=saveat_arr,
-                          save_everystep=everystep, max_saves=max_saves or 0, maxiters=maxiters,
-                          quirks=quirks, schedule=ens.schedule, precision=ens.precision,
-                          save_layout=_abi.SAVE_TRAJ_MAJOR)  # each solution contiguous
-    bufs = runtime.EnsembleBuffers(d, n_state, u0s, ps if n_param else None, shared, tspans,
-                                   dt0s, order)
-    st = ctx.solve(d, bufs, allow=(_abi.E_DTMIN,))
+                          abstol=kw["abstol"], reltol=kw["reltol"], saveat=saveat_arr,
+                          save_everystep=everystep, maxiters=kw["maxiters"], quirks=kw["quirks"],
+                          schedule=ens.schedule, precision=ens.precision,
+                          save_layout=_abi.SAVE_PACKED if everystep else _abi.SAVE_TRAJ_MAJOR)
+    cap = n * int(rows_each) if everystep else 0
+    allow = (_abi.E_DTMIN, _abi.E_SAVE_OVERFLOW)
+    while True:
+        bufs = runtime.EnsembleBuffers(d, n_state, u0s, ps if n_param else None, shared, tspans,
+                                       dt0s, order, steps_capacity=cap)
+        st = ctx.solve(d, bufs, allow=allow)
+        if st != _abi.E_SAVE_OVERFLOW:
+            break
+        need = int(ctx.stats()["steps_needed"])
+        if need <= cap:
+            raise runtime.SolveError(st, ctx.last_error())
+        cap = need
     if st == _abi.E_DTMIN:
         raise DtLessThanMin("dt<dtmin")
 
@@ -333,21 +434,12 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, dt=None, saveat=Non
             t = saveat_arr
             u = list(bufs.u_saveat[i])
         elif everystep:
-            k = int(bufs.n_steps[i])
-            t = bufs.t_steps[i, :k]
-            u = list(bufs.u_steps[i, :k])
+            t, us = bufs.steps_of(i)
+            u = list(us)
         else:
             t = np.array([t0s[i], bufs.t_end[i]])
             u = [u0s[:, i].astype(bufs.u_end.dtype), bufs.u_end[:, i]]
         if base.scalar_u:
             u = [x[0] for x in u]
         sols.append(ODESolution(t, u, rc, bufs.naccept[i], bufs.nreject[i], alg))
-    if single:
-        return sols[0]
-    outs = sols
-    if eprob.output_func is not None:
-        outs = [eprob.output_func(s, i + 1)[0] for i, s in enumerate(sols)]
-    converged = True
-    if eprob.reduction is not None:
-        outs, converged = eprob.reduction(eprob.u_init, outs, range(1, n + 1))
-    return EnsembleSolution(outs, converged, bufs)
+    return sols, bufs

@@ -28,8 +28,9 @@ def make_desc(alg, dtype=_abi.F64, rhs=_abi.RHS_LORENZ, n_traj=0, tspan=(0.0, 1.
     """Build an sdeh_solve_desc.  Unset dt/abstol/reltol take the reference
     defaults (0.1f0, 1f-6, 1f-3).  lanes: lanes per trajectory (0 = the
     right-hand side's group layout where it has one).  save_layout: the layout of
-    u_saveat/u_steps/t_steps (point- or trajectory-major).  Keeps the saveat array
-    alive on the struct."""
+    u_saveat/u_steps/t_steps (point- or trajectory-major, or packed: save_everystep
+    rows of all trajectories back to back, no per-trajectory cap).  Keeps the
+    saveat array alive on the struct."""
     d = _abi.default_desc()
     d.alg = int(alg)
     d.dtype = int(dtype)
@@ -64,10 +65,14 @@ class EnsembleBuffers:
 
     u0: (n_state, n_traj); p: (n_param, n_traj); p_shared: (n_shared,);
     tspan: (2, n_traj) or None; dt0: (n_traj,) or None; order: (n_traj,) int64 or None.
-    Backend "numpy" (host) or "torch" (device tensors on `device`)."""
+    Backend "numpy" (host) or "torch" (device tensors on `device`).
+    steps_capacity: rows of the packed save_everystep output (packed layout only):
+    u_steps (steps_capacity, n_state), t_steps (steps_capacity,), and
+    steps_offset (n_traj + 1,) int64; trajectory i owns rows
+    steps_offset[i] .. steps_offset[i + 1] - 1 (see steps_of)."""
 
     def __init__(self, desc, n_state, u0, p=None, p_shared=None, tspan=None, dt0=None,
-                 order=None, backend="numpy", device=None):
+                 order=None, backend="numpy", device=None, steps_capacity=0):
         self.desc = desc
         self.backend = backend
         n = int(desc.n_traj)
@@ -112,8 +117,15 @@ class EnsembleBuffers:
         if ns > 0:
             self.u_saveat = empty((n, ns, n_state) if tm else (ns, n_state, n), dt_np)
         everystep = ns == 0 and desc.save_everystep
-        self.u_steps = self.t_steps = None
-        if everystep and ms > 0:
+        self.packed = int(desc.save_layout) == _abi.SAVE_PACKED
+        self.u_steps = self.t_steps = self.steps_offset = None
+        self.steps_capacity = int(steps_capacity)
+        if everystep and self.packed:
+            cap = self.steps_capacity
+            self.u_steps = empty((cap, n_state), dt_np)
+            self.t_steps = empty((cap,), np.float64)
+            self.steps_offset = empty((n + 1,), np.int64)
+        elif everystep and ms > 0:
             self.u_steps = empty((n, ms, n_state) if tm else (ms, n_state, n), dt_np)
             self.t_steps = empty((n, ms) if tm else (ms, n), np.float64)
         self.n_steps = empty((n,), np.int32) if everystep else None
@@ -134,7 +146,18 @@ class EnsembleBuffers:
         io.u_steps = ptr(self.u_steps)
         io.t_steps = ptr(self.t_steps)
         io.n_steps = ptr(self.n_steps)
+        io.steps_offset = ptr(self.steps_offset)
+        io.steps_capacity = self.steps_capacity if self.packed else 0
         self.io = io
+
+    def steps_of(self, i):
+        """(t, u) of trajectory i's save_everystep rows in the packed layout, as
+        numpy arrays (t: (k,), u: (k, n_state))."""
+        off = self.numpy("steps_offset")
+        a, b = int(off[i]), int(off[i + 1])
+        if self.backend == "numpy":
+            return self.t_steps[a:b], self.u_steps[a:b]
+        return self.t_steps[a:b].cpu().numpy(), self.u_steps[a:b].cpu().numpy()
 
     def numpy(self, name):
         a = getattr(self, name)

@@ -118,6 +118,41 @@ def test_ensemble_problem_prob_func_output_func_reduction():
     assert [s.t[-1] for s in esol.u] == [0.5, 1.0, 1.5, 2.0]
 
 
+def test_ensemble_batches_and_reruns_match_one_solve():
+    base = sd.ODEProblem(sd.rhs.lorenz, [1.0, 0.0, 0.0], (0.0, 1.0), [10.0, 0.0, 8 / 3])
+    rhos = np.linspace(0.0, 21.0, 100)
+
+    def prob_func(prob, i, repeat):
+        # a rerun solves the same trajectory with rho halved
+        return sd.remake(prob, p=[10.0, rhos[i - 1] / 2 ** (repeat - 1), 8 / 3])
+
+    kw = dict(trajectories=100, dt=0.1, abstol=1e-6, reltol=1e-3, save_everystep=False)
+    one = sd.solve(sd.EnsembleProblem(base, prob_func=prob_func), sd.GPUSimpleATsit5(),
+                   sd.EnsembleHIP(), **kw)
+    many = sd.solve(sd.EnsembleProblem(base, prob_func=prob_func), sd.GPUSimpleATsit5(),
+                    sd.EnsembleHIP(), batch_size=7, **kw)
+    assert len(many.raw) == 15
+    assert bits_equal(np.array([s.u[-1] for s in one.u]), np.array([s.u[-1] for s in many.u]))
+    # output_func asks for reruns while z(1) > 5: those end with the rho that
+    # brought z below 5, exactly as a direct solve with that rho
+    ep = sd.EnsembleProblem(base, prob_func=prob_func,
+                            output_func=lambda sol, i: (sol, bool(sol.u[-1][2] > 5.0)))
+    res = sd.solve(ep, sd.GPUSimpleATsit5(), sd.EnsembleHIP(), batch_size=32, **kw)
+    reran = 0
+    for i, s in enumerate(res.u):
+        assert s.u[-1][2] <= 5.0
+        rho = rhos[i]
+        while True:
+            d = sd.solve(sd.remake(base, p=[10.0, rho, 8 / 3]), sd.GPUSimpleATsit5(), dt=0.1,
+                         abstol=1e-6, reltol=1e-3, save_everystep=False)
+            if d.u[-1][2] <= 5.0:
+                break
+            rho /= 2
+            reran += 1
+        assert bits_equal(s.u[-1], d.u[-1])
+    assert reran > 0
+
+
 def test_dtmin_raises_like_the_reference():
     prob = sd.ODEProblem(sd.rhs.expdecay3, np.ones(3), (0.0, 1.0), [-1.0])
     with pytest.raises(sd.DtLessThanMin, match="dt<dtmin"):
