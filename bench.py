@@ -47,8 +47,8 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {("f64", 1 << 20): 125.23e6}
 
 # Algorithmic FLOPs (per component per attempt, RHS calls per attempt) by solver.
-STAGE_FLOPS = {0: 48, 1: 67, 2: 83, 3: 117, 4: 167, 5: 211}
-RHS_PER_ATTEMPT = {0: 6, 1: 6, 2: 9, 3: 9, 4: 15, 5: 15}
+STAGE_FLOPS = {0: 48, 1: 67, 2: 83, 3: 117, 4: 167, 5: 211, 6: 13, 7: 2}
+RHS_PER_ATTEMPT = {0: 6, 1: 6, 2: 9, 3: 9, 4: 15, 5: 15, 6: 4, 7: 1}
 LORENZ_FLOPS = 8
 
 
@@ -207,6 +207,59 @@ def workload_glv(args):
     return out
 
 
+def workload_everystep(args):
+    """The headline ensemble (Lorenz, 2^20 trajectories, ATsit5 1e-6/1e-3, tspan
+    (0,10)) with every accepted step saved: the packed layout (one pass into the
+    chunk pool, then the offsets scan and compaction; capacity = the points
+    offered) against fixed rows per trajectory (trajectory-major, rows = the
+    largest step count) and endpoint-only output.  Times cover the whole solve on
+    the stream (all kernels)."""
+    import numpy as np
+    import torch
+    from simplediffeq_hip import _abi, inputs, runtime
+    n = args.n
+    u0, p = inputs.lorenz_wide(n, seed=1)
+    ctx = runtime.Context(0)
+    stream = torch.cuda.Stream()
+    kw = dict(rhs=_abi.RHS_LORENZ, n_traj=n, tspan=(0.0, 10.0), dt=float(np.float32(0.1)),
+              abstol=1e-6, reltol=1e-3)
+    out = {"workload": "lorenz-wide-everystep", "trajectories": n}
+    reps = max(1, args.steps // 4)
+    d = runtime.make_desc(_abi.ALG_ATSIT5, **kw)
+    g = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
+    ms, _ = _time_solves(ctx, d, g, stream, reps, 1)
+    out["endpoint"] = {"ms": ms}
+    del g
+    dp = runtime.make_desc(_abi.ALG_ATSIT5, save_everystep=True, save_layout=_abi.SAVE_PACKED,
+                           **kw)
+    probe = runtime.EnsembleBuffers(dp, 3, u0, p, backend="torch", device="cuda:0",
+                                    steps_capacity=0)
+    ctx.solve(dp, probe, sync=True, allow=(_abi.E_SAVE_OVERFLOW,))
+    rows = int(ctx.stats()["steps_needed"])
+    del probe
+    g = runtime.EnsembleBuffers(dp, 3, u0, p, backend="torch", device="cuda:0",
+                                steps_capacity=rows)
+    ctx.solve(dp, g, sync=True)
+    kernel_ms = ctx.stats()["kernel_ms"]
+    counts = g.numpy("n_steps")
+    assert int(counts.sum()) == rows
+    ms, _ = _time_solves(ctx, dp, g, stream, reps, 1)
+    out_bytes = rows * (3 * 8 + 8)
+    out["packed"] = {"ms": ms, "integration_kernel_ms": kernel_ms, "rows": rows,
+                     "rows_max": int(counts.max()), "rows_mean": rows / n,
+                     "output_bytes": out_bytes, "output_GBs": out_bytes / (ms * 1e-3) / 1e9}
+    del g
+    ms_rows = int(counts.max())
+    dt_ = runtime.make_desc(_abi.ALG_ATSIT5, save_everystep=True, max_saves=ms_rows,
+                            save_layout=_abi.SAVE_TRAJ_MAJOR, **kw)
+    g = runtime.EnsembleBuffers(dt_, 3, u0, p, backend="torch", device="cuda:0")
+    ms, _ = _time_solves(ctx, dt_, g, stream, reps, 1)
+    out["fixed_rows_traj_major"] = {"ms": ms, "rows_per_trajectory": ms_rows,
+                                    "output_bytes": n * ms_rows * 32}
+    del g
+    return out
+
+
 def workload_costmodel(args):
     """Learned cost binning + dt0.  Train the 6-64-64-2 MLP on the GPU from the
     accepted-step counts of 2^20 plain binary64 ATsit5 solves (Lorenz, sigma~U(8,12),
@@ -326,7 +379,8 @@ def main():
                     help="persistent schedule: workgroups of 256 (0 = library default, "
                          "CUs x occupancy)")
     ap.add_argument("--workload", default="lorenz-wide",
-                    choices=["lorenz-wide", "lorenz-small", "lorenz-uneven", "glv16", "costmodel"],
+                    choices=["lorenz-wide", "lorenz-small", "lorenz-uneven", "glv16", "costmodel",
+                             "everystep"],
                     help="lorenz-wide is the headline line; the others report their own JSON")
     ap.add_argument("--train-steps", type=int, default=1500)
     args = ap.parse_args()
@@ -337,7 +391,8 @@ def main():
         return 2
 
     extra = {"lorenz-small": workload_small, "lorenz-uneven": workload_uneven,
-             "glv16": workload_glv, "costmodel": workload_costmodel}
+             "glv16": workload_glv, "costmodel": workload_costmodel,
+             "everystep": workload_everystep}
     if args.workload in extra:
         print(json.dumps(extra[args.workload](args)))
         return 0

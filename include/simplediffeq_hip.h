@@ -104,12 +104,15 @@ typedef enum sdeh_save_layout {
                                 [steps_capacity][n_state] and t_steps [steps_capacity];
                                 n_steps[i] = their count.  The reference's per-trajectory
                                 `us`/`ts` vectors grow with push! (gpuatsit5.jl:113-114,
-                                302-303); here one pass saves into chunks claimed from a device
-                                pool and a compaction lays them out contiguously.  When
-                                the points do not fit steps_capacity, trajectories keep a
-                                prefix of their points (retcode SAVE_OVERFLOW; which ones
-                                lose points depends on the order chunks were claimed)
-                                and stats.steps_needed gives the capacity that fits. */
+                                302-303); here the adaptive solvers first run an
+                                endpoint pass that counts each trajectory's steps, a
+                                scan turns the counts into offsets, and the saving pass
+                                writes every row in place (fixed-step solvers know
+                                their counts and skip the first pass).  When the rows
+                                do not fit steps_capacity, trajectories are laid out in
+                                index order until it is full: the later ones keep a
+                                prefix of their rows (retcode SAVE_OVERFLOW), and
+                                stats.steps_needed gives the capacity that fits. */
 } sdeh_save_layout;
 
 /* Scheduling of trajectories onto lanes. */
@@ -193,7 +196,8 @@ typedef struct sdeh_io {
 
 /* Timing and work counters of the last sdeh_solve() on a context. */
 typedef struct sdeh_stats {
-  double kernel_ms;        /* device time of the integration kernel(s) (HIP events) */
+  double kernel_ms;        /* device time of the integration kernel(s) (HIP events; the
+                              packed layout: both passes and the offsets scan) */
   double total_ms;         /* host wall time of the call, staging copies included */
   int64_t attempts;        /* sum over trajectories of naccept + nreject */
   int64_t accepts;

@@ -603,20 +603,26 @@ __device__ __forceinline__ void store_save_time(const sdeh_kargs& a, int64_t row
 }
 
 // save_everystep output of one trajectory.  Fixed rows: point k goes to row k of
-// the trajectory's max_saves rows.  Packed (a.pk_chunk > 0): points fill chunks of
-// the pool claimed one at a time (one device atomic per chunk, by the group
-// leader); once the pool is exhausted the trajectory stores no more points.
+// the trajectory's max_saves rows.  Packed (a.packed): point k goes to row
+// base + k of the shared output while k < lim, where base = steps_off[traj] and lim
+// = n_steps[traj] come from the counting pass and the offsets scan.
 struct StepSink {
-  int64_t chunk;  // current chunk; -1 before the first claim, -2 once the pool ran out
-  int32_t fill;   // rows used in the current chunk
-  int32_t seq;    // chunks claimed
-  int32_t n;      // points offered
-  int32_t kept;   // points stored
+  int64_t base;  // packed: first row of the trajectory
+  int32_t lim;   // rows available (packed) or max_saves
+  int32_t n;     // points offered
+  int32_t kept;  // points stored
 };
 
-__device__ __forceinline__ void sink_reset(StepSink& s) {
-  s.chunk = -1;
-  s.fill = s.seq = s.n = s.kept = 0;
+template <class R>
+__device__ __forceinline__ void sink_start(const sdeh_kargs& a, StepSink& s, int64_t traj) {
+  s.n = s.kept = 0;
+  if (a.packed) {
+    s.base = a.steps_off[traj];
+    s.lim = a.n_steps[traj];
+  } else {
+    s.base = 0;
+    s.lim = a.max_saves;
+  }
 }
 
 template <class T, class R>
@@ -624,47 +630,25 @@ __device__ __forceinline__ void save_step(const sdeh_kargs& a, StepSink& s, int6
                                           double t, const T (&u)[R::N]) {
   const bool lead = lane_comp<R>() == 0;
   ++s.n;
-  if (a.pk_chunk == 0) {
-    if (s.kept < a.max_saves) {
-      store_save<T, R>(a, a.u_steps, s.kept, a.max_saves, traj, u);
-      if (lead) store_save_time(a, s.kept, traj, t);
-      ++s.kept;
-    }
-    return;
-  }
-  if (s.chunk == -2) return;
-  if (s.chunk < 0 || s.fill == a.pk_chunk) {
-    long long c = 0;
-    if (lead) c = (long long)atomicAdd(a.pk_next, 1ull);
-    if constexpr (R::G > 1)  // the group follows its leader's claim
-      c = __shfl(c, (int)(__lane_id() & ~(unsigned)(R::G - 1)));
-    if (c >= a.pk_chunks) {
-      s.chunk = -2;
-      return;
-    }
-    s.chunk = c;
-    s.fill = 0;
-    if (lead) {
-      a.pk_traj[c] = traj;
-      a.pk_seq[c] = s.seq;
-    }
-    ++s.seq;
-  }
-  constexpr int FULL = R::N * R::G;
-  const int64_t row = s.chunk * a.pk_chunk + s.fill;
-  T* o = (T*)a.u_steps + row * FULL + lane_comp<R>() * R::N;
+  if (s.kept >= s.lim) return;
+  if (a.packed) {
+    constexpr int FULL = R::N * R::G;
+    const int64_t row = s.base + s.kept;
+    T* o = (T*)a.u_steps + row * FULL + lane_comp<R>() * R::N;
 #pragma unroll
-  for (int c = 0; c < R::N; ++c) o[c] = u[c];
-  if (lead) a.t_steps[row] = t;
-  ++s.fill;
+    for (int c = 0; c < R::N; ++c) o[c] = u[c];
+    if (lead) a.t_steps[row] = t;
+  } else {
+    store_save<T, R>(a, a.u_steps, s.kept, a.max_saves, traj, u);
+    if (lead) store_save_time(a, s.kept, traj, t);
+  }
   ++s.kept;
 }
 
-// Leader's end-of-trajectory writes of the save_everystep counts.
+// Leader's end-of-trajectory write of the save_everystep count.
 __device__ __forceinline__ void finish_steps(const sdeh_kargs& a, const StepSink& s,
                                              int64_t traj) {
   a.n_steps[traj] = s.kept;
-  if (a.pk_chunk) a.pk_need[traj] = s.n;
 }
 
 template <class T, class R>
@@ -765,8 +749,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
   double t = 0.0, tf = 0.0;
   T dt = T(0), qold = T(0), qold_pow = T(0);
   int32_t nacc = 0, nrej = 0, cur = 0;
-  StepSink sk;
-  sink_reset(sk);
+  StepSink sk{0, 0, 0, 0};
   double ts_next = 0.0;  // save_time(cur)
   int64_t attempts = 0, traj = -1;
   bool active = false;
@@ -819,7 +802,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
         ts_next = save_time(a, cur);
       }
       if constexpr (SAVE == SAVE_STEPS) {
-        sink_reset(sk);
+        sink_start<R>(a, sk, traj);
         save_step<T, R>(a, sk, traj, t, uprev);
       }
       if constexpr (ALG == ALG_ATSIT5) R::f(fsal, uprev, pl, ps, t);  // FSAL seed, :241
@@ -992,8 +975,8 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
   const T dt = (T)a.dt;
   double t = a.t0;
   int32_t cur = 0;
-  StepSink sk;
-  sink_reset(sk);
+  StepSink sk{0, 0, 0, 0};
+  if constexpr (SAVE == SAVE_STEPS) sink_start<R>(a, sk, traj);
   double ts_next = 0.0;  // save_time(cur)
   if constexpr (SAVE == SAVE_AT) {
     if (a.n_saveat > 0 && t == a.saveat[0]) {

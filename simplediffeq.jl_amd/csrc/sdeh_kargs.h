@@ -45,16 +45,10 @@ struct sdeh_kargs {
   int32_t persistent;      // 1: lanes refill from *queue in chunks of `chunk`
   int32_t chunk;
   int32_t traj_major;      // saved-state outputs [n][rows][N] instead of [rows][N][n]
-  // Packed save_everystep (pk_chunk > 0): u_steps/t_steps are a pool of pk_chunks
-  // chunks of pk_chunk rows ([row][N] states, [row] times).  A trajectory claims its
-  // chunks one at a time from *pk_next; chunk c holds rows pk_seq[c]*pk_chunk ...
-  // of trajectory pk_traj[c].  pk_need[traj] receives the number of points the
-  // trajectory offered (stored or not).
-  int32_t pk_chunk;
+  // Packed save_everystep (packed != 0): point k of trajectory i goes to row
+  // steps_off[i] + k of u_steps ([row][N]) / t_steps ([row]) while k < n_steps[i],
+  // the row count a first pass and the offsets scan left there.
+  int32_t packed;
   int32_t pk_pad_;
-  int64_t pk_chunks;
-  unsigned long long* pk_next;
-  int64_t* pk_traj;
-  int32_t* pk_seq;
-  int32_t* pk_need;
+  const int64_t* steps_off;
 };

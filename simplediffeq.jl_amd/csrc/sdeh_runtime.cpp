@@ -43,12 +43,9 @@ SDEH_HIDDEN hipError_t sdeh_util_count(const int32_t*, const int32_t*, const int
                                        unsigned long long*, hipStream_t);
 SDEH_HIDDEN hipError_t sdeh_util_check_order(const int64_t*, int64_t, int32_t*, unsigned long long*,
                                              hipStream_t);
-SDEH_HIDDEN hipError_t sdeh_util_pack(int32_t*, int32_t*, int64_t, int64_t*, int64_t, int64_t*,
-                                      const void*, const double*, const int64_t*, const int32_t*,
-                                      const unsigned long long*, int64_t, int, int, int, void*,
-                                      double*, hipStream_t);
-SDEH_HIDDEN hipError_t sdeh_util_sum_i32(const int32_t*, int64_t, unsigned long long*,
-                                         hipStream_t);
+SDEH_HIDDEN hipError_t sdeh_util_pack_offsets(const int32_t*, int32_t, int64_t, int64_t*, int64_t,
+                                              int64_t*, int32_t*, unsigned long long*,
+                                              hipStream_t);
 SDEH_HIDDEN int sdeh_mlp_forward_impl(const float*, const float*, float*, int64_t, hipStream_t);
 SDEH_HIDDEN int sdeh_mlp_loss_grad_impl(const float*, const float*, const float*, int64_t, float*,
                                         float*, float*, hipStream_t);
@@ -64,9 +61,6 @@ namespace {
 enum { SAVE_END = 0, SAVE_AT = 1, SAVE_STEPS = 2 };
 constexpr int64_t kUserBase = 1000;
 constexpr int64_t kDefaultMaxiters = 10000000;
-// Rows per chunk of the packed save_everystep pool: one device atomic per this
-// many saved points, and at most this many rows of slack per trajectory.
-constexpr int kPackChunk = 8;
 
 struct Builtin {
   int64_t id;
@@ -140,8 +134,8 @@ struct sdeh_ctx {
   int cus = 0;
   char* arena = nullptr;
   size_t arena_cap = 0;
-  // [0] queue, [1..4] count_kernel outputs, [5] order check, [6] packed points
-  // offered, [7] packed chunk claims
+  // [0] queue, [1..4] count_kernel outputs, [5] order check, [6] packed rows
+  // offered (the offsets scan's total)
   unsigned long long* counters = nullptr;
   std::vector<UserRhs> user;
   std::map<std::string, std::pair<hipModule_t, hipFunction_t>> rtc;
@@ -405,14 +399,13 @@ struct Plan {
   int save = SAVE_END;
   int kahan = 0;
   size_t es = 8;  // element size of the state type
-  // packed save_everystep: the kernel fills the pool (a.u_steps/a.t_steps); the
-  // finishing passes write these
+  // packed save_everystep: a counting pass (k_count, the endpoint kernel; adaptive
+  // solvers only), the offsets scan, then the saving kernel
   bool packed = false;
   int64_t pk_cap = 0;
   int64_t* pk_off = nullptr;
-  void* pk_out_u = nullptr;
-  double* pk_out_t = nullptr;
   int64_t* pk_tiles = nullptr;
+  KernelRef k_count;
   KernelRef k;
   Shape shape;
   sdeh_kargs a{};
@@ -487,10 +480,7 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
   a.traj_major = d->save_layout == SDEH_SAVE_TRAJ_MAJOR;
   if (pl->packed) {
     pl->pk_cap = io->steps_capacity;
-    a.pk_chunk = kPackChunk;
-    // every trajectory's last chunk may be partly empty: capacity / chunk chunks
-    // plus one per trajectory hold any ensemble whose points fit the capacity
-    a.pk_chunks = (pl->pk_cap + kPackChunk - 1) / kPackChunk + d->n_traj;
+    a.packed = 1;
   }
   if (!adaptive) {
     if (d->dt == 0.0) return fail(c, SDEH_E_INVALID, "range step cannot be zero");
@@ -505,6 +495,11 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
   }
   r = resolve_kernel(c, d->rhs, d->alg, d->dtype, pl->save, pl->kahan, d->lanes, &pl->k);
   if (r) return r;
+  if (pl->packed && adaptive) {  // the counting pass: the endpoint kernel, same layout
+    r = resolve_kernel(c, d->rhs, d->alg, d->dtype, SAVE_END, pl->kahan, pl->k.lanes,
+                       &pl->k_count);
+    if (r) return r;
+  }
 
   // launch shape
   Shape sh;
@@ -538,8 +533,8 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
 // Bytes of each io buffer for a plan.
 struct Sizes {
   size_t u0, p, ps, tspan, dt0, order, u_end, t_end, i32, u_saveat, u_steps, t_steps, saveat;
-  // packed layout: offsets output, and the pool the kernel fills
-  size_t pk_off, pool_u, pool_t, pk_traj, pk_seq, pk_tiles;
+  // packed layout: offsets output and scan scratch
+  size_t pk_off, pk_tiles;
 };
 
 Sizes sizes_of(const sdeh_solve_desc* d, const sdeh_io* io, const Plan& pl) {
@@ -559,14 +554,10 @@ Sizes sizes_of(const sdeh_solve_desc* d, const sdeh_io* io, const Plan& pl) {
   s.t_steps = pl.save == SAVE_STEPS ? (size_t)d->max_saves * n * 8 : 0;
   s.saveat = pl.save == SAVE_AT ? (size_t)d->n_saveat * 8 : 0;
   if (pl.packed) {
-    const size_t cap = (size_t)pl.pk_cap, rows = (size_t)pl.a.pk_chunks * kPackChunk;
+    const size_t cap = (size_t)pl.pk_cap;
     s.u_steps = cap * pl.n_state * pl.es;
     s.t_steps = cap * 8;
     s.pk_off = (n + 1) * 8;
-    s.pool_u = rows * pl.n_state * pl.es;
-    s.pool_t = rows * 8;
-    s.pk_traj = (size_t)pl.a.pk_chunks * 8;
-    s.pk_seq = (size_t)pl.a.pk_chunks * 4;
     s.pk_tiles = ((n + 1023) / 1024 + 1) * 8;
   }
   return s;
@@ -593,17 +584,24 @@ int run_plan(sdeh_ctx* c, Plan& pl, hipStream_t s, unsigned long long* queue, bo
     else HIPCHK(c, sdeh_util_fill_f32((float*)a.u_saveat, __builtin_nanf(""), cnt, s));
   }
   a.queue = queue;
-  if (a.persistent) HIPCHK(c, sdeh_util_zero_u64(queue, 1, s));
-  if (pl.packed && a.n > 0) HIPCHK(c, sdeh_util_zero_u64(a.pk_next, 1, s));
   if (a.n > 0) {
     if (timed) HIPCHK(c, hipEventRecord(c->ev0, s));
+    if (pl.packed) {
+      // rows per trajectory: naccept + 1 from an endpoint pass (the same arithmetic,
+      // so the same steps), or the grid length of a fixed-step solver
+      const bool counted = pl.k_count.aot || pl.k_count.fn;
+      if (counted) {
+        if (a.persistent) HIPCHK(c, sdeh_util_zero_u64(queue, 1, s));
+        HIPCHK(c, launch(pl.k_count, pl.shape.grid, pl.shape.block, &a, s));
+      }
+      const int32_t add = counted ? 1 : (int32_t)std::max<int64_t>(a.g_len, 1);
+      HIPCHK(c, sdeh_util_pack_offsets(counted ? a.naccept : nullptr, add, a.n, pl.pk_tiles,
+                                       pl.pk_cap, pl.pk_off, a.n_steps, c->counters + 6, s));
+    }
+    if (a.persistent) HIPCHK(c, sdeh_util_zero_u64(queue, 1, s));
     HIPCHK(c, launch(pl.k, pl.shape.grid, pl.shape.block, &a, s));
     if (timed) HIPCHK(c, hipEventRecord(c->ev1, s));
   }
-  if (pl.packed && a.n > 0)
-    HIPCHK(c, sdeh_util_pack(a.n_steps, a.retcode, a.n, pl.pk_tiles, pl.pk_cap, pl.pk_off,
-                             a.u_steps, a.t_steps, a.pk_traj, a.pk_seq, a.pk_next, a.pk_chunks,
-                             a.pk_chunk, pl.n_state, (int)pl.es, pl.pk_out_u, pl.pk_out_t, s));
   return SDEH_OK;
 }
 
@@ -623,7 +621,9 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
       return fail(c, SDEH_E_INVALID, "p_shared is required for this rhs");
     if (pl.save == SAVE_AT && !io->u_saveat)
       return fail(c, SDEH_E_INVALID, "u_saveat is required with saveat");
-    if (pl.save == SAVE_STEPS && (!io->u_steps || !io->t_steps || !io->n_steps))
+    // (a packed call with steps_capacity 0 stores no rows: it reports steps_needed)
+    const bool rows = !(pl.packed && pl.pk_cap == 0);
+    if (pl.save == SAVE_STEPS && ((rows && (!io->u_steps || !io->t_steps)) || !io->n_steps))
       return fail(c, SDEH_E_INVALID, "u_steps, t_steps and n_steps are required");
     if (pl.packed && !io->steps_offset)
       return fail(c, SDEH_E_INVALID, "steps_offset is required with the packed layout");
@@ -648,8 +648,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   if (own_counts) need += 2 * round256(sz.i32);
   if (!host && io->order) need += round256(sz.i32);
   if (pl.packed) {
-    need += round256(sz.pool_u) + round256(sz.pool_t) + round256(sz.pk_traj) +
-            round256(sz.pk_seq) + round256(sz.i32) + round256(sz.pk_tiles);
+    need += round256(sz.pk_tiles);
     if (host) need += round256(sz.pk_off);
   }
   r = ensure_arena(c, need);
@@ -716,17 +715,10 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   a.nreject = (!host && io->nreject) ? io->nreject : nullptr;
   if (!a.naccept) a.naccept = (int32_t*)ar.take(sz.i32);
   if (!a.nreject) a.nreject = (int32_t*)ar.take(sz.i32);
-  if (pl.packed) {  // the kernel saves into the pool; the finishing passes fill the outputs
-    pl.pk_out_u = a.u_steps;
-    pl.pk_out_t = a.t_steps;
+  if (pl.packed) {
     pl.pk_off = host ? (int64_t*)ar.take(sz.pk_off) : io->steps_offset;
-    a.u_steps = ar.take(sz.pool_u);
-    a.t_steps = (double*)ar.take(sz.pool_t);
-    a.pk_traj = (int64_t*)ar.take(sz.pk_traj);
-    a.pk_seq = (int32_t*)ar.take(sz.pk_seq);
-    a.pk_need = (int32_t*)ar.take(sz.i32);
     pl.pk_tiles = (int64_t*)ar.take(sz.pk_tiles);
-    a.pk_next = c->counters + 7;
+    a.steps_off = pl.pk_off;
     if (n == 0 && io->steps_offset) {
       if (host) {
         io->steps_offset[0] = 0;
@@ -753,9 +745,8 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   const bool do_sync = host || sync;
   unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};  // counters [1..6]
   if (do_sync) {  // counters are only read back by synchronous calls
-    HIPCHK(c, sdeh_util_zero_u64(c->counters + 1, 6, s));
+    HIPCHK(c, sdeh_util_zero_u64(c->counters + 1, 4, s));
     HIPCHK(c, sdeh_util_count(a.naccept, a.nreject, a.retcode, n, c->counters + 1, s));
-    if (pl.packed) HIPCHK(c, sdeh_util_sum_i32(a.pk_need, n, c->counters + 6, s));
   }
   if (do_sync) HIPCHK(c, hipMemcpyAsync(cnt, c->counters + 1, sizeof cnt, hipMemcpyDeviceToHost, s));
   if (host && n > 0) {
@@ -772,8 +763,8 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
       int64_t used = 0;
       HIPCHK(c, hipMemcpyAsync(&used, pl.pk_off + n, 8, hipMemcpyDeviceToHost, s));
       HIPCHK(c, hipStreamSynchronize(s));
-      down(io->u_steps, pl.pk_out_u, (size_t)used * pl.n_state * pl.es);
-      down(io->t_steps, pl.pk_out_t, (size_t)used * 8);
+      down(io->u_steps, a.u_steps, (size_t)used * pl.n_state * pl.es);
+      down(io->t_steps, a.t_steps, (size_t)used * 8);
       down(io->steps_offset, pl.pk_off, sz.pk_off);
     } else {
       down(io->u_steps, a.u_steps, sz.u_steps);
@@ -795,7 +786,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
     c->stats.kernel_ms = ms;
     c->stats.attempts = (int64_t)cnt[0];
     c->stats.accepts = (int64_t)cnt[1];
-    c->stats.steps_needed = pl.packed ? (int64_t)cnt[5] : 0;
+    c->stats.steps_needed = pl.packed && n > 0 ? (int64_t)cnt[5] : 0;
     c->stats.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
             .count();

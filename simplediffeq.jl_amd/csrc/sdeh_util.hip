@@ -64,9 +64,9 @@ __global__ void check_order_kernel(const int64_t* order, int64_t n, int32_t* see
 }
 
 // ---- packed save_everystep ------------------------------------------------
-// Counts n_steps[0..n) are scanned in tiles of PK_TILE entries (256 threads x 4):
-// pk_tile_sums -> pk_scan_tiles (one block) -> pk_offsets; pk_compact then moves
-// each stored pool row to its place in the packed output.
+// Row counts cnt_i = (base ? base[i] : 0) + add are scanned in tiles of PK_TILE
+// entries (256 threads x 4): pk_tile_sums -> pk_scan_tiles (one block) ->
+// pk_offsets.
 constexpr int PK_TILE = 1024;
 
 // Exclusive scan over the 256 threads of a block; *total gets the block sum.
@@ -87,19 +87,25 @@ __device__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
   return incl - v;
 }
 
-__global__ void __launch_bounds__(256) pk_tile_sums(const int32_t* cnt, int64_t n,
+__device__ __forceinline__ int64_t pk_count(const int32_t* base, int32_t add, int64_t i) {
+  return (base ? (int64_t)base[i] : 0) + add;
+}
+
+__global__ void __launch_bounds__(256) pk_tile_sums(const int32_t* base, int32_t add, int64_t n,
                                                     int64_t* tile_sum) {
   const int64_t i0 = (int64_t)blockIdx.x * PK_TILE + 4 * threadIdx.x;
   int64_t v = 0;
   for (int j = 0; j < 4; ++j)
-    if (i0 + j < n) v += cnt[i0 + j];
+    if (i0 + j < n) v += pk_count(base, add, i0 + j);
   int64_t total = 0;
   (void)block_exclusive_scan(v, &total);
   if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
 }
 
-// One block: tile_sum[0..n_tiles) -> exclusive prefix sums, in place.
-__global__ void __launch_bounds__(256) pk_scan_tiles(int64_t* tile_sum, int64_t n_tiles) {
+// One block: tile_sum[0..n_tiles) -> exclusive prefix sums, in place; *grand gets
+// the sum of all counts.
+__global__ void __launch_bounds__(256) pk_scan_tiles(int64_t* tile_sum, int64_t n_tiles,
+                                                     unsigned long long* grand) {
   int64_t carry = 0;
   for (int64_t base = 0; base < n_tiles; base += 256) {
     const int64_t i = base + threadIdx.x;
@@ -109,19 +115,19 @@ __global__ void __launch_bounds__(256) pk_scan_tiles(int64_t* tile_sum, int64_t 
     if (i < n_tiles) tile_sum[i] = carry + ex;
     carry += total;
   }
+  if (threadIdx.x == 0) *grand = (unsigned long long)carry;
 }
 
-// off[i] = first row of trajectory i, clamped to the capacity; a trajectory whose
-// rows would pass the capacity keeps the rows that fit (cnt[i] reduced) and, if it
-// had integrated normally, gets retcode 4 (save overflow).  off[n] = rows used.
-__global__ void __launch_bounds__(256) pk_offsets(int32_t* cnt, int32_t* ret, int64_t n,
+// off[i] = first row of trajectory i, clamped to the capacity; lim[i] = the rows
+// it may write (its count, or what is left below the capacity); off[n] = rows used.
+__global__ void __launch_bounds__(256) pk_offsets(const int32_t* base, int32_t add, int64_t n,
                                                   const int64_t* tile_off, int64_t cap,
-                                                  int64_t* off) {
+                                                  int64_t* off, int32_t* lim) {
   const int64_t i0 = (int64_t)blockIdx.x * PK_TILE + 4 * threadIdx.x;
   int64_t c[4];
   int64_t v = 0;
   for (int j = 0; j < 4; ++j) {
-    c[j] = i0 + j < n ? cnt[i0 + j] : 0;
+    c[j] = i0 + j < n ? pk_count(base, add, i0 + j) : 0;
     v += c[j];
   }
   int64_t total = 0;
@@ -129,50 +135,12 @@ __global__ void __launch_bounds__(256) pk_offsets(int32_t* cnt, int32_t* ret, in
   for (int j = 0; j < 4; ++j) {
     const int64_t i = i0 + j;
     if (i >= n) break;
-    if (o + c[j] > cap) {
-      const int64_t keep = cap > o ? cap - o : 0;
-      cnt[i] = (int32_t)keep;
-      if (ret[i] == 0) ret[i] = 4;
-    }
+    const int64_t room = cap > o ? cap - o : 0;
+    lim[i] = (int32_t)(c[j] < room ? c[j] : room);
     off[i] = o < cap ? o : cap;
     o += c[j];
     if (i == n - 1) off[n] = o < cap ? o : cap;
   }
-}
-
-// Rows [0, min(*next, n_chunks) * chunk) of the pool: row r of chunk r / chunk is
-// point pk_seq * chunk + r % chunk of trajectory pk_traj; copied when that point
-// is among the trajectory's kept ones.
-template <class T>
-__global__ void __launch_bounds__(256) pk_compact(const T* pool_u, const double* pool_t,
-                                                  const int64_t* pk_traj, const int32_t* pk_seq,
-                                                  const unsigned long long* next,
-                                                  int64_t n_chunks, int chunk, int n_state,
-                                                  const int32_t* cnt, const int64_t* off,
-                                                  T* out_u, double* out_t) {
-  const unsigned long long claimed = *next;
-  const int64_t used = (int64_t)claimed < n_chunks ? (int64_t)claimed : n_chunks;
-  const int64_t rows = used * chunk;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
-       r += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t ch = r / chunk;
-    const int64_t traj = pk_traj[ch];
-    const int64_t k = (int64_t)pk_seq[ch] * chunk + (r - ch * chunk);
-    if (k >= cnt[traj]) continue;
-    const int64_t dst = off[traj] + k;
-    out_t[dst] = pool_t[r];
-    for (int e = 0; e < n_state; ++e) out_u[dst * n_state + e] = pool_u[r * n_state + e];
-  }
-}
-
-__global__ void __launch_bounds__(256) sum_i32_kernel(const int32_t* v, int64_t n,
-                                                      unsigned long long* out) {
-  unsigned long long s = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    s += (unsigned long long)(uint32_t)v[i];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
-  if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
 }
 
 __global__ void write_i64(int64_t* p, int64_t v) {
@@ -229,40 +197,20 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_check_orde
   return hipGetLastError();
 }
 
-// Finishing passes of the packed layout (see pk_* above).  tiles: scratch of
-// ceil(n / 1024) int64.  es: element size of the state type (8 or 4).
-extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_pack(
-    int32_t* cnt, int32_t* ret, int64_t n, int64_t* tiles, int64_t cap, int64_t* off,
-    const void* pool_u, const double* pool_t, const int64_t* pk_traj, const int32_t* pk_seq,
-    const unsigned long long* next, int64_t n_chunks, int chunk, int n_state, int es,
-    void* out_u, double* out_t, hipStream_t s) {
+// Offsets of the packed layout from the row counts (base[i] + add, or add when
+// base is null): off [n + 1], lim [n] (rows each trajectory may write), *grand =
+// total rows.  tiles: scratch of ceil(n / 1024) int64.
+extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_pack_offsets(
+    const int32_t* base, int32_t add, int64_t n, int64_t* tiles, int64_t cap, int64_t* off,
+    int32_t* lim, unsigned long long* grand, hipStream_t s) {
   if (n <= 0) {
     hipLaunchKernelGGL(write_i64, dim3(1), dim3(64), 0, s, off, (int64_t)0);
     return hipGetLastError();
   }
   const int64_t n_tiles = (n + PK_TILE - 1) / PK_TILE;
-  hipLaunchKernelGGL(pk_tile_sums, dim3((unsigned)n_tiles), dim3(256), 0, s, cnt, n, tiles);
-  hipLaunchKernelGGL(pk_scan_tiles, dim3(1), dim3(256), 0, s, tiles, n_tiles);
-  hipLaunchKernelGGL(pk_offsets, dim3((unsigned)n_tiles), dim3(256), 0, s, cnt, ret, n, tiles,
-                     cap, off);
-  const int g = 2048;  // grid-stride: the used row count is only known on the device
-  if (es == 8)
-    hipLaunchKernelGGL(pk_compact<double>, dim3(g), dim3(256), 0, s, (const double*)pool_u,
-                       pool_t, pk_traj, pk_seq, next, n_chunks, chunk, n_state, cnt, off,
-                       (double*)out_u, out_t);
-  else
-    hipLaunchKernelGGL(pk_compact<float>, dim3(g), dim3(256), 0, s, (const float*)pool_u, pool_t,
-                       pk_traj, pk_seq, next, n_chunks, chunk, n_state, cnt, off,
-                       (float*)out_u, out_t);
-  return hipGetLastError();
-}
-
-// *out += sum(v[0..n)) (out must be cleared by the caller).
-extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_sum_i32(
-    const int32_t* v, int64_t n, unsigned long long* out, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  int g = grid_for(n);
-  if (g > 512) g = 512;
-  hipLaunchKernelGGL(sum_i32_kernel, dim3(g), dim3(256), 0, s, v, n, out);
+  hipLaunchKernelGGL(pk_tile_sums, dim3((unsigned)n_tiles), dim3(256), 0, s, base, add, n, tiles);
+  hipLaunchKernelGGL(pk_scan_tiles, dim3(1), dim3(256), 0, s, tiles, n_tiles, grand);
+  hipLaunchKernelGGL(pk_offsets, dim3((unsigned)n_tiles), dim3(256), 0, s, base, add, n, tiles,
+                     cap, off, lim);
   return hipGetLastError();
 }

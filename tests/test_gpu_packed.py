@@ -1,10 +1,10 @@
 """Packed save_everystep output (no per-trajectory cap): every trajectory's rows
-back to back at steps_offset[i] .. steps_offset[i+1]-1, filled in one pass
-through chunks claimed from a device pool and then compacted.  Checked bit for
-bit against the oracle's per-trajectory rows, for every algorithm and both
-schedules, and, when the rows do not fit, through the invariants of a
-truncation (each trajectory keeps a prefix of its own rows, the offsets stay a
-consistent CSR, and steps_needed names the capacity that fits)."""
+back to back at steps_offset[i] .. steps_offset[i+1]-1 (a counting pass, an
+offsets scan, then the saving pass writing in place).  Checked bit for bit
+against the oracle's per-trajectory rows, for every algorithm and both
+schedules, and, when the rows do not fit, through the truncation rule (rows in
+trajectory order until the capacity is full: earlier trajectories complete, one
+partial, later ones empty; steps_needed names the capacity that fits)."""
 import numpy as np
 import pytest
 
@@ -100,9 +100,10 @@ def test_packed_overflow_keeps_prefixes(ctx, frac):
     assert ctx.stats()["steps_needed"] == int(full.sum())
     k = g.numpy("n_steps").astype(np.int64)
     off = g.numpy("steps_offset")
-    # the pool (capacity / 8 + n chunks of 8 rows) always holds more points than
-    # fit, so exactly the capacity is used
     assert off[0] == 0 and np.array_equal(np.diff(off), k) and off[n] == cap
+    cut = int(np.argmax(k < full))  # the first trajectory that lost rows
+    assert (k[:cut] == full[:cut]).all() and (k[cut + 1:] == 0).all()
+    assert off[cut] + full[cut] > cap
     ret = g.numpy("retcode")
     assert ((ret == _abi.RET_SAVE_OVERFLOW) == (k < full)).all()
     assert ((ret == _abi.RET_DEFAULT) == (k == full)).all()
@@ -130,6 +131,11 @@ def test_packed_validation_and_empty(ctx):
     b.io.steps_capacity = -1
     with pytest.raises(runtime.SolveError, match="steps_capacity < 0"):
         ctx.solve(d, b)
+    # capacity 0 with no row buffers: a probe for the capacity that fits
+    b = runtime.EnsembleBuffers(d, 3, u0, p, steps_capacity=0)
+    b.io.u_steps = b.io.t_steps = None
+    assert ctx.solve(d, b, allow=(_abi.E_SAVE_OVERFLOW,)) == _abi.E_SAVE_OVERFLOW
+    assert ctx.stats()["steps_needed"] > 4 and (b.n_steps == 0).all()
     d0 = _desc(_abi.ALG_ATSIT5, 0)
     e = runtime.EnsembleBuffers(d0, 3, np.zeros((3, 0)), np.zeros((3, 0)), steps_capacity=8)
     e.steps_offset[0] = 7
