@@ -63,7 +63,7 @@ def algorithmic_flops(alg, n_state, f_rhs, attempts, accepts, n_traj):
     return float(total)
 
 
-def cpu_baseline(u0, p, desc_kwargs, sample, threads, min_seconds=10.0, max_reps=40):
+def cpu_baseline(u0, p, desc_kwargs, sample, threads, min_seconds=10.0, max_reps=400):
     """Oracle solves of the first `sample` trajectories, repeated until at least
     `min_seconds` of CPU time have elapsed; returns (trajectories/s, seconds, reps)."""
     from oracle import pyoracle
@@ -448,11 +448,24 @@ def main():
         flops = algorithmic_flops(_abi.ALG_ATSIT5, 3, LORENZ_FLOPS, st["attempts"], st["accepts"], n)
         es = 8 if dtype == _abi.F64 else 4
         alg_bytes = n * (3 * es + 3 * es + 3 * es + 8 + 12)  # u0, p, u_end, t_end, 3 int32
+        host_ms = None
+        if dtype == _abi.F64:
+            # the same solve from host (pageable numpy) buffers: the library stages
+            # inputs to HBM and outputs back, so this rate includes both PCIe copies
+            hb = runtime.EnsembleBuffers(d, 3, u0, p)
+            ctx.solve(d, hb)
+            t1 = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                ctx.solve(d, hb)
+            host_ms = (time.perf_counter() - t1) * 1e3 / reps
+            del hb
         results[dtype_name] = dict(
             wall_s=wall, ms_per_step=wall * 1e3 / args.steps, kernel_ms=dev_ms,
             traj_per_s=n * args.steps / wall, attempts=st["attempts"], accepts=st["accepts"],
             flops=flops, tflops=flops / (dev_ms * 1e-3) / 1e12, bytes=alg_bytes,
-            gbs=alg_bytes / (dev_ms * 1e-3) / 1e9, shape=dict(block=st["block"], grid=st["grid"],
+            gbs=alg_bytes / (dev_ms * 1e-3) / 1e9, host_ms=host_ms,
+            shape=dict(block=st["block"], grid=st["grid"],
                                                              chunk=st["chunk"],
                                                              schedule=st["schedule"]))
         del bufs
@@ -482,6 +495,9 @@ def main():
                      "kernel_ms": r64["kernel_ms"], "algorithmic_flops_per_launch": r64["flops"],
                      "attempts_per_launch": r64["attempts"],
                      "hbm_gbs": r64["gbs"], "hbm_frac": r64["gbs"] / HBM_PEAK_GBS},
+        # not `value`: the rate when the caller hands over host buffers (PCIe both ways)
+        "host_buffers": {"ms_per_solve": r64["host_ms"],
+                         "traj_per_s": n / (r64["host_ms"] * 1e-3)},
     }
     if "f32" in results:
         r32 = results["f32"]
