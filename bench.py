@@ -63,25 +63,54 @@ def algorithmic_flops(alg, n_state, f_rhs, attempts, accepts, n_traj):
     return float(total)
 
 
-def cpu_baseline(u0, p, desc_kwargs, sample, threads, min_seconds=10.0, max_reps=400):
-    """Oracle solves of the first `sample` trajectories, repeated until at least
-    `min_seconds` of CPU time have elapsed; returns (trajectories/s, seconds, reps)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_time(desc, n_state, u0, p, ps=None, tspan=None, threads=1, min_seconds=10.0,
+             max_reps=400):
+    """Oracle solves of the host ensemble (u0, p, ...) described by `desc`, repeated
+    until at least `min_seconds` have elapsed; returns (trajectories/s, seconds, reps)."""
     from oracle import pyoracle
     from simplediffeq_hip import runtime
-    d = runtime.make_desc(n_traj=sample, **desc_kwargs)
-    bufs = runtime.EnsembleBuffers(d, 3, u0[:, :sample], p[:, :sample])
-    pyoracle.solve(d, bufs, nthreads=threads)  # warm-up (page in, thread start)
+    n = int(desc.n_traj)
+    bufs = runtime.EnsembleBuffers(desc, n_state, u0, p, ps, tspan)
+    pyoracle.solve(desc, bufs, nthreads=threads)  # warm-up (page in, thread start)
     reps = 0
     t0 = time.perf_counter()
     while True:
-        st = pyoracle.solve(d, bufs, nthreads=threads)
+        st = pyoracle.solve(desc, bufs, nthreads=threads)
         if st != 0:
             raise RuntimeError("oracle status %d" % st)
         reps += 1
         el = time.perf_counter() - t0
         if el >= min_seconds or reps >= max_reps:
             break
-    return sample * reps / el, el, reps
+    return n * reps / el, el, reps
+
+
+def cpu_leg(desc, n_state, u0, p, ps=None, tspan=None, min_seconds=3.0):
+    """The `cpu_baseline` record of a workload (oracle on the host cores)."""
+    threads = min(16, len(os.sched_getaffinity(0)))
+    rate, el, reps = cpu_time(desc, n_state, u0, p, ps, tspan, threads, min_seconds)
+    return {"value": rate, "unit": "trajectories/s", "cores": threads, "kind": "port",
+            "sample": "first %d trajectories, solved %d times in %.1f s on %s"
+                      % (int(desc.n_traj), reps, el, cpu_model())}
+
+
+def cpu_baseline(u0, p, desc_kwargs, sample, threads, min_seconds=10.0, max_reps=400):
+    """Oracle solves of the first `sample` trajectories, repeated until at least
+    `min_seconds` of CPU time have elapsed; returns (trajectories/s, seconds, reps)."""
+    from simplediffeq_hip import runtime
+    d = runtime.make_desc(n_traj=sample, **desc_kwargs)
+    return cpu_time(d, 3, u0[:, :sample], p[:, :sample], threads=threads,
+                    min_seconds=min_seconds, max_reps=max_reps)
 
 
 def _time_solves(ctx, d, bufs, stream, steps, warmup):
@@ -158,6 +187,10 @@ def workload_uneven(args):
     ms, _ = _time_solves(ctx, d, g, stream, args.steps, args.warmup)
     out["autotuned"] = {"kernel_ms": ms, "traj_per_s": n / (ms * 1e-3),
                         "launch": {k: best[k] for k in ("block", "grid", "chunk", "schedule")}}
+    if not args.no_cpu:
+        k = 1 << 16
+        dc = runtime.make_desc(_abi.ALG_AVERN7, **dict(base, n_traj=k))
+        out["cpu_baseline"] = cpu_leg(dc, 3, u0[:, :k], p[:, :k], tspan=ts[:, :k])
     h = ctx.graph_capture(d, g)
     torch.cuda.synchronize()
     reps = max(args.steps, 5)
@@ -204,6 +237,12 @@ def workload_glv(args):
                                "saveat_GBs": out_bytes / (ms * 1e-3) / 1e9,
                                "tflops_excl_saveat": flops / (ms * 1e-3) / 1e12}
             del g
+        if not args.no_cpu:
+            k = min(n, 1 << 14 if dtype == _abi.F64 else 1 << 16)
+            dc = runtime.make_desc(alg, dtype=dtype, rhs=_abi.RHS_GLV16, n_traj=k,
+                                   tspan=(0.0, 50.0), dt=0.1, abstol=tol, reltol=tol,
+                                   saveat=saveat, save_layout=_abi.SAVE_TRAJ_MAJOR)
+            out[name]["cpu_baseline"] = cpu_leg(dc, 16, u0[:, :k], r[:, :k], A)
     return out
 
 
@@ -317,6 +356,10 @@ def workload_costmodel(args):
     base, ms_base, st_base = run(_abi.F64, _abi.SCHED_ONE_LAUNCH, _abi.PREC_PLAIN)
     out["f64_one_launch_unsorted"] = {"kernel_ms": ms_base, "traj_per_s": n_eval / (ms_base * 1e-3),
                                       "attempts": st_base["attempts"]}
+    if not args.no_cpu:
+        k = min(n_eval, 1 << 16)
+        dc = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=k, dt=f32_01, **tol)
+        out["cpu_baseline"] = cpu_leg(dc, 3, u0[:, :k], p[:, :k], tspan=ts[:, :k])
     rec = {}
     for name, dtype, sched, prec, od, dz in (
             ("f64_persistent_unsorted", _abi.F64, _abi.SCHED_PERSISTENT, _abi.PREC_PLAIN, None, None),
@@ -515,7 +558,8 @@ def main():
         out["cpu_baseline"] = {"value": rate, "unit": "trajectories/s", "cores": threads,
                                "kind": "port",
                                "sample": "first %d trajectories of the same ensemble, binary64, "
-                                         "solved %d times in %.1f s" % (sample, reps, el)}
+                                         "solved %d times in %.1f s on %s"
+                                         % (sample, reps, el, cpu_model())}
     print(json.dumps(out))
     return 0
 
