@@ -30,6 +30,11 @@
  *
  * Threading: a context may be used by one host thread at a time.  Work is
  * ordered on the context's stream (or the caller's stream, see sdeh_solve()).
+ * A device-memory call may return before its kernels finish; they keep using
+ * the context's device scratch (work queue, staged saveat times, counters), so
+ * a later call on the same context that names a different stream first waits
+ * for the earlier stream.  Calls on one stream simply queue behind each other.
+ * For concurrent ensembles on several streams, use one context per stream.
  */
 #ifndef SIMPLEDIFFEQ_HIP_H
 #define SIMPLEDIFFEQ_HIP_H
@@ -256,7 +261,7 @@ int sdeh_graph_destroy(sdeh_ctx* ctx, int64_t graph_handle);
 int sdeh_autotune(sdeh_ctx* ctx, const sdeh_solve_desc* d, const sdeh_io* io,
                   sdeh_stats* best);
 
-/* Synchronise the context stream. */
+/* Synchronise the context stream and the stream of the context's last call. */
 int sdeh_synchronize(sdeh_ctx* ctx);
 
 /* The fixed-step time grid t0:dt:tf with Julia's Float64 range rules (the

@@ -137,6 +137,11 @@ struct sdeh_ctx {
   // [0] queue, [1..4] count_kernel outputs, [5] order check, [6] packed rows
   // offered (the offsets scan's total)
   unsigned long long* counters = nullptr;
+  // The stream of the last call that may still be running and use the arena or
+  // the counters (device-memory calls return once queued): a call on another
+  // stream, or an arena reallocation, waits for it first.
+  hipStream_t pending = nullptr;
+  bool has_pending = false;
   std::vector<UserRhs> user;
   std::map<std::string, std::pair<hipModule_t, hipFunction_t>> rtc;
   std::map<TuneKey, Shape> tuned;
@@ -378,9 +383,25 @@ struct Arena {
 
 size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
 
+// Order this call's use of the context's scratch after the last call's.
+int after_pending(sdeh_ctx* c, hipStream_t s) {
+  if (c->has_pending && c->pending != s) {
+    HIPCHK(c, hipStreamSynchronize(c->pending));
+    c->has_pending = false;
+  }
+  return SDEH_OK;
+}
+
+void note_pending(sdeh_ctx* c, hipStream_t s, bool done) {
+  c->pending = s;
+  c->has_pending = !done;
+}
+
 int ensure_arena(sdeh_ctx* c, size_t need) {
   if (need <= c->arena_cap) return SDEH_OK;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->has_pending) HIPCHK(c, hipStreamSynchronize(c->pending));
+  c->has_pending = false;
   if (c->arena) HIPCHK(c, hipFree(c->arena));
   c->arena = nullptr;
   c->arena_cap = 0;
@@ -611,6 +632,8 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   Plan pl;
   int r = plan_solve(c, d, io, &pl);
   if (r) return r;
+  r = after_pending(c, s);
+  if (r) return r;
   const Sizes sz = sizes_of(d, io, pl);
   const int64_t n = d->n_traj;
   if (n > 0) {
@@ -743,6 +766,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   r = run_plan(c, pl, s, c->counters, true);
   if (r) return r;
   const bool do_sync = host || sync;
+  note_pending(c, s, false);
   unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};  // counters [1..6]
   if (do_sync) {  // counters are only read back by synchronous calls
     HIPCHK(c, sdeh_util_zero_u64(c->counters + 1, 4, s));
@@ -781,6 +805,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   c->stats.lanes = pl.k.lanes;
   if (do_sync) {
     HIPCHK(c, hipStreamSynchronize(s));
+    note_pending(c, s, true);
     float ms = 0.f;
     if (n > 0) HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->stats.kernel_ms = ms;
@@ -915,6 +940,8 @@ int sdeh_get_stats(const sdeh_ctx* c, sdeh_stats* out) {
 int sdeh_synchronize(sdeh_ctx* c) {
   if (!c) return SDEH_E_INVALID;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->has_pending) HIPCHK(c, hipStreamSynchronize(c->pending));
+  c->has_pending = false;
   return SDEH_OK;
 }
 
@@ -1099,10 +1126,12 @@ int sdeh_mlp_loss_grad(sdeh_ctx* c, const float* params, const float* x, const f
   if (r) return r;
   // scratch for per-block partial gradients lives in the arena; its size comes
   // from the MLP unit, which owns the launch shape
-  r = ensure_arena(c, sdeh_mlp_grad_scratch_bytes(n));
+  const hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  r = after_pending(c, s);
+  if (!r) r = ensure_arena(c, sdeh_mlp_grad_scratch_bytes(n));
   if (r) return r;
-  r = sdeh_mlp_loss_grad_impl(params, x, y, n, loss, grad, (float*)c->arena,
-                              stream ? (hipStream_t)stream : c->stream);
+  r = sdeh_mlp_loss_grad_impl(params, x, y, n, loss, grad, (float*)c->arena, s);
+  note_pending(c, s, false);
   if (r) return fail(c, r, "mlp loss/grad launch failed");
   return SDEH_OK;
 }

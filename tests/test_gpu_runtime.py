@@ -36,6 +36,31 @@ def test_order_permutation_leaves_results_unchanged(ctx):
                 assert bits_equal(b.numpy(f), ref.numpy(f)), (sched, chunk, f)
 
 
+def test_async_solves_on_two_streams_share_the_context_safely(ctx):
+    """Device-memory solves return once queued; a second call on another stream
+    must not overwrite the first one's staged saveat times or work queue."""
+    import torch
+    n = 200000
+    u0, p = inputs.lorenz_wide(n, seed=5)
+    sa = [np.linspace(0.0, 5.0, 11), np.linspace(0.5, 4.5, 7)]
+    descs = [_desc(n=n, saveat=sa[k], schedule=_abi.SCHED_PERSISTENT) for k in range(2)]
+    refs = []
+    for d in descs:
+        b = _dev_bufs(d, u0, p)
+        ctx.solve(d, b)
+        refs.append(b)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    outs = [_dev_bufs(d, u0, p) for d in descs]
+    torch.cuda.synchronize()
+    ctx.solve(descs[0], outs[0], stream=s1.cuda_stream, sync=False)
+    ctx.solve(descs[1], outs[1], stream=s2.cuda_stream, sync=False)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    for k in range(2):
+        for f in ("u_end", "u_saveat", "naccept", "nreject", "retcode"):
+            assert bits_equal(outs[k].numpy(f), refs[k].numpy(f)), (k, f)
+
+
 def test_persistent_grid_smaller_than_needed_still_covers_all(ctx):
     n = 3000
     u0, p = inputs.lorenz_wide(n, seed=1)
