@@ -13,6 +13,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -902,9 +903,12 @@ int sdeh_compile_rhs(sdeh_ctx* c, const char* src, const char* fn_name, int32_t 
   if (!c || !src || !fn_name || !handle) return SDEH_E_INVALID;
   if (n_state < 1 || n_state > 64 || n_param < 0 || n_param > 256 || n_shared < 0)
     return fail(c, SDEH_E_INVALID, "rhs shape out of range (1 <= n_state <= 64)");
-  for (const char* q = fn_name; *q; ++q)
-    if (!(isalnum((unsigned char)*q) || *q == '_'))
-      return fail(c, SDEH_E_INVALID, "fn_name must be an identifier");
+  const size_t len = strlen(fn_name);
+  bool ident = len > 0 && len <= 200 && !isdigit((unsigned char)fn_name[0]);
+  for (const char* q = fn_name; *q && ident; ++q)
+    ident = isalnum((unsigned char)*q) || *q == '_';
+  if (!ident)
+    return fail(c, SDEH_E_INVALID, "fn_name must be a C identifier of at most 200 characters");
   UserRhs u;
   u.src = src;
   u.fn = fn_name;

@@ -61,6 +61,21 @@ def test_async_solves_on_two_streams_share_the_context_safely(ctx):
             assert bits_equal(outs[k].numpy(f), refs[k].numpy(f)), (k, f)
 
 
+def test_compile_rhs_rejects_bad_names_and_sources(ctx):
+    src = ("template <class T> __device__ void decay(T* du, const T* u, const T* p,"
+           " const T* __restrict__ ps, double t) { du[0] = p[0] * u[0]; }")
+    for bad in ("", "1decay", "de cay", "x" * 201):
+        with pytest.raises(runtime.SolveError, match="C identifier"):
+            ctx.compile_rhs(src, bad, 1, 1)
+    with pytest.raises(runtime.SolveError):
+        ctx.compile_rhs(src.replace("du[0] =", "du[0] = = "), "decay", 1, 1)
+    h = ctx.compile_rhs(src, "decay", 1, 1)
+    d = runtime.make_desc(_abi.ALG_TSIT5, rhs=h, n_traj=4, tspan=(0.0, 1.0), dt=0.125)
+    b = runtime.EnsembleBuffers(d, 1, np.ones((1, 4)), np.full((1, 4), -1.0))
+    ctx.solve(d, b)
+    assert np.allclose(b.u_end[0], np.exp(-1.0), rtol=1e-9)
+
+
 def test_persistent_grid_smaller_than_needed_still_covers_all(ctx):
     n = 3000
     u0, p = inputs.lorenz_wide(n, seed=1)
