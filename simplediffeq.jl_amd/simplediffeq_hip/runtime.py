@@ -210,9 +210,26 @@ class Context:
                                               ctypes.byref(h)))
         return h.value
 
+    @staticmethod
+    def _torch_stream(bufs):
+        """The stream for a solve on device buffers: torch's current stream, so
+        that work queued there (the kernels that produced the inputs) completes
+        before the solve starts, as for any torch operation.  torch's default
+        stream has handle 0, which the library reads as its own stream: then
+        torch's stream is synchronised first and None is returned."""
+        import torch
+        ts = torch.cuda.current_stream(bufs.u0.device)
+        if ts.cuda_stream:
+            return ts.cuda_stream
+        ts.synchronize()
+        return None
+
     def solve(self, desc, bufs, stream=None, sync=True, allow=(_abi.E_DTMIN,)):
-        """Run sdeh_solve on an EnsembleBuffers; returns the status."""
+        """Run sdeh_solve on an EnsembleBuffers; returns the status.  Device
+        (torch) buffers run on torch's current stream unless `stream` is given."""
         mem = _abi.MEM_HOST if bufs.backend == "numpy" else _abi.MEM_DEVICE
+        if stream is None and bufs.backend == "torch":
+            stream = self._torch_stream(bufs)
         st = self.lib.sdeh_solve(self.h, ctypes.byref(desc), ctypes.byref(bufs.io), mem,
                                  ctypes.c_void_p(stream), 1 if sync else 0)
         return self._check(st, allow)
@@ -223,12 +240,18 @@ class Context:
         return {k: getattr(s, k) for k, _ in _abi.Stats._fields_}
 
     def autotune(self, desc, bufs):
+        if bufs.backend == "torch":  # the tuner runs on the context's stream
+            import torch
+            torch.cuda.current_stream(bufs.u0.device).synchronize()
         s = _abi.Stats()
         self._check(self.lib.sdeh_autotune(self.h, ctypes.byref(desc), ctypes.byref(bufs.io),
                                            ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in _abi.Stats._fields_}
 
     def graph_capture(self, desc, bufs):
+        if bufs.backend == "torch":  # capture runs on the context's stream
+            import torch
+            torch.cuda.current_stream(bufs.u0.device).synchronize()
         h = ctypes.c_int64()
         self._check(self.lib.sdeh_graph_capture(self.h, ctypes.byref(desc), ctypes.byref(bufs.io),
                                                 ctypes.byref(h)))
