@@ -81,6 +81,9 @@ class CostModel:
         self.v = torch.zeros(NPARAM, dtype=torch.float32, device=self.dev)
         self.grad = torch.zeros(NPARAM, dtype=torch.float32, device=self.dev)
         self.loss = torch.zeros(1, dtype=torch.float32, device=self.dev)
+        # the library's kernels run on the context's stream: the zero fills above
+        # (torch's stream) must be complete before the first Adam step reads m, v
+        torch.cuda.synchronize(self.dev)
         self.t = 0
         # y normalisation (kept on the host, applied to targets/predictions)
         self.y_mean = np.zeros(NOUT, dtype=np.float32)
