@@ -62,6 +62,12 @@ namespace {
 enum { SAVE_END = 0, SAVE_AT = 1, SAVE_STEPS = 2 };
 constexpr int64_t kUserBase = 1000;
 constexpr int64_t kDefaultMaxiters = 10000000;
+// Host-memory solves of at least kPipeMin trajectories run as kPipeChunks chunks:
+// the inputs of one chunk travel to HBM while the previous chunk integrates, and
+// its outputs travel back while the next one does.
+constexpr int64_t kPipeMin = 1 << 17;
+constexpr int kPipeChunks = 4;
+static_assert(8 + kPipeChunks <= 16, "chunk queues live in counters[8..16)");
 
 struct Builtin {
   int64_t id;
@@ -129,14 +135,17 @@ uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ull) {
 struct sdeh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t s_in = nullptr, s_out = nullptr;  // host-memory pipeline copies
+  hipStream_t s_k2 = nullptr;  // the pipeline's second kernel stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_pipe[2 * kPipeChunks] = {};  // per chunk: inputs landed, kernel done
   std::string err;
   sdeh_stats stats{};
   int cus = 0;
   char* arena = nullptr;
   size_t arena_cap = 0;
   // [0] queue, [1..4] count_kernel outputs, [5] order check, [6] packed rows
-  // offered (the offsets scan's total)
+  // offered (the offsets scan's total), [8..8+kPipeChunks) pipeline chunk queues
   unsigned long long* counters = nullptr;
   // The stream of the last call that may still be running and use the arena or
   // the counters (device-memory calls return once queued): a call on another
@@ -403,6 +412,9 @@ int ensure_arena(sdeh_ctx* c, size_t need) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->has_pending) HIPCHK(c, hipStreamSynchronize(c->pending));
   c->has_pending = false;
+  HIPCHK(c, hipStreamSynchronize(c->s_in));  // pipeline work on the old arena
+  HIPCHK(c, hipStreamSynchronize(c->s_k2));
+  HIPCHK(c, hipStreamSynchronize(c->s_out));
   if (c->arena) HIPCHK(c, hipFree(c->arena));
   c->arena = nullptr;
   c->arena_cap = 0;
@@ -627,6 +639,114 @@ int run_plan(sdeh_ctx* c, Plan& pl, hipStream_t s, unsigned long long* queue, bo
   return SDEH_OK;
 }
 
+// Trajectories [a, a + m) of a host array with `rows` rows per trajectory, to or
+// from its device copy.  Host: [row][n] (element (r, i) at r * n + i), or, with
+// traj_major, [n][row].  Device: the chunk-blocked copy, in which chunk [a, a+m)
+// is a standalone [row][m] (or [m][row]) array starting at element rows * a.
+hipError_t chunk_copy(char* dev, const void* host_in, void* host_out, size_t es, int64_t rows,
+                      int64_t n, int64_t a, int64_t m, bool traj_major, hipStream_t st) {
+  const bool h2d = host_in != nullptr;
+  char* host = h2d ? (char*)host_in : (char*)host_out;
+  if (!host || rows == 0 || m == 0) return hipSuccess;
+  char* dblk = dev + (size_t)rows * a * es;
+  if (traj_major || rows == 1) {
+    char* hblk = host + (size_t)rows * a * es;
+    const size_t bytes = (size_t)rows * m * es;
+    return h2d ? hipMemcpyAsync(dblk, hblk, bytes, hipMemcpyHostToDevice, st)
+               : hipMemcpyAsync(hblk, dblk, bytes, hipMemcpyDeviceToHost, st);
+  }
+  char* hcol = host + (size_t)a * es;
+  const size_t w = (size_t)m * es, hp = (size_t)n * es;
+  return h2d ? hipMemcpy2DAsync(dblk, w, hcol, hp, w, (size_t)rows, hipMemcpyHostToDevice, st)
+             : hipMemcpy2DAsync(hcol, hp, dblk, w, w, (size_t)rows, hipMemcpyDeviceToHost, st);
+}
+
+// Host-memory solve in kPipeChunks chunks of trajectories: inputs go up on s_in,
+// each chunk integrates as a standalone ensemble once its inputs have landed
+// (chunks alternate between `s` and s_k2, so one chunk's end-of-launch drain
+// overlaps the next one's start; each has its own work-queue counter), and its
+// outputs come back on s_out once its kernel is done.  The device buffers in
+// pl.a are chunk-blocked (chunk_copy).  ev0/ev1 bracket the kernels, and `s`
+// finishes after all of them.  Per-trajectory results do not depend on the
+// chunking.
+int run_host_pipeline(sdeh_ctx* c, Plan& pl, const sdeh_io* io, hipStream_t s) {
+  const sdeh_kargs base = pl.a;
+  const int64_t n = base.n, N = pl.n_state, P = pl.n_param;
+  const int64_t ns = base.n_saveat, ms = base.max_saves;
+  const size_t es = pl.es;
+  const bool tm = base.traj_major != 0;
+  const int K = kPipeChunks;
+  auto offs = [](const void* p, int64_t rows, int64_t a, size_t e) -> void* {
+    return p ? (void*)((char*)p + (size_t)rows * a * e) : nullptr;
+  };
+  auto down = [&](int k) -> int {
+    const int64_t a0 = n * k / K, m = n * (k + 1) / K - a0;
+    HIPCHK(c, hipStreamWaitEvent(c->s_out, c->ev_pipe[K + k], 0));
+    const hipStream_t so = c->s_out;
+    HIPCHK(c, chunk_copy((char*)base.u_end, nullptr, io->u_end, es, N, n, a0, m, false, so));
+    HIPCHK(c, chunk_copy((char*)base.t_end, nullptr, io->t_end, 8, 1, n, a0, m, false, so));
+    HIPCHK(c, chunk_copy((char*)base.retcode, nullptr, io->retcode, 4, 1, n, a0, m, false, so));
+    HIPCHK(c, chunk_copy((char*)base.naccept, nullptr, io->naccept, 4, 1, n, a0, m, false, so));
+    HIPCHK(c, chunk_copy((char*)base.nreject, nullptr, io->nreject, 4, 1, n, a0, m, false, so));
+    if (pl.save == SAVE_AT)
+      HIPCHK(c, chunk_copy((char*)base.u_saveat, nullptr, io->u_saveat, es, ns * N, n, a0, m, tm,
+                           so));
+    if (pl.save == SAVE_STEPS) {
+      HIPCHK(c, chunk_copy((char*)base.u_steps, nullptr, io->u_steps, es, ms * N, n, a0, m, tm,
+                           so));
+      HIPCHK(c, chunk_copy((char*)base.t_steps, nullptr, io->t_steps, 8, ms, n, a0, m, tm, so));
+      HIPCHK(c, chunk_copy((char*)base.n_steps, nullptr, io->n_steps, 4, 1, n, a0, m, false, so));
+    }
+    return SDEH_OK;
+  };
+  for (int k = 0; k < K; ++k) {
+    const int64_t a0 = n * k / K, m = n * (k + 1) / K - a0;
+    const hipStream_t si = c->s_in;
+    HIPCHK(c, chunk_copy((char*)base.u0, io->u0, nullptr, es, N, n, a0, m, false, si));
+    HIPCHK(c, chunk_copy((char*)base.p, io->p, nullptr, es, P, n, a0, m, false, si));
+    if (io->tspan)
+      HIPCHK(c, chunk_copy((char*)base.tspan, io->tspan, nullptr, 8, 2, n, a0, m, false, si));
+    if (io->dt0)
+      HIPCHK(c, chunk_copy((char*)base.dt0, io->dt0, nullptr, 8, 1, n, a0, m, false, si));
+    const hipStream_t sk = (k & 1) ? c->s_k2 : s;
+    HIPCHK(c, hipEventRecord(c->ev_pipe[k], si));
+    HIPCHK(c, hipStreamWaitEvent(sk, c->ev_pipe[k], 0));
+    Plan cp = pl;
+    sdeh_kargs& ka = cp.a;
+    ka.n = m;
+    if (cp.shape.schedule == SDEH_SCHED_ONE_LAUNCH)
+      cp.shape.grid = (int)((m * cp.k.lanes + cp.shape.block - 1) / cp.shape.block);
+    ka.u0 = offs(base.u0, N, a0, es);
+    ka.p = offs(base.p, P, a0, es);
+    ka.tspan = (const double*)offs(base.tspan, 2, a0, 8);
+    ka.dt0 = (const double*)offs(base.dt0, 1, a0, 8);
+    ka.u_end = offs(base.u_end, N, a0, es);
+    ka.t_end = (double*)offs(base.t_end, 1, a0, 8);
+    ka.retcode = (int32_t*)offs(base.retcode, 1, a0, 4);
+    ka.naccept = (int32_t*)offs(base.naccept, 1, a0, 4);
+    ka.nreject = (int32_t*)offs(base.nreject, 1, a0, 4);
+    ka.u_saveat = offs(base.u_saveat, ns * N, a0, es);
+    ka.u_steps = offs(base.u_steps, ms * N, a0, es);
+    ka.t_steps = (double*)offs(base.t_steps, ms, a0, 8);
+    ka.n_steps = (int32_t*)offs(base.n_steps, 1, a0, 4);
+    if (k == 0) HIPCHK(c, hipEventRecord(c->ev0, s));
+    int r = run_plan(c, cp, sk, c->counters + 8 + k, false);
+    if (r) return r;
+    HIPCHK(c, hipEventRecord(c->ev_pipe[K + k], sk));
+    // the previous chunk's outputs go back now: its kernel is done or nearly so,
+    // and this chunk's kernel is already queued behind it
+    if (k > 0) {
+      r = down(k - 1);
+      if (r) return r;
+    }
+  }
+  int r = down(K - 1);
+  if (r) return r;
+  for (int k = 1; k < K; k += 2) HIPCHK(c, hipStreamWaitEvent(s, c->ev_pipe[K + k], 0));
+  HIPCHK(c, hipEventRecord(c->ev1, s));
+  return SDEH_OK;
+}
+
 int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t memory,
                hipStream_t s, int32_t sync) {
   const auto t_start = std::chrono::steady_clock::now();
@@ -681,18 +801,22 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   double* d_saveat = (double*)ar.take(sz.saveat);
   if (sz.saveat) HIPCHK(c, hipMemcpyAsync(d_saveat, d->saveat, sz.saveat, hipMemcpyHostToDevice, s));
   a.saveat = d_saveat;
+  // host memory: the pipeline moves per-trajectory inputs and outputs chunk by
+  // chunk; shared inputs (p_shared, order) go up once
+  const bool piped = host && n >= kPipeMin && !io->order && !pl.packed;
   if (host) {
-    auto up = [&](const void* src, size_t b) -> void* {
+    auto up = [&](const void* src, size_t b, bool per_traj) -> void* {
       void* p = ar.take(b);
-      if (b && src) (void)hipMemcpyAsync(p, src, b, hipMemcpyHostToDevice, s);
+      if (b && src && !(piped && per_traj))
+        (void)hipMemcpyAsync(p, src, b, hipMemcpyHostToDevice, s);
       return p;
     };
-    a.u0 = up(io->u0, sz.u0);
-    a.p = up(io->p, sz.p);
-    a.ps = up(io->p_shared, sz.ps);
-    a.tspan = (const double*)up(io->tspan, sz.tspan);
-    a.dt0 = (const double*)up(io->dt0, sz.dt0);
-    a.order = (const int64_t*)up(io->order, sz.order);
+    a.u0 = up(io->u0, sz.u0, true);
+    a.p = up(io->p, sz.p, true);
+    a.ps = up(io->p_shared, sz.ps, false);
+    a.tspan = (const double*)up(io->tspan, sz.tspan, true);
+    a.dt0 = (const double*)up(io->dt0, sz.dt0, true);
+    a.order = (const int64_t*)up(io->order, sz.order, false);
     HIPCHK(c, hipGetLastError());
     a.u_end = ar.take(sz.u_end);
     a.t_end = (double*)ar.take(sz.t_end);
@@ -764,7 +888,12 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
     if (bad) return fail(c, SDEH_E_INVALID, "order is not a permutation of 0..n_traj-1");
   }
 
-  r = run_plan(c, pl, s, c->counters, true);
+  if (piped) {
+    // p_shared and the saveat times went up on `s`; the kernels queue behind them
+    r = run_host_pipeline(c, pl, io, s);
+  } else {
+    r = run_plan(c, pl, s, c->counters, true);
+  }
   if (r) return r;
   const bool do_sync = host || sync;
   note_pending(c, s, false);
@@ -774,7 +903,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
     HIPCHK(c, sdeh_util_count(a.naccept, a.nreject, a.retcode, n, c->counters + 1, s));
   }
   if (do_sync) HIPCHK(c, hipMemcpyAsync(cnt, c->counters + 1, sizeof cnt, hipMemcpyDeviceToHost, s));
-  if (host && n > 0) {
+  if (host && n > 0 && !piped) {
     auto down = [&](void* dst, const void* src, size_t b) {
       if (dst && b) (void)hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToHost, s);
     };
@@ -805,6 +934,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   c->stats.schedule = pl.shape.schedule;
   c->stats.lanes = pl.k.lanes;
   if (do_sync) {
+    if (piped) HIPCHK(c, hipStreamSynchronize(c->s_out));
     HIPCHK(c, hipStreamSynchronize(s));
     note_pending(c, s, true);
     float ms = 0.f;
@@ -861,11 +991,19 @@ int sdeh_create(sdeh_ctx** out, int device) {
   c->device = device;
   if (hipSetDevice(device) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->s_in, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->s_out, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->s_k2, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipMalloc((void**)&c->counters, 16 * sizeof(unsigned long long)) != hipSuccess) {
     sdeh_destroy(c);
     return SDEH_E_HIP;
   }
+  for (hipEvent_t& e : c->ev_pipe)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      sdeh_destroy(c);
+      return SDEH_E_HIP;
+    }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
   if (c->cus <= 0) c->cus = 256;
@@ -886,6 +1024,20 @@ void sdeh_destroy(sdeh_ctx* c) {
   if (c->counters) (void)hipFree(c->counters);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->ev_pipe)
+    if (e) (void)hipEventDestroy(e);
+  if (c->s_in) {
+    (void)hipStreamSynchronize(c->s_in);
+    (void)hipStreamDestroy(c->s_in);
+  }
+  if (c->s_out) {
+    (void)hipStreamSynchronize(c->s_out);
+    (void)hipStreamDestroy(c->s_out);
+  }
+  if (c->s_k2) {
+    (void)hipStreamSynchronize(c->s_k2);
+    (void)hipStreamDestroy(c->s_k2);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -1,0 +1,108 @@
+"""Host-memory solves of 2^17 trajectories and more run as a pipeline of
+chunks (inputs of one chunk travel while another integrates); device-memory
+solves run as one launch.  The per-trajectory results must be the same bits,
+for every output layout and input the pipeline moves."""
+import numpy as np
+import pytest
+
+from helpers import bits_equal
+from simplediffeq_hip import _abi, inputs, runtime
+
+pytestmark = pytest.mark.gpu
+
+N = (1 << 17) + 333  # not a multiple of the chunk count
+
+
+def _pair(ctx, d, n_state, u0, p, ps=None, tspan=None, dt0=None):
+    h = runtime.EnsembleBuffers(d, n_state, u0, p, ps, tspan, dt0)
+    sh = ctx.solve(d, h, allow=(_abi.E_DTMIN, _abi.E_SAVE_OVERFLOW))
+    st_h = ctx.stats()
+    g = runtime.EnsembleBuffers(d, n_state, u0, p, ps, tspan, dt0, backend="torch",
+                                device="cuda:0")
+    sg = ctx.solve(d, g, allow=(_abi.E_DTMIN, _abi.E_SAVE_OVERFLOW))
+    assert sh == sg
+    assert st_h["attempts"] == ctx.stats()["attempts"]
+    return h, g
+
+
+def _same(h, g, fields):
+    for f in fields:
+        assert bits_equal(h.numpy(f), g.numpy(f)), f
+
+
+BASE = ("u_end", "t_end", "retcode", "naccept", "nreject")
+
+
+@pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT])
+def test_endpoint_with_spans_and_dt0(ctx, sched):
+    u0, p, ts = inputs.lorenz_uneven(N, seed=8)
+    ts[1] = np.minimum(ts[1], 3.0)
+    dt0 = np.full(N, 0.05)
+    d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=N, dt=0.1,
+                          abstol=1e-6, reltol=1e-4, schedule=sched)
+    h, g = _pair(ctx, d, 3, u0, p, tspan=ts, dt0=dt0)
+    _same(h, g, BASE)
+    assert (h.t_end == ts[1]).all()
+
+
+@pytest.mark.parametrize("layout", [_abi.SAVE_POINT_MAJOR, _abi.SAVE_TRAJ_MAJOR],
+                         ids=["point-major", "traj-major"])
+def test_saveat_both_layouts(ctx, layout):
+    u0, p = inputs.lorenz_wide(N, seed=9)
+    d = runtime.make_desc(_abi.ALG_AVERN7, rhs=_abi.RHS_LORENZ, n_traj=N, tspan=(0.0, 1.0),
+                          dt=0.1, abstol=1e-7, reltol=1e-7, saveat=np.linspace(0.0, 1.0, 9),
+                          save_layout=layout)
+    h, g = _pair(ctx, d, 3, u0, p)
+    _same(h, g, BASE + ("u_saveat",))
+
+
+@pytest.mark.parametrize("alg", [_abi.ALG_TSIT5, _abi.ALG_ATSIT5])
+@pytest.mark.parametrize("layout", [_abi.SAVE_POINT_MAJOR, _abi.SAVE_TRAJ_MAJOR],
+                         ids=["point-major", "traj-major"])
+def test_everystep_fixed_rows(ctx, alg, layout):
+    u0, p = inputs.lorenz_wide(N, seed=10)
+    d = runtime.make_desc(alg, rhs=_abi.RHS_LORENZ, n_traj=N, tspan=(0.0, 0.5), dt=0.05,
+                          abstol=1e-6, reltol=1e-4, save_everystep=True, max_saves=40,
+                          save_layout=layout)
+    h, g = _pair(ctx, d, 3, u0, p)
+    _same(h, g, BASE + ("n_steps",))
+    k = h.n_steps
+    for i in range(0, N, 977):  # rows beyond n_steps are not written
+        if layout == _abi.SAVE_TRAJ_MAJOR:
+            a, b = h.u_steps[i, :k[i]], g.u_steps[i, :k[i]].cpu().numpy()
+        else:
+            a, b = h.u_steps[:k[i], :, i], g.u_steps[:k[i], :, i].cpu().numpy()
+        assert bits_equal(a, b), i
+
+
+def test_binary32_and_fixed_vern9(ctx):
+    u0, p = inputs.lorenz_wide(N, seed=11)
+    d = runtime.make_desc(_abi.ALG_ATSIT5, dtype=_abi.F32, rhs=_abi.RHS_LORENZ, n_traj=N,
+                          tspan=(0.0, 2.0), dt=0.1, abstol=1e-5, reltol=1e-4)
+    h, g = _pair(ctx, d, 3, u0, p)
+    _same(h, g, BASE)
+    d = runtime.make_desc(_abi.ALG_VERN9, rhs=_abi.RHS_LORENZ, n_traj=N, tspan=(0.0, 1.0),
+                          dt=0.05)
+    h, g = _pair(ctx, d, 3, u0, p)
+    _same(h, g, BASE)
+
+
+def test_glv16_shared_block_group_layout(ctx):
+    u0, r, A = inputs.glv16(N, seed=12)
+    d = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_GLV16, n_traj=N, tspan=(0.0, 2.0),
+                          dt=0.1, abstol=1e-8, reltol=1e-8, saveat=np.linspace(0.0, 2.0, 5),
+                          save_layout=_abi.SAVE_TRAJ_MAJOR)
+    h, g = _pair(ctx, d, 16, u0, r, A)
+    assert ctx.stats()["lanes"] == 16
+    _same(h, g, BASE + ("u_saveat",))
+
+
+def test_host_counts_optional(ctx):
+    u0, p = inputs.lorenz_wide(N, seed=13)
+    d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=N, tspan=(0.0, 1.0))
+    h = runtime.EnsembleBuffers(d, 3, u0, p)
+    h.io.naccept = h.io.nreject = None
+    ctx.solve(d, h)
+    g = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
+    ctx.solve(d, g)
+    _same(h, g, ("u_end", "t_end", "retcode"))
