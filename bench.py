@@ -161,7 +161,9 @@ def workload_small(args):
 def workload_uneven(args):
     """2,097,152 Lorenz trajectories, rho ~ U(0,60), t_end ~ LogUniform(0.5,20),
     u0 ~ N(0,1)^3 (seed 2), AVern7 binary64 at 1e-8/1e-8: one-launch vs persistent
-    vs autotuned, and K repeats from a HIP graph vs K solve calls."""
+    vs autotuned, and K repeats from a HIP graph vs K solve calls.  Trajectories are
+    handed out longest span first (the library default with per-trajectory spans);
+    the *_index_order entries time index order for comparison."""
     import torch
     from simplediffeq_hip import _abi, inputs, runtime
     n = args.n if args.n != N_TRAJ else 1 << 21
@@ -171,8 +173,11 @@ def workload_uneven(args):
     out = {"workload": "lorenz-uneven", "trajectories": n}
     base = dict(rhs=_abi.RHS_LORENZ, n_traj=n, tspan=(0.0, 1.0), dt=float(__import__("numpy").float32(0.1)),
                 abstol=1e-8, reltol=1e-8)
-    for sched_name, sched in (("one_launch", _abi.SCHED_ONE_LAUNCH), ("persistent", _abi.SCHED_PERSISTENT)):
-        d = runtime.make_desc(_abi.ALG_AVERN7, schedule=sched, **base)
+    for sched_name, sched, so in (("one_launch", _abi.SCHED_ONE_LAUNCH, 0),
+                                  ("persistent", _abi.SCHED_PERSISTENT, 0),
+                                  ("one_launch_index_order", _abi.SCHED_ONE_LAUNCH, 1),
+                                  ("persistent_index_order", _abi.SCHED_PERSISTENT, 1)):
+        d = runtime.make_desc(_abi.ALG_AVERN7, schedule=sched, span_order=so, **base)
         g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device="cuda:0")
         ctx.solve(d, g, sync=True)
         st = ctx.stats()

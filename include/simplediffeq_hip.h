@@ -168,7 +168,11 @@ typedef struct sdeh_solve_desc {
                                trajectory; or the group size itself.  Results are
                                identical. */
   int32_t save_layout;      /* sdeh_save_layout of u_saveat, u_steps and t_steps */
-  int32_t reserved[2];
+  int32_t span_order;       /* 0: with per-trajectory tspans (io->tspan) and no io->order,
+                               hand trajectories to lanes longest span first (fewer
+                               long trajectories start late; results are unchanged);
+                               1: index order */
+  int32_t reserved;
 } sdeh_solve_desc;
 
 /* Solve buffers.  With memory == SDEH_MEM_HOST these are host pointers and the

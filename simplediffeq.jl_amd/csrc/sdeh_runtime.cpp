@@ -44,6 +44,9 @@ SDEH_HIDDEN hipError_t sdeh_util_count(const int32_t*, const int32_t*, const int
                                        unsigned long long*, hipStream_t);
 SDEH_HIDDEN hipError_t sdeh_util_check_order(const int64_t*, int64_t, int32_t*, unsigned long long*,
                                              hipStream_t);
+SDEH_HIDDEN hipError_t sdeh_util_span_order(const double*, int64_t, unsigned*, int64_t*,
+                                            hipStream_t);
+SDEH_HIDDEN size_t sdeh_util_span_scratch_bytes();
 SDEH_HIDDEN hipError_t sdeh_util_pack_offsets(const int32_t*, int32_t, int64_t, int64_t*, int64_t,
                                               int64_t*, int32_t*, unsigned long long*,
                                               hipStream_t);
@@ -439,6 +442,11 @@ struct Plan {
   int64_t pk_cap = 0;
   int64_t* pk_off = nullptr;
   int64_t* pk_tiles = nullptr;
+  // longest-span-first hand-out order computed on the device (see span_order);
+  // the pipeline's chunks alternate between two histograms
+  bool span = false;
+  int64_t* span_ord = nullptr;
+  unsigned* span_hist[2] = {nullptr, nullptr};
   KernelRef k_count;
   KernelRef k;
   Shape shape;
@@ -516,6 +524,10 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
     pl->pk_cap = io->steps_capacity;
     a.packed = 1;
   }
+  if (d->span_order != 0 && d->span_order != 1)
+    return fail(c, SDEH_E_INVALID, "span_order %d out of range", d->span_order);
+  pl->span = d->span_order == 0 && adaptive && io->tspan && !io->order && d->n_traj > 1 &&
+             d->n_traj < (1ll << 32);
   if (!adaptive) {
     if (d->dt == 0.0) return fail(c, SDEH_E_INVALID, "range step cannot be zero");
     const sdeh_range::Grid g = sdeh_range::make(d->t0, d->dt, d->tf);
@@ -567,8 +579,8 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
 // Bytes of each io buffer for a plan.
 struct Sizes {
   size_t u0, p, ps, tspan, dt0, order, u_end, t_end, i32, u_saveat, u_steps, t_steps, saveat;
-  // packed layout: offsets output and scan scratch
-  size_t pk_off, pk_tiles;
+  // packed layout: offsets output and scan scratch; span order and its histogram
+  size_t pk_off, pk_tiles, span_order, span_hist;
 };
 
 Sizes sizes_of(const sdeh_solve_desc* d, const sdeh_io* io, const Plan& pl) {
@@ -587,6 +599,10 @@ Sizes sizes_of(const sdeh_solve_desc* d, const sdeh_io* io, const Plan& pl) {
   s.u_steps = pl.save == SAVE_STEPS ? (size_t)d->max_saves * pl.n_state * n * pl.es : 0;
   s.t_steps = pl.save == SAVE_STEPS ? (size_t)d->max_saves * n * 8 : 0;
   s.saveat = pl.save == SAVE_AT ? (size_t)d->n_saveat * 8 : 0;
+  if (pl.span) {
+    s.span_order = n * 8;
+    s.span_hist = sdeh_util_span_scratch_bytes();
+  }
   if (pl.packed) {
     const size_t cap = (size_t)pl.pk_cap;
     s.u_steps = cap * pl.n_state * pl.es;
@@ -716,6 +732,7 @@ int run_host_pipeline(sdeh_ctx* c, Plan& pl, const sdeh_io* io, hipStream_t s) {
     ka.n = m;
     if (cp.shape.schedule == SDEH_SCHED_ONE_LAUNCH)
       cp.shape.grid = (int)((m * cp.k.lanes + cp.shape.block - 1) / cp.shape.block);
+    ka.order = nullptr;
     ka.u0 = offs(base.u0, N, a0, es);
     ka.p = offs(base.p, P, a0, es);
     ka.tspan = (const double*)offs(base.tspan, 2, a0, 8);
@@ -729,6 +746,10 @@ int run_host_pipeline(sdeh_ctx* c, Plan& pl, const sdeh_io* io, hipStream_t s) {
     ka.u_steps = offs(base.u_steps, ms * N, a0, es);
     ka.t_steps = (double*)offs(base.t_steps, ms, a0, 8);
     ka.n_steps = (int32_t*)offs(base.n_steps, 1, a0, 4);
+    if (pl.span && m > 1) {
+      HIPCHK(c, sdeh_util_span_order(ka.tspan, m, pl.span_hist[k & 1], pl.span_ord + a0, sk));
+      ka.order = pl.span_ord + a0;
+    }
     if (k == 0) HIPCHK(c, hipEventRecord(c->ev0, s));
     int r = run_plan(c, cp, sk, c->counters + 8 + k, false);
     if (r) return r;
@@ -795,6 +816,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
     need += round256(sz.pk_tiles);
     if (host) need += round256(sz.pk_off);
   }
+  if (pl.span) need += round256(sz.span_order) + 2 * round256(sz.span_hist);
   r = ensure_arena(c, need);
   if (r) return r;
   Arena ar{c->arena};
@@ -875,6 +897,15 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
         if (r) return r;
         HIPCHK(c, hipMemsetAsync(io->steps_offset, 0, 8, s));
       }
+    }
+  }
+  if (pl.span) {  // longest span first (the pipeline orders each chunk itself)
+    pl.span_ord = (int64_t*)ar.take(sz.span_order);
+    pl.span_hist[0] = (unsigned*)ar.take(sz.span_hist);
+    pl.span_hist[1] = (unsigned*)ar.take(sz.span_hist);
+    if (!piped) {
+      HIPCHK(c, sdeh_util_span_order(a.tspan, n, pl.span_hist[0], pl.span_ord, s));
+      a.order = pl.span_ord;
     }
   }
 
@@ -1140,7 +1171,8 @@ int sdeh_graph_capture(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io,
   if (r) return r;
   if (n == 0) return fail(c, SDEH_E_INVALID, "empty ensemble");
   GraphRec g;
-  HIPCHK(c, hipMalloc(&g.mem, 256 + round256(sz.saveat)));
+  HIPCHK(c, hipMalloc(&g.mem, 256 + round256(sz.saveat) + round256(sz.span_order) +
+                                  round256(sz.span_hist)));
   double* d_saveat = (double*)((char*)g.mem + 256);
   if (sz.saveat) {
     HIPCHK(c, hipMemcpy(d_saveat, d->saveat, sz.saveat, hipMemcpyHostToDevice));
@@ -1164,7 +1196,18 @@ int sdeh_graph_capture(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io,
   a.n_steps = io->n_steps;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  r = run_plan(c, pl, c->stream, (unsigned long long*)g.mem, false);
+  if (pl.span) {  // the hand-out order is recomputed on every replay
+    char* q = (char*)g.mem + 256 + round256(sz.saveat);
+    pl.span_ord = (int64_t*)q;
+    pl.span_hist[0] = (unsigned*)(q + round256(sz.span_order));
+    a.order = pl.span_ord;
+  }
+  r = pl.span ? (sdeh_util_span_order(a.tspan, n, pl.span_hist[0], pl.span_ord, c->stream) ==
+                         hipSuccess
+                     ? SDEH_OK
+                     : fail(c, SDEH_E_HIP, "span order launch failed"))
+              : SDEH_OK;
+  if (!r) r = run_plan(c, pl, c->stream, (unsigned long long*)g.mem, false);
   hipGraph_t graph = nullptr;
   const hipError_t e = hipStreamEndCapture(c->stream, &graph);
   if (r) {

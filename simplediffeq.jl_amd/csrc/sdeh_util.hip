@@ -147,6 +147,85 @@ __global__ void write_i64(int64_t* p, int64_t v) {
   if (threadIdx.x == 0) *p = v;
 }
 
+// ---- longest span first ----------------------------------------------------
+// A hand-out order by decreasing tf - t0, counting-sorted on a bucket of the span:
+// 16 buckets per octave over 64 octaves (2^-30 .. 2^34; spans beyond clamp to the
+// end buckets; spans <= 0 and NaN in bucket 0, last).  Each of SPAN_BLOCKS blocks
+// takes a contiguous range of trajectories: span_hist counts its buckets in LDS,
+// span_scan turns the counts into each (bucket, block)'s first slot (buckets in
+// decreasing order, blocks in index order), and span_scatter places the block's
+// trajectories, ranking them with LDS atomics.  No global atomics: the few buckets
+// a narrow span distribution fills would serialise them.  Within a (bucket, block)
+// the order is whatever the LDS atomics produce; no result depends on it.
+constexpr int SPAN_BUCKETS = 1024;
+constexpr int SPAN_BLOCKS = 256;
+
+__device__ __forceinline__ int span_bucket(const double* tspan, int64_t n, int64_t i) {
+  const double span = tspan[n + i] - tspan[i];
+  if (!(span > 0.0)) return 0;  // also NaN
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(span);
+  int oct = (int)((bits >> 52) & 0x7ff) - (1023 - 30);
+  oct = oct < 0 ? 0 : (oct > 63 ? 63 : oct);
+  return oct * 16 + (int)((bits >> 48) & 15);
+}
+
+__device__ __forceinline__ void span_range(int64_t n, int64_t* lo, int64_t* hi) {
+  const int64_t per = (n + SPAN_BLOCKS - 1) / SPAN_BLOCKS;
+  *lo = (int64_t)blockIdx.x * per;
+  *hi = *lo + per < n ? *lo + per : n;
+}
+
+// bh[block][bucket] = trajectories of the block's range in the bucket.
+__global__ void __launch_bounds__(256) span_hist(const double* tspan, int64_t n, unsigned* bh) {
+  __shared__ unsigned h[SPAN_BUCKETS];
+  for (int k = threadIdx.x; k < SPAN_BUCKETS; k += blockDim.x) h[k] = 0u;
+  __syncthreads();
+  int64_t lo, hi;
+  span_range(n, &lo, &hi);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+    atomicAdd(h + span_bucket(tspan, n, i), 1u);
+  __syncthreads();
+  for (int k = threadIdx.x; k < SPAN_BUCKETS; k += blockDim.x)
+    bh[(size_t)blockIdx.x * SPAN_BUCKETS + k] = h[k];
+}
+
+// One block of SPAN_BUCKETS threads, thread t for bucket SPAN_BUCKETS-1-t: bh ->
+// first slot of each (block, bucket).
+__global__ void __launch_bounds__(1024) span_scan(unsigned* bh) {
+  __shared__ unsigned long long part[SPAN_BUCKETS];
+  const int t = threadIdx.x;
+  const int k = SPAN_BUCKETS - 1 - t;
+  unsigned long long tot = 0;
+  for (int b = 0; b < SPAN_BLOCKS; ++b) tot += bh[(size_t)b * SPAN_BUCKETS + k];
+  part[t] = tot;
+  __syncthreads();
+  for (int off = 1; off < SPAN_BUCKETS; off <<= 1) {
+    const unsigned long long x = t >= off ? part[t - off] : 0ull;
+    __syncthreads();
+    part[t] += x;
+    __syncthreads();
+  }
+  unsigned long long run = part[t] - tot;
+  for (int b = 0; b < SPAN_BLOCKS; ++b) {
+    const size_t at = (size_t)b * SPAN_BUCKETS + k;
+    const unsigned c = bh[at];
+    bh[at] = (unsigned)run;
+    run += c;
+  }
+}
+
+__global__ void __launch_bounds__(256) span_scatter(const double* tspan, int64_t n,
+                                                    const unsigned* bh, int64_t* order) {
+  __shared__ unsigned cur[SPAN_BUCKETS];
+  for (int k = threadIdx.x; k < SPAN_BUCKETS; k += blockDim.x)
+    cur[k] = bh[(size_t)blockIdx.x * SPAN_BUCKETS + k];
+  __syncthreads();
+  int64_t lo, hi;
+  span_range(n, &lo, &hi);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+    order[atomicAdd(cur + span_bucket(tspan, n, i), 1u)] = i;
+}
+
 inline int grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
   if (g > 4096) g = 4096;
@@ -213,4 +292,19 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_pack_offse
   hipLaunchKernelGGL(pk_offsets, dim3((unsigned)n_tiles), dim3(256), 0, s, base, add, n, tiles,
                      cap, off, lim);
   return hipGetLastError();
+}
+
+// order[0..n) = trajectories by decreasing span (see span_hist).  bh: scratch
+// of SPAN_BLOCKS x SPAN_BUCKETS uint32.  n must fit in 32 bits (the slot counters).
+extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_span_order(
+    const double* tspan, int64_t n, unsigned* bh, int64_t* order, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(span_hist, dim3(SPAN_BLOCKS), dim3(256), 0, s, tspan, n, bh);
+  hipLaunchKernelGGL(span_scan, dim3(1), dim3(SPAN_BUCKETS), 0, s, bh);
+  hipLaunchKernelGGL(span_scatter, dim3(SPAN_BLOCKS), dim3(256), 0, s, tspan, n, bh, order);
+  return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) size_t sdeh_util_span_scratch_bytes() {
+  return (size_t)SPAN_BLOCKS * SPAN_BUCKETS * sizeof(unsigned);
 }

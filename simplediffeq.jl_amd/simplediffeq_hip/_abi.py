@@ -76,7 +76,8 @@ class SolveDesc(ctypes.Structure):
         ("chunk", ctypes.c_int32),
         ("lanes", ctypes.c_int32),
         ("save_layout", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("span_order", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

@@ -24,7 +24,8 @@ class SolveError(RuntimeError):
 def make_desc(alg, dtype=_abi.F64, rhs=_abi.RHS_LORENZ, n_traj=0, tspan=(0.0, 1.0), dt=None,
               abstol=None, reltol=None, saveat=None, save_everystep=False, max_saves=0,
               maxiters=0, quirks=True, schedule=_abi.SCHED_AUTO, precision=_abi.PREC_PLAIN,
-              block=0, grid=0, chunk=0, lanes=0, save_layout=_abi.SAVE_POINT_MAJOR):
+              block=0, grid=0, chunk=0, lanes=0, save_layout=_abi.SAVE_POINT_MAJOR,
+              span_order=0):
     """Build an sdeh_solve_desc.  Unset dt/abstol/reltol take the reference
     defaults (0.1f0, 1f-6, 1f-3).  lanes: lanes per trajectory (0 = the
     right-hand side's group layout where it has one).  save_layout: the layout of
@@ -52,6 +53,7 @@ def make_desc(alg, dtype=_abi.F64, rhs=_abi.RHS_LORENZ, n_traj=0, tspan=(0.0, 1.
     d.block, d.grid, d.chunk = int(block), int(grid), int(chunk)
     d.lanes = int(lanes)
     d.save_layout = int(save_layout)
+    d.span_order = int(span_order)
     if saveat is not None and len(saveat) > 0:
         arr = np.ascontiguousarray(np.asarray(saveat, dtype=np.float64))
         d._saveat_keep = arr
