@@ -348,9 +348,12 @@ def workload_costmodel(args):
     dt0 = cm.dt0(pred)
     out["predict_and_sort_s"] = time.perf_counter() - t0
 
-    def run(dtype, sched, precision, order_=None, dt0_=None):
+    def run(dtype, sched, precision, order_=None, dt0_=None, span=False):
+        # "unsorted" runs hand trajectories out in index order (span_order 1); with
+        # span=True the library's longest-span-first order applies (no model)
         dd = runtime.make_desc(_abi.ALG_ATSIT5, dtype=dtype, rhs=_abi.RHS_LORENZ, n_traj=n_eval,
-                               dt=f32_01, schedule=sched, precision=precision, **tol)
+                               dt=f32_01, schedule=sched, precision=precision,
+                               span_order=0 if span else 1, **tol)
         gg = runtime.EnsembleBuffers(dd, 3, u0, p, tspan=ts, dt0=dt0_, order=order_,
                                      backend="torch", device=dev)
         ctx.solve(dd, gg, sync=True)
@@ -366,14 +369,16 @@ def workload_costmodel(args):
         dc = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=k, dt=f32_01, **tol)
         out["cpu_baseline"] = cpu_leg(dc, 3, u0[:, :k], p[:, :k], tspan=ts[:, :k])
     rec = {}
-    for name, dtype, sched, prec, od, dz in (
-            ("f64_persistent_unsorted", _abi.F64, _abi.SCHED_PERSISTENT, _abi.PREC_PLAIN, None, None),
-            ("f64_persistent_binned", _abi.F64, _abi.SCHED_PERSISTENT, _abi.PREC_PLAIN, order, None),
-            ("f32k_persistent_unsorted", _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN, None, None),
-            ("f32k_one_launch_binned", _abi.F32, _abi.SCHED_ONE_LAUNCH, _abi.PREC_KAHAN, order, None),
-            ("f32k_persistent_binned", _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN, order, None),
-            ("f32k_persistent_binned_dt0", _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN, order, dt0)):
-        gg, ms, st = run(dtype, sched, prec, od, dz)
+    for name, dtype, sched, prec, od, dz, sp in (
+            ("f64_persistent_unsorted", _abi.F64, _abi.SCHED_PERSISTENT, _abi.PREC_PLAIN, None, None, False),
+            ("f64_one_launch_span_order", _abi.F64, _abi.SCHED_ONE_LAUNCH, _abi.PREC_PLAIN, None, None, True),
+            ("f32k_one_launch_span_order", _abi.F32, _abi.SCHED_ONE_LAUNCH, _abi.PREC_KAHAN, None, None, True),
+            ("f64_persistent_binned", _abi.F64, _abi.SCHED_PERSISTENT, _abi.PREC_PLAIN, order, None, False),
+            ("f32k_persistent_unsorted", _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN, None, None, False),
+            ("f32k_one_launch_binned", _abi.F32, _abi.SCHED_ONE_LAUNCH, _abi.PREC_KAHAN, order, None, False),
+            ("f32k_persistent_binned", _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN, order, None, False),
+            ("f32k_persistent_binned_dt0", _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN, order, dt0, False)):
+        gg, ms, st = run(dtype, sched, prec, od, dz, sp)
         rec[name] = gg
         out[name] = {"kernel_ms": ms, "traj_per_s": n_eval / (ms * 1e-3), "attempts": st["attempts"],
                      "speedup_vs_f64_one_launch": ms_base / ms}

@@ -195,8 +195,16 @@ __global__ void __launch_bounds__(1024) span_scan(unsigned* bh) {
   __shared__ unsigned long long part[SPAN_BUCKETS];
   const int t = threadIdx.x;
   const int k = SPAN_BUCKETS - 1 - t;
+  // eight loads in flight per step (the counts of one bucket are a column of bh)
+  static_assert(SPAN_BLOCKS % 8 == 0, "blocks in steps of eight");
   unsigned long long tot = 0;
-  for (int b = 0; b < SPAN_BLOCKS; ++b) tot += bh[(size_t)b * SPAN_BUCKETS + k];
+  for (int b = 0; b < SPAN_BLOCKS; b += 8) {
+    unsigned v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = bh[(size_t)(b + j) * SPAN_BUCKETS + k];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) tot += v[j];
+  }
   part[t] = tot;
   __syncthreads();
   for (int off = 1; off < SPAN_BUCKETS; off <<= 1) {
@@ -206,11 +214,15 @@ __global__ void __launch_bounds__(1024) span_scan(unsigned* bh) {
     __syncthreads();
   }
   unsigned long long run = part[t] - tot;
-  for (int b = 0; b < SPAN_BLOCKS; ++b) {
-    const size_t at = (size_t)b * SPAN_BUCKETS + k;
-    const unsigned c = bh[at];
-    bh[at] = (unsigned)run;
-    run += c;
+  for (int b = 0; b < SPAN_BLOCKS; b += 8) {
+    unsigned v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = bh[(size_t)(b + j) * SPAN_BUCKETS + k];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      bh[(size_t)(b + j) * SPAN_BUCKETS + k] = (unsigned)run;
+      run += v[j];
+    }
   }
 }
 
