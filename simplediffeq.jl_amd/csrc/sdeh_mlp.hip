@@ -55,19 +55,91 @@ __device__ inline void forward_one(const float* sw, const float (&x)[NIN], float
   }
 }
 
-__global__ void __launch_bounds__(256) mlp_forward(const float* __restrict__ params,
-                                                   const float* __restrict__ x, float* y,
-                                                   int64_t n) {
+// ---- inference on the matrix cores ------------------------------------------
+// Each wave takes tiles of 32 samples.  Layer 1 (6 -> 64) is vector work: lane
+// (row i = lane % 32, parity q = lane / 32) computes the 32 hidden units k = 2m + q
+// of sample i, exactly the A operand v_mfma_f32_32x32x2f32 wants (A[i][k] with
+// k = lane / 32 for K-step m).  Layer 2 (64 x 64) is 2 x 32 MFMAs per tile: D tile t
+// (units 32t .. 32t+31) = sum over m of A_m x B_m with B_m[k][j] = W2[32t + j][2m + k],
+// held in 64 registers per lane.  D register r of lane l holds sample row
+// 8(r/4) + 4(l/32) + r%4 and unit l % 32 (+32 t); after tanh the tile goes through
+// LDS (rows padded to 65 floats) so that layer 3 (64 -> 2) reads each sample's row.
+// Inference uses a fast tanh (1 - 2/(e^{2x} + 1) with the hardware exp2 and
+// reciprocal; absolute error about 1e-7); training keeps tanhf.
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+constexpr int H2P = NH + 1;
+
+__device__ __forceinline__ float tanh_fast(float x) {
+  const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);  // e^{2x}
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(256) mlp_forward_mfma(const float* __restrict__ params,
+                                                        const float* __restrict__ x, float* y,
+                                                        int64_t n) {
   __shared__ float sw[NPARAM];
+  __shared__ float s_h2[4][32 * H2P];
   load_params(sw, params);
-  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n;
-       s += (int64_t)gridDim.x * blockDim.x) {
-    float xi[NIN], h1[NH], h2[NH], out[NOUT];
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = l & 31, q = l >> 5;
+  float wb[2][32];
 #pragma unroll
-    for (int j = 0; j < NIN; ++j) xi[j] = x[s * NIN + j];
-    forward_one(sw, xi, h1, h2, out);
+  for (int t = 0; t < 2; ++t)
 #pragma unroll
-    for (int o = 0; o < NOUT; ++o) y[s * NOUT + o] = out[o];
+    for (int m = 0; m < 32; ++m) wb[t][m] = sw[OFF_W2 + (32 * t + li) * NH + 2 * m + q];
+  const float b2a = sw[OFF_B2 + li], b2b = sw[OFF_B2 + 32 + li];
+  float* hs = s_h2[wv];
+  const int64_t tiles = (n + 31) / 32;
+  for (int64_t tile = (int64_t)blockIdx.x * 4 + wv; tile < tiles;
+       tile += (int64_t)gridDim.x * 4) {
+    const int64_t smp = tile * 32 + li;
+    const bool valid = smp < n;
+    float xi[NIN];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) xi[j] = valid ? x[smp * NIN + j] : 0.f;
+    float h1[32];
+#pragma unroll
+    for (int m = 0; m < 32; ++m) {
+      const int k = 2 * m + q;
+      float z = sw[OFF_B1 + k];
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) z = fmaf(sw[OFF_W1 + k * NIN + j], xi[j], z);
+      h1[m] = tanh_fast(z);
+    }
+    floatx16 acc0 = {}, acc1 = {};
+#pragma unroll
+    for (int m = 0; m < 32; ++m) {
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(h1[m], wb[0][m], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(h1[m], wb[1][m], acc1, 0, 0, 0);
+    }
+    wave_sync_lds();  // the previous tile's reads of hs are done
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = 8 * (r >> 2) + 4 * q + (r & 3);
+      hs[row * H2P + li] = tanh_fast(acc0[r] + b2a);
+      hs[row * H2P + 32 + li] = tanh_fast(acc1[r] + b2b);
+    }
+    wave_sync_lds();
+    float o0 = 0.f, o1 = 0.f;
+#pragma unroll
+    for (int m = 0; m < 32; ++m) {
+      const int k = 2 * m + q;
+      const float h = hs[li * H2P + k];
+      o0 = fmaf(sw[OFF_W3 + k], h, o0);
+      o1 = fmaf(sw[OFF_W3 + NH + k], h, o1);
+    }
+    o0 += __shfl_xor(o0, 32);
+    o1 += __shfl_xor(o1, 32);
+    if (q == 0 && valid) {
+      y[smp * NOUT] = o0 + sw[OFF_B3];
+      y[smp * NOUT + 1] = o1 + sw[OFF_B3 + 1];
+    }
   }
 }
 
@@ -170,13 +242,26 @@ __global__ void __launch_bounds__(64) mlp_grad_partial(const float* __restrict__
   if (i == 0) pb[NPARAM] = gl;
 }
 
-// Sum the block partials in block order (deterministic).
+// Sum the block partials in a fixed order (deterministic): accumulator r takes
+// blocks r, r + 16, r + 32, ... in turn, then the 16 accumulators are added in
+// order.  Sixteen independent loads per step keep the memory pipeline busy.
 __global__ void mlp_grad_reduce(const float* __restrict__ partial, int nblocks, float* grad,
                                 float* loss, float inv_count) {
+  constexpr int R = 16;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j > NPARAM) return;
-  float s = 0.f;
-  for (int b = 0; b < nblocks; ++b) s += partial[(int64_t)b * (NPARAM + 1) + j];
+  float acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.f;
+  int b = 0;
+  for (; b + R <= nblocks; b += R) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] += partial[(int64_t)(b + r) * (NPARAM + 1) + j];
+  }
+  for (int r = 0; b + r < nblocks; ++r) acc[r] += partial[(int64_t)(b + r) * (NPARAM + 1) + j];
+  float s = acc[0];
+#pragma unroll
+  for (int r = 1; r < R; ++r) s += acc[r];
   if (j < NPARAM) grad[j] = s;
   else *loss = s * inv_count;
 }
@@ -200,9 +285,9 @@ __global__ void mlp_adam(float* p, const float* __restrict__ g, float* m, float*
 extern "C" SDEH_HIDDEN int sdeh_mlp_forward_impl(const float* params, const float* x, float* y,
                                                  int64_t n, hipStream_t s) {
   if (n == 0) return 0;
-  int64_t g = (n + 255) / 256;
-  if (g > 2048) g = 2048;
-  hipLaunchKernelGGL(mlp_forward, dim3((unsigned)g), dim3(256), 0, s, params, x, y, n);
+  int64_t g = ((n + 31) / 32 + 3) / 4;  // four waves of 32-sample tiles per block
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(mlp_forward_mfma, dim3((unsigned)g), dim3(256), 0, s, params, x, y, n);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -47,6 +47,22 @@ def features(p, u0, tspan):
     return x
 
 
+def features_device(p, u0, tspan):
+    """features() for device-resident ensemble inputs (torch tensors p (3, n),
+    u0 (3, n), tspan (2, n), binary64): the same six scaled features as an (n, 6)
+    float32 tensor, computed on the device (no host round trip)."""
+    import torch
+    n = p.shape[1]
+    x = torch.empty((n, NIN), dtype=torch.float32, device=p.device)
+    x[:, 0] = (p[0] - 10.0) / 2.0
+    x[:, 1] = (p[1] - 14.0) / 14.0
+    x[:, 2] = (p[2] - 2.5) / 0.5
+    x[:, 3] = (torch.log(tspan[1] - tspan[0]) - _LOG_T_MID) / _LOG_T_HALF
+    x[:, 4] = torch.sqrt((u0 ** 2).sum(dim=0)) / math.sqrt(3.0) * 2.0 - 1.0
+    x[:, 5] = u0[2]
+    return x
+
+
 def targets(naccept, tspan):
     n = np.maximum(naccept.astype(np.float64), 1.0)
     y = np.empty((naccept.shape[0], NOUT), dtype=np.float32)
@@ -141,6 +157,24 @@ class CostModel:
 
     def dt0(self, pred):
         return np.exp(pred[:, 1]).astype(np.float64)
+
+    def plan_device(self, xd, stream):
+        """The whole binning step on the device, ordered on the torch stream
+        `stream`: predictions for the (n, 6) float32 features `xd` (the library's
+        forward kernel), the hand-out order (stable sort by decreasing predicted
+        cost) and the seeded dt0.  Returns (pred (n, 2), order int64, dt0 float64),
+        all device tensors; nothing travels to the host."""
+        torch = self.torch
+        n = xd.shape[0]
+        with torch.cuda.stream(stream):
+            yd = torch.empty((n, NOUT), dtype=torch.float32, device=self.dev)
+            self.ctx.mlp_forward(self.params.data_ptr(), xd.data_ptr(), yd.data_ptr(), n,
+                                 stream=stream.cuda_stream)
+            pred = yd * torch.as_tensor(self.y_std, device=self.dev) + \
+                torch.as_tensor(self.y_mean, device=self.dev)
+            order = torch.argsort(-pred[:, 0], stable=True)
+            dt0 = torch.exp(pred[:, 1]).to(torch.float64)
+        return pred, order, dt0
 
 
 def wave_efficiency(attempts, order=None, wave=64):

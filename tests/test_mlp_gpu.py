@@ -21,6 +21,7 @@ def test_forward_loss_grad_adam_match_reference(ctx):
     xd, yd = cm._dev(x), cm._dev(y)
     out = torch.empty((n, 2), dtype=torch.float32, device=dev)
     ctx.mlp_forward(cm.params.data_ptr(), xd.data_ptr(), out.data_ptr(), n)
+    torch.cuda.synchronize()  # the library's kernels run on its own stream
     ref, _ = mlp_ref.forward(w.astype(np.float64), x.astype(np.float64))
     assert np.max(np.abs(out.cpu().numpy() - ref)) < 2e-5
     loss, grad = cm.loss_grad(xd, yd)
