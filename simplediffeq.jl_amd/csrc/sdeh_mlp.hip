@@ -3,8 +3,8 @@
 // parameters, initial state and horizon, the log of its accepted-step count
 // and the log of a good initial dt; the runtime's caller uses the predictions
 // to bin trajectories into wavefronts of similar cost and to seed dt0.  The
-// network has 4738 parameters, so every pass is tiny next to one ensemble
-// solve; the kernels favour determinism (fixed reduction order) over speed.
+// network has 4738 parameters; its two 64 x 64 products (and the backward pass's)
+// run on the matrix cores, and every reduction has a fixed order (deterministic).
 //
 // Parameter layout (FP32): W1[64][6], b1[64], W2[64][64], b2[64], W3[2][64], b3[2].
 #include <hip/hip_runtime.h>
@@ -27,32 +27,6 @@ static_assert(NPARAM == 4738, "parameter count");
 __device__ inline void load_params(float* sw, const float* __restrict__ params) {
   for (int i = threadIdx.x; i < NPARAM; i += blockDim.x) sw[i] = params[i];
   __syncthreads();
-}
-
-// Forward pass of one sample from LDS weights.
-__device__ inline void forward_one(const float* sw, const float (&x)[NIN], float (&h1)[NH],
-                                   float (&h2)[NH], float (&out)[NOUT]) {
-#pragma unroll 4
-  for (int i = 0; i < NH; ++i) {
-    float z = sw[OFF_B1 + i];
-#pragma unroll
-    for (int j = 0; j < NIN; ++j) z = fmaf(sw[OFF_W1 + i * NIN + j], x[j], z);
-    h1[i] = tanhf(z);
-  }
-#pragma unroll 2
-  for (int i = 0; i < NH; ++i) {
-    float z = sw[OFF_B2 + i];
-#pragma unroll
-    for (int k = 0; k < NH; ++k) z = fmaf(sw[OFF_W2 + i * NH + k], h1[k], z);
-    h2[i] = tanhf(z);
-  }
-#pragma unroll
-  for (int o = 0; o < NOUT; ++o) {
-    float z = sw[OFF_B3 + o];
-#pragma unroll
-    for (int k = 0; k < NH; ++k) z = fmaf(sw[OFF_W3 + o * NH + k], h2[k], z);
-    out[o] = z;
-  }
 }
 
 // ---- inference on the matrix cores ------------------------------------------
@@ -143,103 +117,210 @@ __global__ void __launch_bounds__(256) mlp_forward_mfma(const float* __restrict_
   }
 }
 
-// One wave per block.  Each pass takes 64 samples (one per lane), runs the
-// forward and backward pass, stages the per-sample vectors in LDS, and lane i
-// accumulates the gradient entries of hidden unit i over the 64 samples.
-// Partial sums per block go to `partial` [gridDim.x][NPARAM + 1] (last = loss).
-constexpr int LDP = NH + 1;  // padded row stride
-__global__ void __launch_bounds__(64) mlp_grad_partial(const float* __restrict__ params,
-                                                       const float* __restrict__ x,
-                                                       const float* __restrict__ y, int64_t n,
-                                                       float* partial) {
-  __shared__ float sw[NPARAM];
-  __shared__ float s_h1[64 * LDP], s_h2[64 * LDP], s_d1[64 * LDP], s_d2[64 * LDP];
-  __shared__ float s_x[64 * NIN], s_g[64 * NOUT], s_l[64];
-  load_params(sw, params);
-  const int lane = threadIdx.x;
-  const float inv = 1.0f / (float)(n * NOUT);  // loss = mean over n * NOUT outputs
-  float gW2[NH], gW1[NIN], gb1 = 0.f, gb2 = 0.f, gW3[NOUT], gb3 = 0.f, gl = 0.f;
-#pragma unroll
-  for (int k = 0; k < NH; ++k) gW2[k] = 0.f;
-#pragma unroll
-  for (int j = 0; j < NIN; ++j) gW1[j] = 0.f;
-#pragma unroll
-  for (int o = 0; o < NOUT; ++o) gW3[o] = 0.f;
+// ---- training pass on the matrix cores ---------------------------------------
+// One wave per block; each wave takes 32-sample tiles and accumulates the gradient
+// of its tiles.  The three 64 x 64 products run on v_mfma_f32_32x32x2f32 (layouts as
+// in mlp_forward_mfma; A[i][k]: lane i + 32 (k % 2), B[k][j]: lane j + 32 (k % 2),
+// D register r of lane l: row 8(r/4) + 4(l/32) + r%4, column l % 32):
+//   forward  Z2[s][j] = sum_k H1[s][k] W2[j][k]      (M = samples, K = units)
+//   backward D1[s][k] = sum_j D2[s][j] W2[j][k]      (M = samples, K = units)
+//   gradient gW2[j][k] += sum_s D2[s][j] H1[s][k]    (M = units, K = samples)
+// H1, H2, D2 and the output gradients of a tile pass through the wave's LDS tile
+// to change layout.  Layers 1 and 3, the biases, the loss and the tanh derivative
+// are vector work.  The wave's gradient (and squared-error sum) goes to
+// partial[blockIdx.x]; mlp_grad_reduce adds the waves in a fixed order.
+struct GradTile {
+  float x[32 * NIN];
+  float h1[32 * H2P];
+  float h2[32 * H2P];
+  float d2[32 * H2P];
+  float g[32 * NOUT];
+};
 
-  for (int64_t base = (int64_t)blockIdx.x * 64; base < n; base += (int64_t)gridDim.x * 64) {
-    const int64_t s = base + lane;
-    const bool valid = s < n;
-    float xi[NIN], h1[NH], h2[NH], out[NOUT], g[NOUT];
+__global__ void __launch_bounds__(64) mlp_grad_mfma(const float* __restrict__ params,
+                                                    const float* __restrict__ x,
+                                                    const float* __restrict__ y, int64_t n,
+                                                    float* partial) {
+  __shared__ float sw[NPARAM];
+  __shared__ GradTile L;
+  load_params(sw, params);
+  const int l = threadIdx.x, li = l & 31, q = l >> 5;
+  const float inv = 1.0f / (float)(n * NOUT);
+  float wb[2][32];
 #pragma unroll
-    for (int j = 0; j < NIN; ++j) xi[j] = valid ? x[s * NIN + j] : 0.f;
-    forward_one(sw, xi, h1, h2, out);
-    float l = 0.f;
+  for (int t = 0; t < 2; ++t)
 #pragma unroll
-    for (int o = 0; o < NOUT; ++o) {
-      const float e = valid ? out[o] - y[s * NOUT + o] : 0.f;
-      l = fmaf(e, e, l);
-      g[o] = 2.f * e * inv;
+    for (int m = 0; m < 32; ++m) wb[t][m] = sw[OFF_W2 + (32 * t + li) * NH + 2 * m + q];
+  const float b2v[2] = {sw[OFF_B2 + li], sw[OFF_B2 + 32 + li]};
+  const float w3c[2][2] = {{sw[OFF_W3 + li], sw[OFF_W3 + 32 + li]},
+                           {sw[OFF_W3 + NH + li], sw[OFF_W3 + NH + 32 + li]}};
+  floatx16 gw2[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) gw2[a][b] = floatx16{};
+  float gw1[2][NIN], gb1[2] = {0.f, 0.f}, gb2[2] = {0.f, 0.f}, gw3[NOUT][2], gb3[NOUT] = {0.f, 0.f};
+  float gl = 0.f;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) gw1[t][j] = 0.f;
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) gw3[o][t] = 0.f;
+  }
+  const int64_t tiles = (n + 31) / 32;
+  for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const int64_t smp = tile * 32 + li;
+    const bool valid = smp < n;
+    float xi[NIN];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) xi[j] = valid ? x[smp * NIN + j] : 0.f;
+    wave_sync_lds();  // the previous tile's LDS reads are done
+    if (q == 0)
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) L.x[li * NIN + j] = xi[j];
+    // layer 1 in the A layout: lane (li, q) holds units k = 2m + q of sample li
+    float h1a[32];
+#pragma unroll
+    for (int m = 0; m < 32; ++m) {
+      const int k = 2 * m + q;
+      float z = sw[OFF_B1 + k];
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) z = fmaf(sw[OFF_W1 + k * NIN + j], xi[j], z);
+      h1a[m] = tanhf(z);
+      L.h1[li * H2P + k] = h1a[m];
     }
-    // dL/dz2 = (W3^T g) * (1 - h2^2)
-    float d2[NH];
+    floatx16 z0 = {}, z1 = {};
 #pragma unroll
-    for (int k = 0; k < NH; ++k) {
-      float a = 0.f;
-#pragma unroll
-      for (int o = 0; o < NOUT; ++o) a = fmaf(sw[OFF_W3 + o * NH + k], g[o], a);
-      d2[k] = a * (1.f - h2[k] * h2[k]);
-    }
-    // dL/dz1 = (W2^T d2) * (1 - h1^2)
-#pragma unroll 2
-    for (int k = 0; k < NH; ++k) {
-      float a = 0.f;
-#pragma unroll
-      for (int i = 0; i < NH; ++i) a = fmaf(sw[OFF_W2 + i * NH + k], d2[i], a);
-      s_d1[lane * LDP + k] = a * (1.f - h1[k] * h1[k]);
+    for (int m = 0; m < 32; ++m) {
+      z0 = __builtin_amdgcn_mfma_f32_32x32x2f32(h1a[m], wb[0][m], z0, 0, 0, 0);
+      z1 = __builtin_amdgcn_mfma_f32_32x32x2f32(h1a[m], wb[1][m], z1, 0, 0, 0);
     }
 #pragma unroll
-    for (int k = 0; k < NH; ++k) {
-      s_h1[lane * LDP + k] = h1[k];
-      s_h2[lane * LDP + k] = h2[k];
-      s_d2[lane * LDP + k] = d2[k];
+    for (int r = 0; r < 16; ++r) {
+      const int row = 8 * (r >> 2) + 4 * q + (r & 3);
+      L.h2[row * H2P + li] = tanhf(z0[r] + b2v[0]);
+      L.h2[row * H2P + 32 + li] = tanhf(z1[r] + b2v[1]);
+    }
+    wave_sync_lds();
+    // layer 3, error and output gradient of sample li (the two halves add up)
+    float o0 = 0.f, o1 = 0.f;
+#pragma unroll
+    for (int m = 0; m < 32; ++m) {
+      const int k = 2 * m + q;
+      const float h = L.h2[li * H2P + k];
+      o0 = fmaf(sw[OFF_W3 + k], h, o0);
+      o1 = fmaf(sw[OFF_W3 + NH + k], h, o1);
+    }
+    o0 += __shfl_xor(o0, 32);
+    o1 += __shfl_xor(o1, 32);
+    const float e0 = valid ? o0 + sw[OFF_B3] - y[smp * NOUT] : 0.f;
+    const float e1 = valid ? o1 + sw[OFF_B3 + 1] - y[smp * NOUT + 1] : 0.f;
+    if (q == 0) {
+      const float g0 = 2.f * e0 * inv, g1 = 2.f * e1 * inv;
+      L.g[li * NOUT] = g0;
+      L.g[li * NOUT + 1] = g1;
+      gb3[0] += g0;
+      gb3[1] += g1;
+      gl = fmaf(e0, e0, gl);
+      gl = fmaf(e1, e1, gl);
+    }
+    wave_sync_lds();
+    // D2 = (W3^T g) (1 - H2^2) in the D layout (rows: samples, column: unit)
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r >> 2) + 4 * q + (r & 3);
+        const float g0 = L.g[row * NOUT], g1 = L.g[row * NOUT + 1];
+        const float h = L.h2[row * H2P + 32 * t + li];
+        const float d = fmaf(w3c[0][t], g0, w3c[1][t] * g1) * (1.f - h * h);
+        L.d2[row * H2P + 32 * t + li] = d;
+        gb2[t] += d;
+        gw3[0][t] = fmaf(g0, h, gw3[0][t]);
+        gw3[1][t] = fmaf(g1, h, gw3[1][t]);
+      }
+    wave_sync_lds();
+    // D1 = D2 W2 on the matrix cores, then times (1 - H1^2)
+    floatx16 d0 = {}, d1 = {};
+#pragma unroll
+    for (int m = 0; m < 32; ++m) {
+      const float a = L.d2[li * H2P + 2 * m + q];
+      d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, sw[OFF_W2 + (2 * m + q) * NH + li], d0, 0, 0,
+                                                0);
+      d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, sw[OFF_W2 + (2 * m + q) * NH + 32 + li], d1,
+                                                0, 0, 0);
     }
 #pragma unroll
-    for (int j = 0; j < NIN; ++j) s_x[lane * NIN + j] = xi[j];
+    for (int r = 0; r < 16; ++r) {
+      const int row = 8 * (r >> 2) + 4 * q + (r & 3);
 #pragma unroll
-    for (int o = 0; o < NOUT; ++o) s_g[lane * NOUT + o] = g[o];
-    s_l[lane] = l;
-    __syncthreads();
-    // lane i owns hidden unit i: rows W2[i][:], W1[i][:], b1[i], b2[i], column W3[:][i]
-    const int i = lane;
-    for (int t = 0; t < 64; ++t) {
-      const float d2ti = s_d2[t * LDP + i];
-      const float d1ti = s_d1[t * LDP + i];
+      for (int t = 0; t < 2; ++t) {
+        const float h = L.h1[row * H2P + 32 * t + li];
+        const float dd = (t == 0 ? d0[r] : d1[r]) * (1.f - h * h);
+        gb1[t] += dd;
 #pragma unroll
-      for (int k = 0; k < NH; ++k) gW2[k] = fmaf(d2ti, s_h1[t * LDP + k], gW2[k]);
-#pragma unroll
-      for (int j = 0; j < NIN; ++j) gW1[j] = fmaf(d1ti, s_x[t * NIN + j], gW1[j]);
-      gb1 += d1ti;
-      gb2 += d2ti;
-      const float h2ti = s_h2[t * LDP + i];
-#pragma unroll
-      for (int o = 0; o < NOUT; ++o) gW3[o] = fmaf(s_g[t * NOUT + o], h2ti, gW3[o]);
-      if (i < NOUT) gb3 += s_g[t * NOUT + i];
-      if (i == 0) gl += s_l[t];
+        for (int j = 0; j < NIN; ++j) gw1[t][j] = fmaf(dd, L.x[row * NIN + j], gw1[t][j]);
+      }
     }
-    __syncthreads();
+    // gW2 += D2^T H1 on the matrix cores (K = the tile's samples)
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int srow = 2 * m + q;
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj) {
+        const float a = L.d2[srow * H2P + 32 * tj + li];
+#pragma unroll
+        for (int tk = 0; tk < 2; ++tk)
+          gw2[tj][tk] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+              a, L.h1[srow * H2P + 32 * tk + li], gw2[tj][tk], 0, 0, 0);
+      }
+    }
+  }
+  // the two halves of the wave hold different sample rows of the same units
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    gb1[t] += __shfl_xor(gb1[t], 32);
+    gb2[t] += __shfl_xor(gb2[t], 32);
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) gw1[t][j] += __shfl_xor(gw1[t][j], 32);
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) gw3[o][t] += __shfl_xor(gw3[o][t], 32);
+  }
+  // per-sample sums over the 32 sample lanes (the q == 1 lanes hold zeros)
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    gb3[0] += __shfl_xor(gb3[0], off);
+    gb3[1] += __shfl_xor(gb3[1], off);
+    gl += __shfl_xor(gl, off);
   }
   float* pb = partial + (int64_t)blockIdx.x * (NPARAM + 1);
-  const int i = lane;
 #pragma unroll
-  for (int k = 0; k < NH; ++k) pb[OFF_W2 + i * NH + k] = gW2[k];
+  for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-  for (int j = 0; j < NIN; ++j) pb[OFF_W1 + i * NIN + j] = gW1[j];
-  pb[OFF_B1 + i] = gb1;
-  pb[OFF_B2 + i] = gb2;
+    for (int tk = 0; tk < 2; ++tk)
 #pragma unroll
-  for (int o = 0; o < NOUT; ++o) pb[OFF_W3 + o * NH + i] = gW3[o];
-  if (i < NOUT) pb[OFF_B3 + i] = gb3;
-  if (i == 0) pb[NPARAM] = gl;
+      for (int r = 0; r < 16; ++r) {
+        const int jrow = 32 * tj + 8 * (r >> 2) + 4 * q + (r & 3);
+        pb[OFF_W2 + jrow * NH + 32 * tk + li] = gw2[tj][tk][r];
+      }
+  if (q == 0) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int k = 32 * t + li;
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) pb[OFF_W1 + k * NIN + j] = gw1[t][j];
+      pb[OFF_B1 + k] = gb1[t];
+      pb[OFF_B2 + k] = gb2[t];
+#pragma unroll
+      for (int o = 0; o < NOUT; ++o) pb[OFF_W3 + o * NH + k] = gw3[o][t];
+    }
+  }
+  if (l == 0) {
+    pb[OFF_B3] = gb3[0];
+    pb[OFF_B3 + 1] = gb3[1];
+    pb[NPARAM] = gl;
+  }
 }
 
 // Sum the block partials in a fixed order (deterministic): accumulator r takes
@@ -295,7 +376,7 @@ extern "C" SDEH_HIDDEN int sdeh_mlp_forward_impl(const float* params, const floa
 // (the blocks then stride over the rest).  The partial sums need
 // sdeh_mlp_grad_scratch_bytes(n) bytes of scratch.
 static int64_t grad_blocks(int64_t n) {
-  int64_t nb = (n + 63) / 64;
+  int64_t nb = (n + 31) / 32;  // one wave of 32-sample tiles per block
   return nb > 1024 ? 1024 : (nb < 1 ? 1 : nb);
 }
 
@@ -308,7 +389,7 @@ extern "C" SDEH_HIDDEN int sdeh_mlp_loss_grad_impl(const float* params, const fl
                                                    const float* y, int64_t n, float* loss,
                                                    float* grad, float* scratch, hipStream_t s) {
   const int64_t nb = grad_blocks(n);
-  hipLaunchKernelGGL(mlp_grad_partial, dim3((unsigned)nb), dim3(64), 0, s, params, x, y, n,
+  hipLaunchKernelGGL(mlp_grad_mfma, dim3((unsigned)nb), dim3(64), 0, s, params, x, y, n,
                      scratch);
   if (hipGetLastError() != hipSuccess) return -2;
   hipLaunchKernelGGL(mlp_grad_reduce, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, s, scratch,
