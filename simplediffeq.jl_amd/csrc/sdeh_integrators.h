@@ -77,11 +77,53 @@ struct Ops<double> {
   __device__ static inline double pw_from(const Log& l, double y) {
     return sdeh_math::pow_from_ctrl(l, y);
   }
+
+  // Division and square root with the range handling of the IEEE sequences left
+  // out, for operands the caller has checked.  gfx950 divides binary64 as
+  // v_div_scale (twice), Newton's iteration on v_rcp_f64, v_div_fmas and
+  // v_div_fixup; for |a| in [2^-700, 2^700] and |b| in [2^-60, 2^60] (so b and 1/b
+  // are normal, exponent(a) - exponent(b) < 768, the biased exponent of a exceeds 53
+  // and a/b is normal) both scalings are by 2^0, v_div_fmas is a plain fma and the
+  // fix-up returns the quotient unchanged, so div_r below is the same operations on
+  // the same values.  Likewise sqrt is a scaling for x < 2^-767,
+  // Newton's iteration on v_rsq_f64, the unscaling and a fix-up for 0 and +Inf:
+  // for x in [2^-767, 2^1023] only the iteration remains.  Same bits as `/` and
+  // `__builtin_sqrt` on that domain.
+  static constexpr bool kFast = true;
+  __device__ static inline double rcp_nr(double b) {  // b's refined reciprocal
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    return fma(r, e, r);
+  }
+  __device__ static inline double div_r(double a, double b, double r) {  // r = rcp_nr(b)
+    const double q = a * r;
+    const double rem = fma(-b, q, a);
+    return fma(rem, r, q);
+  }
+  __device__ static inline double div_in(double a, double b) { return div_r(a, b, rcp_nr(b)); }
+  __device__ static inline double root_in(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+  }
 };
 
 template <>
 struct Ops<float> {
   typedef float Log;
+  static constexpr bool kFast = false;  // binary32 divides through v_rcp_f32 already
+  __device__ static inline float rcp_nr(float b) { return b; }
+  __device__ static inline float div_r(float a, float b, float) { return div(a, b); }
+  __device__ static inline float div_in(float a, float b) { return div(a, b); }
+  __device__ static inline float root_in(float x) { return root(x); }
   __device__ static inline float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   __device__ static inline float root(float x) { return __builtin_amdgcn_sqrtf(x); }
   __device__ static inline float pw(float x, float y) {
@@ -188,16 +230,23 @@ __device__ __forceinline__ double stage_t(double t, double c, double dt) {
   return fma(c, dt, t);
 }
 
-// tmp <- tmp / (abstol + max(|uprev|, |u|) * reltol); return ODE_DEFAULT_NORM(tmp)
-// over all components of the trajectory (gathered from the group when G > 1; the
-// sum of squares runs over components 0, 1, ... in that order either way).
+// tmp <- tmp / (abstol + max(|uprev|, |u|) * reltol); return the sum of squares of
+// ODE_DEFAULT_NORM(tmp) over all components of the trajectory (gathered from the
+// group when G > 1; the sum runs over components 0, 1, ... in that order either
+// way).  The norm is sqrt(sum / NormLen<R>).
+//
+// Julia's max(|uprev|, |u|) is the larger value, or NaN if either is NaN.  Here u
+// = uprev + (terms), so a NaN in uprev is a NaN in u, and one comparison select
+// that keeps |u| unless |u| < |uprev| gives the same value (a NaN only through u,
+// and a NaN makes the norm NaN whatever its payload).
 template <class T, class R>
-__device__ __forceinline__ T err_norm(T (&tmp)[R::N], const T (&uprev)[R::N], const T (&u)[R::N],
-                                      T abstol, T reltol) {
+__device__ __forceinline__ T err_sumsq(T (&tmp)[R::N], const T (&uprev)[R::N],
+                                       const T (&u)[R::N], T abstol, T reltol) {
   constexpr int N = R::N;
 #pragma unroll
   for (int c = 0; c < N; ++c) {
-    const T m = sdeh_math::jl_max_nonneg(fabs(uprev[c]), fabs(u[c]));
+    const T au = fabs(u[c]), ap = fabs(uprev[c]);
+    const T m = au < ap ? ap : au;
     tmp[c] = Ops<T>::div(tmp[c], fma(m, reltol, abstol));
   }
   if constexpr (R::G > 1) {
@@ -207,13 +256,77 @@ __device__ __forceinline__ T err_norm(T (&tmp)[R::N], const T (&uprev)[R::N], co
     T s = all[0] * all[0];
 #pragma unroll
     for (int c = 1; c < R::G; ++c) s = s + all[c] * all[c];
-    return Ops<T>::root(Ops<T>::div(s, T(R::G)));
+    return s;
   } else {
     T s = tmp[0] * tmp[0];
 #pragma unroll
     for (int c = 1; c < N; ++c) s = s + tmp[c] * tmp[c];
-    return Ops<T>::root(Ops<T>::div(s, T(N)));
+    return s;
   }
+}
+
+template <class R>
+struct NormLen {
+  static constexpr int value = R::G > 1 ? R::G : R::N;
+  static constexpr bool pow2 = (value & (value - 1)) == 0;
+};
+
+// The step-size controller from the error sum of squares s:
+// EEst = sqrt(s / len), q11 = EEst^beta1, and the proposed dt / q, where
+// q = clamp((q11 / qold^beta2) / gamma) on accept and min(1/qmin, q11 / gamma) on
+// reject (gpuatsit5.jl:279-299, gpuvern7.jl:406-426, gpuvern9.jl:575-595).  It
+// is evaluated for every lane without branching on accept/reject: the reject
+// step dt / min(inv_qmin, q11/gamma) and the accept step
+// dt / clamp((q11/qold^beta2)/gamma) share their three divisions, a reject
+// dividing q11 by 1 (exact), and EEst == 0 dividing inv_qmax by 1.  EEst^beta1 now
+// and EEst^beta2 (the next step's qold^beta2 when qold becomes EEst) share one
+// logarithm, returned in `lg`: same bits as two pow calls.
+//
+// IN: every division and the root go through Ops<T>::div_r / div_in / root_in,
+// valid when s is in [2^-700, 2^700] and dt <= 2^700.  Then EEst is in
+// [2^-351, 2^350], q11 in [2^-50, 2^49], qold^beta2 in [0.47, 1] (qold =
+// max(EEst, 1e-4) of an accepted step, EEst <= 1), q in [1/qmax, 1/qmin] and dt >=
+// the solver's dtmin (>= 1e-14): every numerator lies in [2^-700, 2^700] and every
+// divisor (len, qold^beta2 or 1, gamma, q) in [2^-60, 2^60], the range those need.
+// For other s (0, tiny, huge, Inf, NaN) the IEEE operations run instead and the
+// IN results are discarded; computing them is plain arithmetic (the logarithm's and
+// exponential's table indices are masked to the tables) on any input.
+template <class T>
+struct CtrlK {
+  T beta1, inv_qmax, inv_qmin, gamma, r_len, r_gamma;
+};
+template <class T>
+struct CtrlOut {
+  T EEst, dtq;
+  typename Ops<T>::Log lg;
+};
+template <class T>
+__device__ __forceinline__ bool ctrl_in_range(T s, T dt) {
+  if constexpr (!Ops<T>::kFast) return false;
+  else return s >= T(0x1p-700) && s <= T(0x1p700) && dt <= T(0x1p700);
+}
+template <class T, class R, bool IN>
+__device__ __forceinline__ CtrlOut<T> controller(T s, T dt, T qold_pow, const CtrlK<T>& k) {
+  constexpr int L = NormLen<R>::value;
+  CtrlOut<T> o;
+  T x;
+  if constexpr (!IN || NormLen<R>::pow2) x = Ops<T>::div(s, T(L));  // a power of 2: exact
+  else x = Ops<T>::div_r(s, T(L), k.r_len);
+  o.EEst = IN ? Ops<T>::root_in(x) : Ops<T>::root(x);
+  o.lg = Ops<T>::plog(o.EEst);
+  const T q11 = Ops<T>::pw_from(o.lg, k.beta1);
+  const bool accept = !(o.EEst > T(1));
+  const bool zero = (o.EEst == T(0));
+  const T num = (accept && zero) ? k.inv_qmax : q11;
+  const T den = (accept && !zero) ? qold_pow : T(1);
+  const T qg = IN ? Ops<T>::div_r(Ops<T>::div_in(num, den), k.gamma, k.r_gamma)
+                  : Ops<T>::div(Ops<T>::div(num, den), k.gamma);
+  // With EEst not NaN, q11 and qold^beta2 are finite or +Inf and qg is not NaN:
+  // Julia's min (gpuatsit5.jl:287) reduces to one comparison select.
+  const T q = accept ? sdeh_math::fast_max(k.inv_qmax, sdeh_math::fast_min(k.inv_qmin, qg))
+                     : sdeh_math::fast_min(k.inv_qmin, qg);
+  o.dtq = IN ? Ops<T>::div_in(dt, q) : Ops<T>::div(dt, q);
+  return o;
 }
 
 // Horner evaluation of evalpoly(θ, (0, r...)) / evalpoly(θ, (0, 0, r...)).
@@ -741,6 +854,8 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
   const T inv_qmax = T(1) / qmax, inv_qmin = T(1) / qmin;
   const T qoldinit_pow = Ops<T>::pw(qoldinit, beta2);
   const T abstol = (T)a.abstol, reltol = (T)a.reltol;
+  const CtrlK<T> ck{beta1, inv_qmax, inv_qmin, gamma, Ops<T>::rcp_nr(T(NormLen<R>::value)),
+                    Ops<T>::rcp_nr(gamma)};
   const T* ps = (const T*)a.ps;
   const int64_t n = a.n;
 
@@ -841,40 +956,37 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
       }
 #pragma unroll
       for (int c = 0; c < N; ++c) err[c] = dt * err[c];
-      const T EEst = err_norm<T, R>(err, uprev, u, abstol, reltol);
+      const T ssq = err_sumsq<T, R>(err, uprev, u, abstol, reltol);
+      CtrlOut<T> co;
+      // The range-checked controller runs for every lane; the IEEE one again only
+      // where the check fails (a branch the wave skips when no lane needs it).
+      // Measured on the 2^20-trajectory Lorenz ATsit5 solve: 1.135 ms, against 1.27
+      // ms for an if/else on the check and 1.19 ms with IEEE operations only.
+      if constexpr (Ops<T>::kFast) {
+        co = controller<T, R, true>(ssq, dt, qold_pow, ck);
+        if (!ctrl_in_range(ssq, dt)) co = controller<T, R, false>(ssq, dt, qold_pow, ck);
+      } else {
+        co = controller<T, R, false>(ssq, dt, qold_pow, ck);
+      }
+      const T EEst = co.EEst;
       if (EEst != EEst) {
         ret = RET_NAN;
       } else {
-        // ---- PI controller: gpuatsit5.jl:279-299, gpuvern7.jl:406-426,
-        // gpuvern9.jl:575-595 ----
-        // Evaluated for every lane without branching on accept/reject: the reject
-        // step dt / min(inv_qmin, q11/gamma) and the accept step
-        // dt / clamp((q11/qold^beta2)/gamma) share their three divisions, a reject
-        // dividing q11 by 1 (exact), and EEst == 0 dividing inv_qmax by 1.
-        // EEst^beta1 now and EEst^beta2 (the next step's qold^beta2 when qold becomes
-        // EEst) share one logarithm: same bits as two pow calls.
-        const typename Ops<T>::Log lg = Ops<T>::plog(EEst);
-        const T q11 = Ops<T>::pw_from(lg, beta1);
-        const T e_b2 = Ops<T>::pw_from(lg, beta2);
+        // ---- PI controller: accept/reject (gpuatsit5.jl:279-299,
+        // gpuvern7.jl:406-426, gpuvern9.jl:575-595) ----
         const bool accept = !(EEst > T(1));
-        const bool zero = (EEst == T(0));
-        const T num = (accept && zero) ? inv_qmax : q11;
-        const T den = (accept && !zero) ? qold_pow : T(1);
-        const T qg = Ops<T>::div(Ops<T>::div(num, den), gamma);
-        // EEst is not NaN here, so q11 and qold^beta2 are finite or +Inf and qg is not
-        // NaN: Julia's min (gpuatsit5.jl:287) reduces to one comparison select.
-        const T q = accept ? sdeh_math::fast_max(inv_qmax, sdeh_math::fast_min(inv_qmin, qg))
-                           : sdeh_math::fast_min(inv_qmin, qg);
-        const T dtq = Ops<T>::div(dt, q);
+        const T dtq = co.dtq;
         if (!accept) {
           dt = dtq;
           ++nrej;
         } else {
           qold = sdeh_math::fast_max(EEst, qoldinit);  // jl_max: EEst >= 0, not NaN
-          qold_pow = (qold == EEst) ? e_b2 : qoldinit_pow;
+          qold_pow = (qold == EEst) ? Ops<T>::pw_from(co.lg, beta2) : qoldinit_pow;
           const T dtold = dt;
           const double rem = tf - t - (double)dtold;
-          dt = (T)sdeh_math::jl_min_nonneg((double)fabs(dtq), fabs(rem));
+          // Julia's min: neither argument is NaN here (dtq is dt / q with q in
+          // [1/qmax, 1/qmin]; t and tf are finite or tf = +Inf), so one select.
+          dt = (T)sdeh_math::fast_min((double)fabs(dtq), fabs(rem));
           const double told = t;
           t = (rem < SNAP) ? tf : t + (double)dtold;
           ++nacc;
