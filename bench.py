@@ -225,10 +225,14 @@ def workload_glv(args):
     out = {"workload": "glv16-saveat", "trajectories": n}
     for name, alg, dtype, tol in (("avern9_f64", _abi.ALG_AVERN9, _abi.F64, 1e-10),
                                   ("atsit5_f32", _abi.ALG_ATSIT5, _abi.F32, 1e-6)):
-        for layout, tag in ((_abi.SAVE_TRAJ_MAJOR, ""), (_abi.SAVE_POINT_MAJOR, "_point_major")):
+        # default lane layout in both save layouts; binary64 also in each layout
+        variants = [(_abi.SAVE_TRAJ_MAJOR, 0, ""), (_abi.SAVE_POINT_MAJOR, 0, "_point_major")]
+        if dtype == _abi.F64:
+            variants += [(_abi.SAVE_TRAJ_MAJOR, 16, "_lanes16"), (_abi.SAVE_TRAJ_MAJOR, 4, "_lanes4")]
+        for layout, lanes, tag in variants:
             d = runtime.make_desc(alg, dtype=dtype, rhs=_abi.RHS_GLV16, n_traj=n,
                                   tspan=(0.0, 50.0), dt=0.1, abstol=tol, reltol=tol,
-                                  saveat=saveat, save_layout=layout)
+                                  saveat=saveat, save_layout=layout, lanes=lanes)
             g = runtime.EnsembleBuffers(d, 16, u0, r, A, backend="torch", device="cuda:0")
             ctx.solve(d, g, sync=True)
             st = ctx.stats()

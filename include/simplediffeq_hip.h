@@ -160,13 +160,16 @@ typedef struct sdeh_solve_desc {
   int32_t precision;        /* sdeh_precision */
   int32_t block;            /* 0 = tuned; else workgroup size (multiple of 64, <= 1024) */
   int32_t grid;             /* 0 = tuned; persistent schedule: number of workgroups */
-  int32_t chunk;            /* 0 = tuned; persistent schedule: at most this many
-                               trajectories per queue grab (grabs shrink near the end) */
+  int32_t chunk;            /* 0 = tuned; persistent schedule: trajectories per queue
+                               grab (the last grab takes what is left) */
   int32_t lanes;            /* lanes per trajectory: 0 = for binary64, the right-hand
-                               side's group layout where it has one (GLV16: 16 lanes,
-                               one component each), else 1; 1 = one lane holds a whole
-                               trajectory; or the group size itself.  Results are
-                               identical. */
+                               side's matrix-core layout where it has a kernel for the
+                               solver and output (GLV16: 4 lanes 16 apart, 4 components
+                               each, A u on v_mfma_f64_16x16x4f64; adaptive solvers,
+                               endpoint and saveat), else its group layout (GLV16: 16
+                               adjacent lanes, one component each), else 1; 1 = one
+                               lane holds a whole trajectory; or either layout's size
+                               (4, 16) itself.  Results are identical. */
   int32_t save_layout;      /* sdeh_save_layout of u_saveat, u_steps and t_steps */
   int32_t span_order;       /* 0: with per-trajectory tspans (io->tspan) and no io->order,
                                hand trajectories to lanes longest span first (fewer

@@ -30,7 +30,11 @@ const void* adaptive_ptr() {
 // (each of its kernels is large); SDEH_LIMITED selects that subset.
 template <class T, int ALG, int SAVE>
 constexpr bool allowed() {
-#ifdef SDEH_LIMITED
+#if defined(SDEH_WAVE_ONLY)
+  // The matrix-core layout: binary64 adaptive solvers, endpoint and saveat output.
+  return sizeof(T) == 8 && (ALG == sdeh::ALG_AVERN9 || ALG == sdeh::ALG_ATSIT5) &&
+         SAVE != sdeh::SAVE_STEPS;
+#elif defined(SDEH_LIMITED)
   return (ALG == sdeh::ALG_AVERN9 && sizeof(T) == 8 && SAVE != sdeh::SAVE_STEPS) ||
          (ALG == sdeh::ALG_ATSIT5 && SAVE != sdeh::SAVE_STEPS) ||
          (ALG == sdeh::ALG_TSIT5 && SAVE == sdeh::SAVE_END);

@@ -141,10 +141,36 @@ struct Ops<float> {
 // exchanges the state across the group through LDS.  The stage, error and
 // dense-output arithmetic is the same code in both layouts; only loads, stores,
 // the error norm and work distribution look at G.
+//
+// Wave layout (R::WAVE, for right-hand sides evaluated on the matrix cores): the
+// G lanes of a trajectory are 64/G apart, so a wave holds 64/G trajectories, lane
+// l serving trajectory slot l % (64/G) of the wave at group position q = l / (64/G),
+// and lane q holds components q, q + G, q + 2G, ... (R::N of them).  R::f is then a
+// wave-wide operation: the integrator calls it in uniform control flow only
+// (adaptive_body_wave).
 // ---------------------------------------------------------------------------
+template <class R, class = int>
+struct IsWave {
+  static constexpr bool value = false;
+};
+template <class R>
+struct IsWave<R, decltype((void)R::WAVE, 0)> {
+  static constexpr bool value = R::WAVE;
+};
+
 template <class R>
 __device__ __forceinline__ int lane_comp() {
-  return R::G == 1 ? 0 : (int)(threadIdx.x & (unsigned)(R::G - 1));
+  if constexpr (IsWave<R>::value)
+    return (int)((threadIdx.x & 63u) / (unsigned)(64 / R::G));
+  else
+    return R::G == 1 ? 0 : (int)(threadIdx.x & (unsigned)(R::G - 1));
+}
+
+// Component index of this lane's register c.
+template <class R>
+__device__ __forceinline__ int comp_idx(int c) {
+  if constexpr (IsWave<R>::value) return lane_comp<R>() + R::G * c;
+  else return lane_comp<R>() * R::N + c;
 }
 
 // all[j] = v of lane j of this lane's group (j = 0..G-1), for lanes in uniform
@@ -163,6 +189,26 @@ __device__ __forceinline__ void group_gather(T v, T (&all)[G]) {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
   for (int j = 0; j < G; ++j) all[j] = xbuf[base + j];
+}
+
+// Wave layout: all[c] = component c (c = 0 .. N*G-1) of this lane's trajectory,
+// for a wave in uniform control flow.  LDS [wave][slot][component], 16 x 16 values
+// per wave of a block of at most 256 threads.
+template <class T, class R>
+__device__ __forceinline__ void wave_gather(const T (&v)[R::N], T (&all)[R::N * R::G]) {
+  constexpr int FULL = R::N * R::G, SLOTS = 64 / R::G;
+  static_assert(FULL * SLOTS <= 256, "one wave's share of the exchange buffer");
+  __shared__ T wbuf[4][256];
+  T* w = wbuf[threadIdx.x >> 6] + (threadIdx.x & (unsigned)(SLOTS - 1)) * FULL;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int c = 0; c < R::N; ++c) w[comp_idx<R>(c)] = v[c];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int c = 0; c < FULL; ++c) all[c] = w[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -249,7 +295,15 @@ __device__ __forceinline__ T err_sumsq(T (&tmp)[R::N], const T (&uprev)[R::N],
     const T m = au < ap ? ap : au;
     tmp[c] = Ops<T>::div(tmp[c], fma(m, reltol, abstol));
   }
-  if constexpr (R::G > 1) {
+  if constexpr (IsWave<R>::value) {
+    constexpr int FULL = N * R::G;
+    T all[FULL];
+    wave_gather<T, R>(tmp, all);
+    T s = all[0] * all[0];
+#pragma unroll
+    for (int c = 1; c < FULL; ++c) s = s + all[c] * all[c];
+    return s;
+  } else if constexpr (R::G > 1) {
     static_assert(N == 1, "a lane of a group holds one component");
     T all[R::G];
     group_gather<T, R::G>(tmp[0], all);
@@ -267,7 +321,7 @@ __device__ __forceinline__ T err_sumsq(T (&tmp)[R::N], const T (&uprev)[R::N],
 
 template <class R>
 struct NormLen {
-  static constexpr int value = R::G > 1 ? R::G : R::N;
+  static constexpr int value = R::N * R::G;
   static constexpr bool pow2 = (value & (value - 1)) == 0;
 };
 
@@ -686,10 +740,9 @@ template <class T, class R>
 __device__ __forceinline__ void store_state(void* base, int64_t row, int64_t n, int64_t traj,
                                             const T (&u)[R::N]) {
   constexpr int FULL = R::N * R::G;
-  const int c0 = lane_comp<R>() * R::N;
   T* o = (T*)base;
 #pragma unroll
-  for (int c = 0; c < R::N; ++c) o[(row * FULL + c0 + c) * n + traj] = u[c];
+  for (int c = 0; c < R::N; ++c) o[(row * FULL + comp_idx<R>(c)) * n + traj] = u[c];
 }
 
 // Row `row` of a saved-state output with `rows` rows, in the layout the kernel
@@ -703,10 +756,9 @@ __device__ __forceinline__ void store_save(const sdeh_kargs& a, void* base, int6
     store_state<T, R>(base, row, a.n, traj, u);
     return;
   }
-  const int c0 = lane_comp<R>() * R::N;
-  T* o = (T*)base + (traj * rows + row) * FULL + c0;
+  T* o = (T*)base + (traj * rows + row) * FULL;
 #pragma unroll
-  for (int c = 0; c < R::N; ++c) o[c] = u[c];
+  for (int c = 0; c < R::N; ++c) o[comp_idx<R>(c)] = u[c];
 }
 
 __device__ __forceinline__ void store_save_time(const sdeh_kargs& a, int64_t row, int64_t traj,
@@ -747,9 +799,9 @@ __device__ __forceinline__ void save_step(const sdeh_kargs& a, StepSink& s, int6
   if (a.packed) {
     constexpr int FULL = R::N * R::G;
     const int64_t row = s.base + s.kept;
-    T* o = (T*)a.u_steps + row * FULL + lane_comp<R>() * R::N;
+    T* o = (T*)a.u_steps + row * FULL;
 #pragma unroll
-    for (int c = 0; c < R::N; ++c) o[c] = u[c];
+    for (int c = 0; c < R::N; ++c) o[comp_idx<R>(c)] = u[c];
     if (lead) a.t_steps[row] = t;
   } else {
     store_save<T, R>(a, a.u_steps, s.kept, a.max_saves, traj, u);
@@ -767,9 +819,8 @@ __device__ __forceinline__ void finish_steps(const sdeh_kargs& a, const StepSink
 template <class T, class R>
 __device__ __forceinline__ void load_state(T (&u)[R::N], const void* base, int64_t n,
                                            int64_t traj) {
-  const int c0 = lane_comp<R>() * R::N;
 #pragma unroll
-  for (int c = 0; c < R::N; ++c) u[c] = ((const T*)base)[(c0 + c) * n + traj];
+  for (int c = 0; c < R::N; ++c) u[c] = ((const T*)base)[comp_idx<R>(c) * n + traj];
 }
 
 // Per-lane parameters: p[c][traj] for c < R::P, or the right-hand side's own
@@ -842,8 +893,252 @@ __device__ __forceinline__ double save_time(const sdeh_kargs& a, int64_t i) {
 // ---------------------------------------------------------------------------
 // Adaptive solvers: ATsit5, AVern7, AVern9.
 // ---------------------------------------------------------------------------
+// Adaptive solvers in the wave layout.  The same per-trajectory arithmetic as
+// adaptive_body, arranged so that every right-hand-side evaluation runs with the
+// whole wave active (R::f is one wave-wide matrix operation over all the wave's
+// trajectories): the stages are computed by every lane each iteration, lanes
+// without a running trajectory on stale values whose results are dropped; the
+// FSAL seed of new trajectories and the extra stages of dense output run for the
+// whole wave when any of its lanes needs them.  Decisions and outputs are per
+// lane, as in adaptive_body.
+template <class T, class R, int ALG, int SAVE, bool KAHAN>
+__device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
+  constexpr int N = R::N;
+  constexpr int P = R::P;
+  constexpr int SLOTS = 64 / R::G;  // trajectories per wave
+  constexpr bool V9 = (ALG == ALG_AVERN9);
+  constexpr double DTMIN = V9 ? (double)1.0e-7f : 1.0e-14;  // gpuvern9.jl:466
+  constexpr double SNAP = DTMIN;                            // gpuvern9.jl:591
+  // build_adaptive_controller_cache(T), src/SimpleDiffEq.jl:67-77
+  const T beta1 = T(7.0 / 50), beta2 = T(2.0 / 25), qmax = T(10.0), qmin = T(1.0 / 5),
+          gamma = T(9.0 / 10), qoldinit = T(1.0e-4);
+  const T inv_qmax = T(1) / qmax, inv_qmin = T(1) / qmin;
+  const T qoldinit_pow = Ops<T>::pw(qoldinit, beta2);
+  const T abstol = (T)a.abstol, reltol = (T)a.reltol;
+  const CtrlK<T> ck{beta1, inv_qmax, inv_qmin, gamma, Ops<T>::rcp_nr(T(NormLen<R>::value)),
+                    Ops<T>::rcp_nr(gamma)};
+  const T* ps = (const T*)a.ps;
+  const int64_t n = a.n;
+  const int lane = (int)__lane_id();
+
+  // Lane state; lanes without a trajectory keep finite values.
+  T uprev[N], comp[N], fsal[N], pl[P];
+#pragma unroll
+  for (int c = 0; c < N; ++c) uprev[c] = comp[c] = fsal[c] = T(0);
+#pragma unroll
+  for (int c = 0; c < P; ++c) pl[c] = T(0);
+  R::template init_lane<T>(pl, ps);  // the wave-wide operands, for every lane
+  double t = 0.0, tf = 0.0;
+  T dt = T(1), qold = qoldinit, qold_pow = qoldinit_pow;
+  int32_t nacc = 0, nrej = 0, cur = 0;
+  StepSink sk{0, 0, 0, 0};
+  double ts_next = 0.0;  // save_time(cur)
+  int64_t attempts = 0, traj = -1;
+  bool active = false;
+
+  int64_t qbase = 0, qlim = 0;
+  bool drained = !a.persistent;
+  const bool lead = lane_comp<R>() == 0;  // writes the per-trajectory scalars
+  if (!a.persistent) {
+    const int64_t slot =
+        (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * SLOTS + (lane & (SLOTS - 1));
+    if (slot < n) {
+      traj = a.order ? a.order[slot] : slot;
+      active = true;
+    }
+  }
+  bool fresh = active;
+
+  for (;;) {
+    if (a.persistent) {
+      int64_t slot = wave_claim(a, !active && lead, qbase, qlim, drained);
+      slot = (int64_t)__shfl((long long)slot, lane & (SLOTS - 1));  // from the slot's lead lane
+      if (slot >= 0) {
+        traj = a.order ? a.order[slot] : slot;
+        active = true;
+        fresh = true;
+      }
+    }
+    if (__ballot(active) == 0ull) break;
+
+    if (fresh) {  // start trajectory `traj` (gpuatsit5.jl:221-241)
+      load_state<T, R>(uprev, a.u0, n, traj);
+#pragma unroll
+      for (int c = 0; c < N; ++c) comp[c] = T(0);
+      R::template load_lane<T>(pl, (const T*)a.p, ps, n, traj, lane_comp<R>());
+      t = a.tspan ? a.tspan[traj] : a.t0;
+      tf = a.tspan ? a.tspan[n + traj] : a.tf;
+      dt = (T)(a.dt0 ? a.dt0[traj] : a.dt);
+      qold = qoldinit;
+      qold_pow = qoldinit_pow;
+      nacc = nrej = 0;
+      attempts = 0;
+      cur = 0;
+      if constexpr (SAVE == SAVE_AT) {
+        if (a.n_saveat > 0 && t == a.saveat[0]) {
+          store_save<T, R>(a, a.u_saveat, 0, a.n_saveat, traj, uprev);
+          cur = 1;
+        }
+        ts_next = save_time(a, cur);
+      }
+      if constexpr (SAVE == SAVE_STEPS) {
+        sink_start<R>(a, sk, traj);
+        save_step<T, R>(a, sk, traj, t, uprev);
+      }
+    }
+    if constexpr (ALG == ALG_ATSIT5) {  // FSAL seed, :241
+      if (__ballot(fresh) != 0ull) {
+        T k1[N];
+        R::f(k1, uprev, pl, ps, t);
+        if (fresh) copyv(fsal, k1);
+      }
+    }
+    fresh = false;
+
+    int ret = -1;  // -1: running
+    bool run = false;
+    if (active) {
+      if (!(t < tf)) ret = RET_DEFAULT;
+      else if ((double)dt < DTMIN) ret = RET_DTMIN;  // :256
+      else if (attempts >= a.maxiters) ret = RET_MAXITERS;
+      else run = true;
+    }
+
+    // ---- one attempt for every lane: stages, trial state u and the embedded error ----
+    T u[N], err[N], comp_new[N];
+    T k2[N], k3[N], k4[N], k5[N], k6[N], k7[N];  // ATsit5 stages (FSAL k1 = fsal)
+    V7K<T, N> K7;                                 // AVern7 stages
+    V9S<T, N> S9;                                 // AVern9 stages
+    if constexpr (ALG == ALG_ATSIT5) {
+      tsit5_stages<T, R, KAHAN>(uprev, fsal, k2, k3, k4, k5, k6, k7, u, comp, comp_new, pl, ps,
+                                t, dt);
+      // dt * (btilde1*k1 + ... + btilde7*k7), gpuatsit5.jl:272-275
+      lsum(err, T(sdeh_tab::tsit5::btilde1), fsal, T(sdeh_tab::tsit5::btilde2), k2,
+           T(sdeh_tab::tsit5::btilde3), k3, T(sdeh_tab::tsit5::btilde4), k4,
+           T(sdeh_tab::tsit5::btilde5), k5, T(sdeh_tab::tsit5::btilde6), k6,
+           T(sdeh_tab::tsit5::btilde7), k7);
+    } else if constexpr (V9) {
+      vern9_stages<T, R, true, KAHAN, SAVE == SAVE_AT>(uprev, S9, u, err, comp, comp_new, pl,
+                                                        ps, t, dt, false);
+    } else {
+      vern7_stages<T, R, true, KAHAN>(uprev, K7, u, err, comp, comp_new, pl, ps, t, dt);
+    }
+#pragma unroll
+    for (int c = 0; c < N; ++c) err[c] = dt * err[c];
+    const T ssq = err_sumsq<T, R>(err, uprev, u, abstol, reltol);
+    CtrlOut<T> co;
+    if constexpr (Ops<T>::kFast) {
+      co = controller<T, R, true>(ssq, dt, qold_pow, ck);
+      if (run && !ctrl_in_range(ssq, dt)) co = controller<T, R, false>(ssq, dt, qold_pow, ck);
+    } else {
+      co = controller<T, R, false>(ssq, dt, qold_pow, ck);
+    }
+    const T EEst = co.EEst;
+
+    // ---- PI controller: accept/reject (gpuatsit5.jl:279-299, gpuvern7.jl:406-426,
+    // gpuvern9.jl:575-595) ----
+    bool stepped = false;  // accepted: t has advanced, uprev not yet
+    double told = 0.0;
+    T dtold = T(0);
+    if (run) {
+      ++attempts;
+      if (EEst != EEst) {
+        ret = RET_NAN;
+      } else if (EEst > T(1)) {
+        dt = co.dtq;
+        ++nrej;
+      } else {
+        qold = sdeh_math::fast_max(EEst, qoldinit);  // jl_max: EEst >= 0, not NaN
+        qold_pow = (qold == EEst) ? Ops<T>::pw_from(co.lg, beta2) : qoldinit_pow;
+        dtold = dt;
+        const double rem = tf - t - (double)dtold;
+        dt = (T)sdeh_math::fast_min((double)fabs(co.dtq), fabs(rem));
+        told = t;
+        t = (rem < SNAP) ? tf : t + (double)dtold;
+        ++nacc;
+        stepped = true;
+      }
+    }
+    if constexpr (SAVE == SAVE_AT) {
+      if constexpr (ALG == ALG_ATSIT5) {  // gpuatsit5.jl:305-316
+        if (stepped) {
+          while (cur < a.n_saveat && ts_next <= t) {
+            const double ts_after = save_time(a, cur + 1);
+            const T th = (T)((ts_next - told) / (double)dtold);
+            T o[N];
+            tsit5_dense(o, uprev, dtold, th, fsal, k2, k3, k4, k5, k6, k7);
+            store_save<T, R>(a, a.u_saveat, cur, a.n_saveat, traj, o);
+            ++cur;
+            ts_next = ts_after;
+          }
+        }
+      } else {
+        // Extra stages depend only on the step, not on θ: computed once, for the
+        // whole wave when one of its lanes has a save point in the step.
+        const bool saves = stepped && cur < a.n_saveat && ts_next <= t;
+        if (__ballot(saves) != 0ull) {
+          if constexpr (V9) {
+            T x[10][N];
+            vern9_extra<T, R>(uprev, S9, x, pl, ps, told, dtold);  // told + c*dtold
+            if (saves) {
+              while (cur < a.n_saveat && ts_next <= t) {
+                const double ts_after = save_time(a, cur + 1);
+                const T th = (T)((ts_next - told) / (double)dtold);
+                T o[N];
+                vern9_dense(o, uprev, dtold, th, S9, x);
+                store_save<T, R>(a, a.u_saveat, cur, a.n_saveat, traj, o);
+                ++cur;
+                ts_next = ts_after;
+              }
+            }
+          } else {
+            T x[6][N];
+            // Quirk (gpuvern7.jl:449): stage times from the post-step t.
+            vern7_extra<T, R>(uprev, K7, x, pl, ps, a.quirks ? t : told, dtold);
+            if (saves) {
+              while (cur < a.n_saveat && ts_next <= t) {
+                const double ts_after = save_time(a, cur + 1);
+                const T th = (T)((ts_next - told) / (double)dtold);
+                T o[N];
+                vern7_dense(o, uprev, dtold, th, K7, x);
+                store_save<T, R>(a, a.u_saveat, cur, a.n_saveat, traj, o);
+                ++cur;
+                ts_next = ts_after;
+              }
+            }
+          }
+        }
+      }
+    }
+    if (stepped) {
+      if constexpr (SAVE == SAVE_STEPS) save_step<T, R>(a, sk, traj, t, u);
+      copyv(uprev, u);
+      if constexpr (ALG == ALG_ATSIT5) copyv(fsal, k7);
+      if constexpr (KAHAN) copyv(comp, comp_new);
+      if (!(t < tf)) ret = RET_DEFAULT;
+    }
+
+    if (ret >= 0) {  // trajectory ends: write outputs (gpuatsit5.jl:322-329)
+      if (SAVE == SAVE_STEPS && ret == RET_DEFAULT && sk.kept < sk.n) ret = RET_SAVE_OVERFLOW;
+      store_state<T, R>(a.u_end, 0, n, traj, uprev);
+      if (lead) {
+        a.t_end[traj] = t;
+        a.retcode[traj] = ret;
+        if (a.naccept) a.naccept[traj] = nacc;
+        if (a.nreject) a.nreject[traj] = nrej;
+        if constexpr (SAVE == SAVE_STEPS) finish_steps(a, sk, traj);
+      }
+      active = false;
+    }
+  }
+}
+
 template <class T, class R, int ALG, int SAVE, bool KAHAN>
 __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
+  if constexpr (IsWave<R>::value) {
+    adaptive_body_wave<T, R, ALG, SAVE, KAHAN>(a);
+    return;
+  }
   constexpr int N = R::N;
   constexpr int P = R::P > 0 ? R::P : 1;
   constexpr double DTMIN = (ALG == ALG_AVERN9) ? (double)1.0e-7f : 1.0e-14;  // gpuvern9.jl:466
@@ -1071,6 +1366,7 @@ __device__ __forceinline__ double grid_elem(const sdeh_kargs& a, int64_t i) {
 
 template <class T, class R, int ALG, int SAVE>
 __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
+  static_assert(!IsWave<R>::value, "fixed-step solvers run the lane and group layouts");
   constexpr int N = R::N;
   constexpr int P = R::P > 0 ? R::P : 1;
   const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / R::G;
@@ -1207,12 +1503,17 @@ namespace sdeh {
 // Waves per SIMD the compiler is asked to keep resident.  The binary64 adaptive
 // kernels of small states with endpoint output need a few VGPRs more than the 96
 // that five waves allow when left alone (ATsit5/Lorenz: 101-103); held to 96 they
-// lose nothing and run 6% faster (five waves instead of four).  Larger kernels
-// keep the compiler's choice (1 = no minimum).
+// lose nothing and run 6% faster (five waves instead of four).  The wave-layout
+// (matrix-core) kernels with saveat output take 328 registers left alone (one wave
+// per SIMD); held to two waves they spill 31 registers to scratch and still run 22%
+// faster (GLV-16 AVern9, 2^17 trajectories, 256 points: 10.8 against 13.8 ms).
+// Other kernels keep the compiler's choice (1 = no minimum).
 template <class T, class R, int ALG, int SAVE>
 struct WavesHint {
   static constexpr int value =
-      (sizeof(T) == 8 && R::N * R::G <= 4 && SAVE == SAVE_END && ALG == ALG_ATSIT5) ? 5 : 1;
+      IsWave<R>::value ? (SAVE == SAVE_AT ? 2 : 1)
+                       : ((sizeof(T) == 8 && R::N * R::G <= 4 && SAVE == SAVE_END &&
+                           ALG == ALG_ATSIT5) ? 5 : 1);
 };
 }  // namespace sdeh
 

@@ -83,6 +83,49 @@ struct GLV16G {
   }
 };
 
+// GLV-16 on the matrix cores (binary64), in the wave layout: 4 lanes per
+// trajectory and 16 trajectories per wave.  Trajectory slot j of a wave is on lanes
+// j, j+16, j+32, j+48; lane q (= lane / 16) holds components q, q+4, q+8, q+12.
+// A u for the wave's 16 trajectories is four v_mfma_f64_16x16x4f64 (D = A B + C,
+// A 16x4, B 4x16, C and D 16x16): product kb covers components 4kb .. 4kb+3.  Its B
+// operand is the lane's own component q + 4kb (register kb: no data movement), its A
+// operand the lane's fixed entry A[lane % 16][4kb + q], C the running sums, whose
+// row for lane q, register r is component q + 4r (the lane's own).  The instruction
+// forms each output as fma(a3, b3, fma(a2, b2, fma(a1, b1, fma(a0, b0, c)))) over
+// k = 0..3 in order (lane layout and rounding checked on the device, 2^20 outputs
+// bit for bit), so acc_i = r_i, then fma(A_ij, u_j, acc) for j = 0..15:
+// GLV16's arithmetic, with no exchange of the state between lanes.
+// pl[0..3] = r of the lane's components (per trajectory), pl[4..7] = the lane's A
+// operands (the same for every trajectory: loaded once for every lane).
+struct GLV16M {
+  static constexpr int N = 4, P = 8, S = 256, G = 4;
+  static constexpr bool WAVE = true;
+  template <class T>
+  __device__ static inline void init_lane(T* pl, const T* __restrict__ ps) {
+    const int l = (int)__lane_id();
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) pl[4 + kb] = ps[(l & 15) * 16 + 4 * kb + (l >> 4)];
+  }
+  template <class T>
+  __device__ static inline void load_lane(T* pl, const T* p, const T* __restrict__ ps, int64_t n,
+                                          int64_t traj, int q) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) pl[r] = p[(q + 4 * r) * n + traj];
+  }
+  template <class T>
+  __device__ static inline void f(T* du, const T* u, const T* p, const T* __restrict__ ps,
+                                  double t) {
+    static_assert(sizeof(T) == 8, "binary64 matrix-core product");
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    d4 acc = {p[0], p[1], p[2], p[3]};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(p[4 + kb], u[kb], acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) du[r] = u[r] * acc[r];
+  }
+};
+
 // u' = p0 u, three components (test/gpu_ode_regression.jl:2-9 with p0 = -1).
 struct ExpDecay3 {
   static constexpr int N = 3, P = 1, S = 0, G = 1;

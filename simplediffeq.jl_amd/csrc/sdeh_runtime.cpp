@@ -34,6 +34,7 @@ extern "C" {
 SDEH_HIDDEN const void* sdeh_aot_Lorenz(int, int, int, int);
 SDEH_HIDDEN const void* sdeh_aot_GLV16(int, int, int, int);
 SDEH_HIDDEN const void* sdeh_aot_GLV16G(int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_GLV16M(int, int, int, int);
 SDEH_HIDDEN const void* sdeh_aot_ExpDecay3(int, int, int, int);
 SDEH_HIDDEN const void* sdeh_aot_Affine1(int, int, int, int);
 SDEH_HIDDEN const void* sdeh_aot_Linear2(int, int, int, int);
@@ -81,14 +82,18 @@ struct Builtin {
   // Kernels of the group layout (`group` lanes per trajectory), if built.
   int32_t group;
   const void* (*group_lookup)(int, int, int, int);
+  // Kernels of the matrix-core wave layout (`mgroup` lanes per trajectory), if built.
+  int32_t mgroup;
+  const void* (*mgroup_lookup)(int, int, int, int);
 };
 
 const Builtin kBuiltins[] = {
-    {SDEH_RHS_LORENZ, "lorenz", 3, 3, 0, sdeh_aot_Lorenz, 1, nullptr},
-    {SDEH_RHS_GLV16, "glv16", 16, 16, 256, sdeh_aot_GLV16, 16, sdeh_aot_GLV16G},
-    {SDEH_RHS_EXPDECAY3, "expdecay3", 3, 1, 0, sdeh_aot_ExpDecay3, 1, nullptr},
-    {SDEH_RHS_AFFINE1, "affine1", 1, 2, 0, sdeh_aot_Affine1, 1, nullptr},
-    {SDEH_RHS_LINEAR2, "linear2", 2, 4, 0, sdeh_aot_Linear2, 1, nullptr},
+    {SDEH_RHS_LORENZ, "lorenz", 3, 3, 0, sdeh_aot_Lorenz, 1, nullptr, 0, nullptr},
+    {SDEH_RHS_GLV16, "glv16", 16, 16, 256, sdeh_aot_GLV16, 16, sdeh_aot_GLV16G, 4,
+     sdeh_aot_GLV16M},
+    {SDEH_RHS_EXPDECAY3, "expdecay3", 3, 1, 0, sdeh_aot_ExpDecay3, 1, nullptr, 0, nullptr},
+    {SDEH_RHS_AFFINE1, "affine1", 1, 2, 0, sdeh_aot_Affine1, 1, nullptr, 0, nullptr},
+    {SDEH_RHS_LINEAR2, "linear2", 2, 4, 0, sdeh_aot_Linear2, 1, nullptr, 0, nullptr},
 };
 
 const Builtin* find_builtin(int64_t id) {
@@ -308,21 +313,37 @@ int user_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kaha
   return SDEH_OK;
 }
 
-// lanes: 0 = the group layout for binary64 when the right-hand side has kernels for
-// it, else one lane per trajectory; 1 = one lane per trajectory; the group size =
-// the group layout.  (Measured on GLV-16: binary64 AVern9 with saveat runs 6.8x
-// faster in the group layout, where the one-lane kernel spills; binary32 ATsit5
-// fits one lane's registers and runs 1.6x faster there, the group layout repeating
-// the controller on all 16 lanes.)
+// lanes: 0 = for binary64, the matrix-core wave layout when the right-hand side has
+// a kernel for the variant, else its group layout, else one lane per trajectory;
+// 1 = one lane per trajectory; the group size = the group layout; the wave-layout
+// size = that layout.  (Measured on GLV-16, 2^17 trajectories, AVern9 binary64 with
+// 256 saveat points: 150.6 ms one lane per trajectory (the kernel spills), 18.0 ms
+// in the 16-lane group layout, 10.8 ms in the 4-lane matrix-core layout; endpoint
+// only 4.35 / 2.86 ms.  binary32 ATsit5 fits one lane's registers and runs 1.6x
+// faster there than in the group layout, which repeats the controller on all 16
+// lanes.)
 int resolve_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kahan,
                    int lanes, KernelRef* out) {
   if (const Builtin* b = find_builtin(rhs)) {
-    if (lanes != 0 && lanes != 1 && lanes != b->group) {
+    if (lanes != 0 && lanes != 1 && lanes != b->group && !(b->mgroup > 1 && lanes == b->mgroup)) {
+      if (b->mgroup > 1)
+        return fail(c, SDEH_E_INVALID,
+                    "rhs %s runs with 1, %d or %d lanes per trajectory, not %d", b->name,
+                    b->mgroup, b->group, lanes);
       if (b->group > 1)
         return fail(c, SDEH_E_INVALID, "rhs %s runs with 1 or %d lanes per trajectory, not %d",
                     b->name, b->group, lanes);
       return fail(c, SDEH_E_INVALID, "rhs %s runs with 1 lane per trajectory, not %d", b->name,
                   lanes);
+    }
+    if (b->mgroup > 1 && (lanes == b->mgroup || (lanes == 0 && dtype == SDEH_F64))) {
+      out->aot = b->mgroup_lookup(dtype, alg, save, kahan);
+      out->lanes = b->mgroup;
+      if (out->aot) return SDEH_OK;
+      if (lanes == b->mgroup)
+        return fail(c, SDEH_E_UNSUPPORTED,
+                    "no matrix-core kernel for rhs %s, alg %d, dtype %d, save mode %d, "
+                    "precision %d", b->name, alg, dtype, save, kahan);
     }
     if ((lanes == b->group || (lanes == 0 && dtype == SDEH_F64)) && b->group > 1 &&
         b->group_lookup) {

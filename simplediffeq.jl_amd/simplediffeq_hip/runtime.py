@@ -27,8 +27,9 @@ def make_desc(alg, dtype=_abi.F64, rhs=_abi.RHS_LORENZ, n_traj=0, tspan=(0.0, 1.
               block=0, grid=0, chunk=0, lanes=0, save_layout=_abi.SAVE_POINT_MAJOR,
               span_order=0):
     """Build an sdeh_solve_desc.  Unset dt/abstol/reltol take the reference
-    defaults (0.1f0, 1f-6, 1f-3).  lanes: lanes per trajectory (0 = the
-    right-hand side's group layout where it has one).  save_layout: the layout of
+    defaults (0.1f0, 1f-6, 1f-3).  lanes: lanes per trajectory (0 = for
+    binary64 the right-hand side's matrix-core layout, else its group layout,
+    where it has one; 1 = one lane per trajectory).  save_layout: the layout of
     u_saveat/u_steps/t_steps (point- or trajectory-major, or packed: save_everystep
     rows of all trajectories back to back, no per-trajectory cap).  Keeps the
     saveat array alive on the struct."""

@@ -1,9 +1,11 @@
 """GLV-16 (16 species, interaction matrix shared by every trajectory) on the GPU
-against the CPU oracle, in both lane layouts: one lane per trajectory, and the
+against the CPU oracle, in its three lane layouts: one lane per trajectory; the
 group layout (16 adjacent lanes per trajectory, one component each, state
-exchanged through LDS).  binary64 is bit for bit against the oracle (endpoint
-and saveat, both schedules); binary32 is bit for bit between the two layouts,
-which evaluate the same operations in the same order."""
+exchanged through LDS); and, for the binary64 adaptive solvers, the matrix-core
+layout (4 lanes per trajectory 16 apart, 4 components each, A u as four
+v_mfma_f64_16x16x4f64 per wave).  binary64 is bit for bit against the oracle
+(endpoint and saveat, both schedules); binary32 is bit for bit between the one-lane
+and group layouts, which evaluate the same operations in the same order."""
 import numpy as np
 import pytest
 
@@ -26,7 +28,7 @@ def _desc(alg, n, saveat, lanes, **kw):
 @pytest.mark.parametrize("layout", [_abi.SAVE_POINT_MAJOR, _abi.SAVE_TRAJ_MAJOR],
                          ids=["point-major", "traj-major"])
 @pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT])
-@pytest.mark.parametrize("lanes", [1, 16])
+@pytest.mark.parametrize("lanes", [1, 16, 4])
 @pytest.mark.parametrize("alg,saveat", [(_abi.ALG_AVERN9, False), (_abi.ALG_AVERN9, True),
                                         (_abi.ALG_ATSIT5, False), (_abi.ALG_ATSIT5, True),
                                         (_abi.ALG_TSIT5, False)],
@@ -34,9 +36,11 @@ def _desc(alg, n, saveat, lanes, **kw):
 def test_glv16_binary64_bitexact(ctx, alg, saveat, lanes, sched, layout):
     if alg == _abi.ALG_TSIT5 and sched == _abi.SCHED_PERSISTENT:
         pytest.skip("fixed-step solvers always run one launch")
+    if alg == _abi.ALG_TSIT5 and lanes == 4:
+        pytest.skip("the matrix-core layout has adaptive kernels only")
     if layout == _abi.SAVE_TRAJ_MAJOR and not saveat:
         pytest.skip("the save layout only concerns saveat output")
-    n = 203  # not a multiple of the four trajectories a wave holds in the group layout
+    n = 203  # not a multiple of the 4 (group layout) or 16 (matrix-core) trajectories of a wave
     u0, r, A = problem(_abi.RHS_GLV16, n, seed=3)
     d = _desc(alg, n, saveat, lanes, schedule=sched, chunk=3, save_layout=layout)
     st_g, g, st_c, c = run_pair(ctx, d, u0, r, A)
@@ -76,16 +80,28 @@ def test_glv16_binary32_layouts_identical(ctx, kahan, saveat):
 def test_glv16_layout_selection(ctx):
     n = 64
     u0, r, A = problem(_abi.RHS_GLV16, n, seed=5)
-    # auto: the group layout where it is built
+    # auto, binary64: the matrix-core layout where it is built, else the group layout
     d = _desc(_abi.ALG_AVERN9, n, False, 0)
+    ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0, r, A))
+    assert ctx.stats()["lanes"] == 4
+    d = _desc(_abi.ALG_TSIT5, n, False, 0)
     ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0, r, A))
     assert ctx.stats()["lanes"] == 16
     # binary32 defaults to one lane per trajectory
     d = _desc(_abi.ALG_ATSIT5, n, False, 0, dtype=_abi.F32)
     ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0, r, A))
     assert ctx.stats()["lanes"] == 1
-    # an unsupported group size, and a group layout for a right-hand side without one
+    # the matrix-core layout on request; it has binary64 adaptive kernels only
     d = _desc(_abi.ALG_AVERN9, n, False, 4)
+    ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0, r, A))
+    assert ctx.stats()["lanes"] == 4
+    for kw in ({"alg": _abi.ALG_TSIT5}, {"alg": _abi.ALG_ATSIT5, "dtype": _abi.F32}):
+        alg = kw.pop("alg")
+        d = _desc(alg, n, False, 4, **kw)
+        with pytest.raises(runtime.SolveError, match="no matrix-core kernel"):
+            ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0, r, A))
+    # an unsupported group size, and a group layout for a right-hand side without one
+    d = _desc(_abi.ALG_AVERN9, n, False, 8)
     with pytest.raises(runtime.SolveError, match="lanes per trajectory"):
         ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0, r, A))
     dl = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=8, lanes=16)

@@ -87,13 +87,14 @@ def test_binary32_and_fixed_vern9(ctx):
     _same(h, g, BASE)
 
 
-def test_glv16_shared_block_group_layout(ctx):
+@pytest.mark.parametrize("lanes", [16, 4], ids=["group", "matrix-core"])
+def test_glv16_shared_block_lane_layouts(ctx, lanes):
     u0, r, A = inputs.glv16(N, seed=12)
     d = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_GLV16, n_traj=N, tspan=(0.0, 2.0),
                           dt=0.1, abstol=1e-8, reltol=1e-8, saveat=np.linspace(0.0, 2.0, 5),
-                          save_layout=_abi.SAVE_TRAJ_MAJOR)
+                          save_layout=_abi.SAVE_TRAJ_MAJOR, lanes=lanes)
     h, g = _pair(ctx, d, 16, u0, r, A)
-    assert ctx.stats()["lanes"] == 16
+    assert ctx.stats()["lanes"] == lanes
     _same(h, g, BASE + ("u_saveat",))
 
 
