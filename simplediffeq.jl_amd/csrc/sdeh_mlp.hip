@@ -54,9 +54,72 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Feature sources and output sinks of the forward pass.
+struct XFromBuffer {  // x [n][6] FP32
+  const float* __restrict__ x;
+  __device__ void operator()(int64_t smp, bool valid, float (&xi)[NIN]) const {
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) xi[j] = valid ? x[smp * NIN + j] : 0.f;
+  }
+};
+
+// The Lorenz cost model's features from binary64 ensemble inputs (component-major
+// p[3][n], u0[3][n], tspan[2][n]), each computed in binary64 in this order and then
+// rounded to FP32: (p_c - c[2c]) / c[2c+1] for c = 0, 1, 2; (log(tf - t0) - c[6]) /
+// c[7]; sqrt((u0_0^2 + u0_1^2) + u0_2^2) / sqrt(3) * 2 - 1; u0_2.
+struct XLorenz {
+  const double* __restrict__ p;
+  const double* __restrict__ u0;
+  const double* __restrict__ ts;
+  int64_t n;
+  double c[8];
+  __device__ void operator()(int64_t smp, bool valid, float (&xi)[NIN]) const {
+    if (!valid) {
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) xi[j] = 0.f;
+      return;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) xi[k] = (float)((p[k * n + smp] - c[2 * k]) / c[2 * k + 1]);
+    xi[3] = (float)((log(ts[n + smp] - ts[smp]) - c[6]) / c[7]);
+    const double a = u0[smp], b = u0[n + smp], z = u0[2 * n + smp];
+    const double r2 = (a * a + b * b) + z * z;
+    xi[4] = (float)(sqrt(r2) / 1.7320508075688772 * 2.0 - 1.0);
+    xi[5] = (float)z;
+  }
+};
+
+struct YToBuffer {  // y [n][2] FP32
+  float* y;
+  __device__ void operator()(int64_t smp, float o0, float o1) const {
+    y[smp * NOUT] = o0;
+    y[smp * NOUT + 1] = o1;
+  }
+};
+
+// Binning step: pred = y * std + mean (FP32, two roundings), key = -pred_0 (an
+// ascending stable sort of the keys hands out the largest predicted costs first),
+// dt0 = exp(pred_1) in FP32 widened to binary64; pred is optional.
+struct YPlan {
+  float* key;
+  double* dt0;
+  float* pred;
+  float m0, m1, s0, s1;
+  __device__ void operator()(int64_t smp, float o0, float o1) const {
+    const float a0 = o0 * s0, a1 = o1 * s1;
+    const float p0 = a0 + m0, p1 = a1 + m1;
+    key[smp] = -p0;
+    dt0[smp] = (double)expf(p1);
+    if (pred) {
+      pred[smp * NOUT] = p0;
+      pred[smp * NOUT + 1] = p1;
+    }
+  }
+};
+
+template <class Src, class Sink>
 __global__ void __launch_bounds__(256) mlp_forward_mfma(const float* __restrict__ params,
-                                                        const float* __restrict__ x, float* y,
-                                                        int64_t n) {
+                                                        Src src, Sink sink, int64_t n) {
   __shared__ float sw[NPARAM];
   __shared__ float s_h2[4][32 * H2P];
   load_params(sw, params);
@@ -75,8 +138,7 @@ __global__ void __launch_bounds__(256) mlp_forward_mfma(const float* __restrict_
     const int64_t smp = tile * 32 + li;
     const bool valid = smp < n;
     float xi[NIN];
-#pragma unroll
-    for (int j = 0; j < NIN; ++j) xi[j] = valid ? x[smp * NIN + j] : 0.f;
+    src(smp, valid, xi);
     float h1[32];
 #pragma unroll
     for (int m = 0; m < 32; ++m) {
@@ -110,10 +172,7 @@ __global__ void __launch_bounds__(256) mlp_forward_mfma(const float* __restrict_
     }
     o0 += __shfl_xor(o0, 32);
     o1 += __shfl_xor(o1, 32);
-    if (q == 0 && valid) {
-      y[smp * NOUT] = o0 + sw[OFF_B3];
-      y[smp * NOUT + 1] = o1 + sw[OFF_B3 + 1];
-    }
+    if (q == 0 && valid) sink(smp, o0 + sw[OFF_B3], o1 + sw[OFF_B3 + 1]);
   }
 }
 
@@ -363,12 +422,50 @@ __global__ void mlp_adam(float* p, const float* __restrict__ g, float* m, float*
 
 }  // namespace
 
+// Resident blocks of a forward kernel on the current device (LDS bounds it: 52 KB
+// per block), the grid of a forward launch: the blocks then loop over tiles with no
+// partial last round of blocks.
+template <class Src, class Sink>
+static int64_t forward_grid(int64_t n) {
+  static int resident = 0;  // per process; every MI355X has the same count
+  if (resident == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&mlp_forward_mfma<Src, Sink>), 256, 0) !=
+            hipSuccess ||
+        cus <= 0 || per_cu <= 0)
+      return -1;
+    resident = cus * per_cu;
+  }
+  const int64_t want = ((n + 31) / 32 + 3) / 4;  // four waves of 32-sample tiles per block
+  return want < resident ? want : resident;
+}
+
 extern "C" SDEH_HIDDEN int sdeh_mlp_forward_impl(const float* params, const float* x, float* y,
                                                  int64_t n, hipStream_t s) {
   if (n == 0) return 0;
-  int64_t g = ((n + 31) / 32 + 3) / 4;  // four waves of 32-sample tiles per block
-  if (g > 1024) g = 1024;
-  hipLaunchKernelGGL(mlp_forward_mfma, dim3((unsigned)g), dim3(256), 0, s, params, x, y, n);
+  const int64_t g = forward_grid<XFromBuffer, YToBuffer>(n);
+  if (g <= 0) return -2;
+  hipLaunchKernelGGL((mlp_forward_mfma<XFromBuffer, YToBuffer>), dim3((unsigned)g), dim3(256), 0,
+                     s, params, XFromBuffer{x}, YToBuffer{y}, n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// The Lorenz cost model's binning step (features, forward, de-normalisation, sort
+// key, dt0) in one launch; c[8] are the feature constants of XLorenz.
+extern "C" SDEH_HIDDEN int sdeh_mlp_plan_lorenz_impl(const float* params, const double* p,
+                                                     const double* u0, const double* tspan,
+                                                     int64_t n, const double* c, float m0,
+                                                     float m1, float s0, float s1, float* key,
+                                                     double* dt0, float* pred, hipStream_t s) {
+  if (n == 0) return 0;
+  const int64_t g = forward_grid<XLorenz, YPlan>(n);
+  if (g <= 0) return -2;
+  XLorenz src{p, u0, tspan, n, {c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7]}};
+  hipLaunchKernelGGL((mlp_forward_mfma<XLorenz, YPlan>), dim3((unsigned)g), dim3(256), 0, s,
+                     params, src, YPlan{key, dt0, pred, m0, m1, s0, s1}, n);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

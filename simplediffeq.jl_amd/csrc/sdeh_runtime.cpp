@@ -52,6 +52,9 @@ SDEH_HIDDEN hipError_t sdeh_util_pack_offsets(const int32_t*, int32_t, int64_t, 
                                               int64_t*, int32_t*, unsigned long long*,
                                               hipStream_t);
 SDEH_HIDDEN int sdeh_mlp_forward_impl(const float*, const float*, float*, int64_t, hipStream_t);
+SDEH_HIDDEN int sdeh_mlp_plan_lorenz_impl(const float*, const double*, const double*,
+                                          const double*, int64_t, const double*, float, float,
+                                          float, float, float*, double*, float*, hipStream_t);
 SDEH_HIDDEN int sdeh_mlp_loss_grad_impl(const float*, const float*, const float*, int64_t, float*,
                                         float*, float*, hipStream_t);
 SDEH_HIDDEN size_t sdeh_mlp_grad_scratch_bytes(int64_t);
@@ -1332,6 +1335,28 @@ int sdeh_mlp_forward(sdeh_ctx* c, const float* params, const float* x, float* y,
   if (r) return r;
   r = sdeh_mlp_forward_impl(params, x, y, n, stream ? (hipStream_t)stream : c->stream);
   if (r) return fail(c, r, "mlp forward launch failed");
+  return SDEH_OK;
+}
+
+int sdeh_mlp_plan_lorenz(sdeh_ctx* c, const float* params, const double* p, const double* u0,
+                         const double* tspan, int64_t n, const double* feat, const float* y_norm,
+                         float* key, double* dt0, float* pred, void* stream) {
+  if (!c || !params || !p || !u0 || !tspan || !feat || !y_norm || !key || !dt0 || n < 0)
+    return SDEH_E_INVALID;
+  for (int k = 0; k < 8; k += 2)
+    if (!(feat[k + 1] != 0.0)) return fail(c, SDEH_E_INVALID, "feature scale %d is zero", k / 2);
+  int r = check_dev(c, params, SDEH_MLP_NPARAM * 4, "params");
+  if (!r) r = check_dev(c, p, (size_t)n * 3 * 8, "p");
+  if (!r) r = check_dev(c, u0, (size_t)n * 3 * 8, "u0");
+  if (!r) r = check_dev(c, tspan, (size_t)n * 2 * 8, "tspan");
+  if (!r) r = check_dev(c, key, (size_t)n * 4, "key");
+  if (!r) r = check_dev(c, dt0, (size_t)n * 8, "dt0");
+  if (!r && pred) r = check_dev(c, pred, (size_t)n * SDEH_MLP_NOUT * 4, "pred");
+  if (r) return r;
+  r = sdeh_mlp_plan_lorenz_impl(params, p, u0, tspan, n, feat, y_norm[0], y_norm[1], y_norm[2],
+                                y_norm[3], key, dt0, pred,
+                                stream ? (hipStream_t)stream : c->stream);
+  if (r) return fail(c, r, "mlp plan launch failed");
   return SDEH_OK;
 }
 

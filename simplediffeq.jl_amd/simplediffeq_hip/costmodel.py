@@ -30,6 +30,8 @@ NPARAM = _abi.MLP_NPARAM
 # rho ~ U(0,28), beta ~ U(2,3), u0 ~ U(-1,1)^3, t_end ~ LogUniform(0.5,10))
 _LOG_T_MID = 0.5 * (math.log(0.5) + math.log(10.0))
 _LOG_T_HALF = 0.5 * (math.log(10.0) - math.log(0.5))
+# offset, scale of the features (p_0, p_1, p_2, log span) as features() applies them
+FEAT_LORENZ = (10.0, 2.0, 14.0, 14.0, 2.5, 0.5, _LOG_T_MID, _LOG_T_HALF)
 
 
 def features(p, u0, tspan):
@@ -174,6 +176,25 @@ class CostModel:
                 torch.as_tensor(self.y_mean, device=self.dev)
             order = torch.argsort(-pred[:, 0], stable=True)
             dt0 = torch.exp(pred[:, 1]).to(torch.float64)
+        return pred, order, dt0
+
+    def plan_lorenz(self, p, u0, tspan, stream):
+        """plan_device on the raw binary64 inputs of a Lorenz ensemble (device tensors
+        p (3, n), u0 (3, n), tspan (2, n)): features, forward pass, de-normalisation,
+        sort key and dt0 in one library launch (sdeh_mlp_plan_lorenz), then the stable
+        device sort.  The same (pred, order, dt0) as
+        plan_device(features_device(p, u0, tspan), stream)."""
+        torch = self.torch
+        n = p.shape[1]
+        with torch.cuda.stream(stream):
+            key = torch.empty(n, dtype=torch.float32, device=self.dev)
+            dt0 = torch.empty(n, dtype=torch.float64, device=self.dev)
+            pred = torch.empty((n, NOUT), dtype=torch.float32, device=self.dev)
+            norm = (self.y_mean[0], self.y_mean[1], self.y_std[0], self.y_std[1])
+            self.ctx.mlp_plan_lorenz(self.params.data_ptr(), p.data_ptr(), u0.data_ptr(),
+                                     tspan.data_ptr(), n, FEAT_LORENZ, norm, key.data_ptr(),
+                                     dt0.data_ptr(), pred.data_ptr(), stream=stream.cuda_stream)
+            order = torch.argsort(key, stable=True)
         return pred, order, dt0
 
 

@@ -273,6 +273,16 @@ class Context:
     def mlp_forward(self, params, x, y, n, stream=None):
         self._check(self.lib.sdeh_mlp_forward(self.h, params, x, y, int(n), ctypes.c_void_p(stream)))
 
+    def mlp_plan_lorenz(self, params, p, u0, tspan, n, feat, y_norm, key, dt0, pred=None,
+                        stream=None):
+        """sdeh_mlp_plan_lorenz on device pointers; feat: 8 float64, y_norm: 4 float32
+        (host sequences, kept alive for the call)."""
+        fa = (ctypes.c_double * 8)(*[float(v) for v in feat])
+        ya = (ctypes.c_float * 4)(*[float(v) for v in y_norm])
+        self._check(self.lib.sdeh_mlp_plan_lorenz(
+            self.h, params, p, u0, tspan, int(n), ctypes.cast(fa, ctypes.c_void_p),
+            ctypes.cast(ya, ctypes.c_void_p), key, dt0, pred, ctypes.c_void_p(stream)))
+
     def mlp_loss_grad(self, params, x, y, n, loss, grad, stream=None):
         self._check(self.lib.sdeh_mlp_loss_grad(self.h, params, x, y, int(n), loss, grad,
                                                 ctypes.c_void_p(stream)))

@@ -50,3 +50,40 @@ def test_training_reduces_loss(ctx):
     assert hist[-1][1] < 0.1 * hist[0][1]
     pred = cm.predict(x[:1000])
     assert np.mean((pred - y[:1000]) ** 2) < 0.1 * np.var(y[:1000])
+
+
+def test_fused_lorenz_plan_equals_separate_steps(ctx):
+    """sdeh_mlp_plan_lorenz (features, forward, de-normalisation, sort key, dt0 in
+    one launch) against the same step built from torch features, the plain forward
+    pass and torch operations: predictions, hand-out order and dt0 bit for bit; the
+    features against the numpy definition through the forward pass."""
+    torch = pytest.importorskip("torch")
+    from simplediffeq_hip import inputs, runtime
+    n = 100_003  # not a multiple of a block's 128 samples
+    u0, p, ts = inputs.lorenz_costmodel(n, seed=9)
+    cm = costmodel.CostModel(ctx, params=costmodel.init_params(5))
+    cm.y_mean = np.array([3.1, -2.2], dtype=np.float32)
+    cm.y_std = np.array([0.7, 1.3], dtype=np.float32)
+    dev = torch.device("cuda:0")
+    pd, u0d, tsd = (torch.as_tensor(a, device=dev) for a in (p, u0, ts))
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        xd = costmodel.features_device(pd, u0d, tsd)
+    pred_u, order_u, dt0_u = cm.plan_device(xd, stream)
+    pred_f, order_f, dt0_f = cm.plan_lorenz(pd, u0d, tsd, stream)
+    torch.cuda.synchronize()
+    assert torch.equal(pred_f, pred_u)
+    assert torch.equal(order_f, order_u)
+    assert torch.equal(dt0_f, dt0_u)
+    # the numpy feature definition gives the same predictions (host features)
+    assert np.array_equal(cm.predict(costmodel.features(p, u0, ts)), pred_f.cpu().numpy())
+    # validation: a zero feature scale, and an empty ensemble
+    key = torch.empty(4, dtype=torch.float32, device=dev)
+    dt0 = torch.empty(4, dtype=torch.float64, device=dev)
+    bad = list(costmodel.FEAT_LORENZ)
+    bad[3] = 0.0
+    with pytest.raises(runtime.SolveError, match="feature scale 1"):
+        ctx.mlp_plan_lorenz(cm.params.data_ptr(), pd.data_ptr(), u0d.data_ptr(), tsd.data_ptr(),
+                            4, bad, (0, 0, 1, 1), key.data_ptr(), dt0.data_ptr())
+    ctx.mlp_plan_lorenz(cm.params.data_ptr(), pd.data_ptr(), u0d.data_ptr(), tsd.data_ptr(), 0,
+                        costmodel.FEAT_LORENZ, (0, 0, 1, 1), key.data_ptr(), dt0.data_ptr())
