@@ -108,3 +108,32 @@ def test_glv16_layout_selection(ctx):
     ul, pl, _ = problem(_abi.RHS_LORENZ, 8)
     with pytest.raises(runtime.SolveError, match="lane per trajectory"):
         ctx.solve(dl, runtime.EnsembleBuffers(dl, 3, ul, pl))
+
+
+@pytest.mark.parametrize("lanes", [16, 4], ids=["group", "matrix-core"])
+@pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT])
+def test_glv16_spans_dt0_order_and_retcodes(ctx, lanes, sched):
+    """Per-trajectory spans (handed out longest first), per-trajectory dt0, a caller's
+    order, and trajectories that stop early (maxiters, dt < dtmin) in the group and
+    matrix-core layouts: every output bit for bit the oracle's."""
+    n = 333
+    u0, r, A = problem(_abi.RHS_GLV16, n, seed=21)
+    rng = np.random.default_rng(21)
+    ts = np.stack([rng.uniform(0.0, 1.0, n), rng.uniform(1.5, 6.0, n)])
+    dt0 = rng.uniform(0.01, 0.3, n)
+    dt0[::37] = 1e-16  # below AVern9's dtmin: ends at once with RET_DTMIN
+    d = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_GLV16, n_traj=n, dt=0.1, abstol=1e-9,
+                          reltol=1e-9, lanes=lanes, schedule=sched, chunk=5, maxiters=12)
+    st_g, g, st_c, c = run_pair(ctx, d, u0, r, A, tspan=ts, dt0=dt0)
+    assert st_g == st_c == _abi.E_DTMIN
+    assert ctx.stats()["lanes"] == lanes
+    assert_bit_identical(g, c, FIELDS)
+    ret = g.numpy("retcode")
+    assert (ret[::37] == _abi.RET_DTLESSTHANMIN).all() and (ret == _abi.RET_MAXITERS).any()
+    # a caller's hand-out order changes nothing
+    order = rng.permutation(n).astype(np.int64)
+    d2 = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_GLV16, n_traj=n, dt=0.1, abstol=1e-9,
+                           reltol=1e-9, lanes=lanes, schedule=sched, chunk=5, maxiters=12)
+    h = runtime.EnsembleBuffers(d2, 16, u0, r, A, ts, dt0, order)
+    assert ctx.solve(d2, h, allow=(_abi.E_DTMIN,)) == _abi.E_DTMIN
+    assert_bit_identical(h, c, FIELDS)
