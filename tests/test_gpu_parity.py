@@ -111,3 +111,35 @@ def test_device_memory_path_matches_host(ctx):
     st_g, g, st_c, c = run_pair(ctx, d, u0, p, backend="torch")
     assert st_g == 0 and st_c == 0
     assert_bit_identical(g, c)
+
+
+@pytest.mark.parametrize("alg", [_abi.ALG_ATSIT5, _abi.ALG_AVERN7, _abi.ALG_AVERN9],
+                         ids=["atsit5", "avern7", "avern9"])
+def test_controller_range_edges_bitexact(ctx, alg):
+    """u' = lambda u with lambda spanning the error estimate's whole range, mixed in
+    every wave: EEst exactly 0 (lambda = 0), subnormal or underflowing error sums
+    (|lambda| ~ 1e-160 .. 1e-200), sums near the edges of the range-checked controller
+    ([2^-700, 2^700]: |lambda| ~ 1e-100 .. 1e-110 puts it on either side of the lower
+    edge), ordinary ones, and exploding ones that overflow and stop with a NaN error
+    estimate.  The controller then takes its range-checked and its IEEE path in the
+    same wave; every output bit for bit the oracle's."""
+    n = 640
+    rng = np.random.default_rng(31)
+    lam = np.concatenate([
+        np.zeros(64),                                   # EEst = 0: dt grows by qmax
+        -np.exp(rng.uniform(np.log(1e-200), np.log(1e-160), 128)),
+        -np.exp(rng.uniform(np.log(1e-110), np.log(1e-100), 64)),  # s near 2^-700
+        -rng.uniform(0.1, 3.0, 256),                    # ordinary
+        np.exp(rng.uniform(np.log(1e100), np.log(1e160), 64)),     # explode: NaN stop
+        rng.uniform(1e300, 1e305, 64),                  # overflow at once: NaN stop
+    ])
+    perm = rng.permutation(n)
+    p = lam[perm][None, :]
+    u0 = rng.uniform(0.5, 2.0, (3, n))
+    d = runtime.make_desc(alg, rhs=_abi.RHS_EXPDECAY3, n_traj=n, tspan=(0.0, 1.0), dt=0.1,
+                          abstol=1e-6, reltol=1e-3, maxiters=200)
+    st_g, g, st_c, c = run_pair(ctx, d, u0, p)
+    assert st_g == st_c
+    assert_bit_identical(g, c)
+    ret = c.numpy("retcode")[np.argsort(perm)]
+    assert (ret[:512] == _abi.RET_DEFAULT).all() and (ret[512:] == _abi.RET_NAN).all()
