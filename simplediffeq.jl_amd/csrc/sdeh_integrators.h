@@ -191,15 +191,22 @@ __device__ __forceinline__ void group_gather(T v, T (&all)[G]) {
   for (int j = 0; j < G; ++j) all[j] = xbuf[base + j];
 }
 
+// Wave layout: this wave's share (NW values) of a block's LDS exchange buffer
+// (blocks of at most 256 threads).  One buffer per NW; a kernel uses one NW for all
+// its exchanges, which never overlap.
+template <class T, int NW>
+__device__ __forceinline__ T* wave_lds() {
+  __shared__ T buf[4 * NW];
+  return buf + (threadIdx.x >> 6) * NW;
+}
+
 // Wave layout: all[c] = component c (c = 0 .. N*G-1) of this lane's trajectory,
-// for a wave in uniform control flow.  LDS [wave][slot][component], 16 x 16 values
-// per wave of a block of at most 256 threads.
-template <class T, class R>
+// for a wave in uniform control flow.  LDS [slot][component] in the wave's share.
+template <class T, class R, int NW>
 __device__ __forceinline__ void wave_gather(const T (&v)[R::N], T (&all)[R::N * R::G]) {
   constexpr int FULL = R::N * R::G, SLOTS = 64 / R::G;
-  static_assert(FULL * SLOTS <= 256, "one wave's share of the exchange buffer");
-  __shared__ T wbuf[4][256];
-  T* w = wbuf[threadIdx.x >> 6] + (threadIdx.x & (unsigned)(SLOTS - 1)) * FULL;
+  static_assert(FULL * SLOTS <= NW, "one wave's share of the exchange buffer");
+  T* w = wave_lds<T, NW>() + (threadIdx.x & (unsigned)(SLOTS - 1)) * FULL;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -285,7 +292,7 @@ __device__ __forceinline__ double stage_t(double t, double c, double dt) {
 // = uprev + (terms), so a NaN in uprev is a NaN in u, and one comparison select
 // that keeps |u| unless |u| < |uprev| gives the same value (a NaN only through u,
 // and a NaN makes the norm NaN whatever its payload).
-template <class T, class R>
+template <class T, class R, int NW = 256>
 __device__ __forceinline__ T err_sumsq(T (&tmp)[R::N], const T (&uprev)[R::N],
                                        const T (&u)[R::N], T abstol, T reltol) {
   constexpr int N = R::N;
@@ -298,7 +305,7 @@ __device__ __forceinline__ T err_sumsq(T (&tmp)[R::N], const T (&uprev)[R::N],
   if constexpr (IsWave<R>::value) {
     constexpr int FULL = N * R::G;
     T all[FULL];
-    wave_gather<T, R>(tmp, all);
+    wave_gather<T, R, NW>(tmp, all);
     T s = all[0] * all[0];
 #pragma unroll
     for (int c = 1; c < FULL; ++c) s = s + all[c] * all[c];
@@ -708,10 +715,11 @@ __device__ __forceinline__ void vern9_extra(const T (&uprev)[R::N], const V9S<T,
 #undef SDEH_M9
 }
 
-// Vern9 dense output (bθs at verner_tableaus.jl:1362-1398).
-template <class T, int N>
-__device__ __forceinline__ void vern9_dense(T (&out)[N], const T (&uprev)[N], T h, T th,
-                                            const V9S<T, N>& S, const T (&x)[10][N]) {
+// Vern9 dense output (bθs at verner_tableaus.jl:1362-1398): the 19 weights of θ,
+// then out = uprev + h * (w_1 k1 + w_8 k8 + ... + w_26 k26).
+constexpr int V9W = 19;
+template <class T>
+__device__ __forceinline__ void vern9_weights(T th, T (&w)[V9W]) {
   const T r1[9] = {V9I(r011), V9I(r012), V9I(r013), V9I(r014), V9I(r015),
                    V9I(r016), V9I(r017), V9I(r018), V9I(r019)};
 #define SDEH_R9(j)                                                                      \
@@ -721,12 +729,31 @@ __device__ __forceinline__ void vern9_dense(T (&out)[N], const T (&uprev)[N], T 
   SDEH_R9(15) SDEH_R9(17) SDEH_R9(18) SDEH_R9(19) SDEH_R9(20) SDEH_R9(21) SDEH_R9(22)
   SDEH_R9(23) SDEH_R9(24) SDEH_R9(25) SDEH_R9(26)
 #undef SDEH_R9
-  comb(out, uprev, h, poly1(th, r1), S.s[0], poly2(th, r08), S.s[1], poly2(th, r09), S.s[2],
-       poly2(th, r10), S.s[3], poly2(th, r11), S.s[4], poly2(th, r12), S.s[5], poly2(th, r13),
-       S.s[6], poly2(th, r14), S.s[7], poly2(th, r15), S.s[8], poly2(th, r17), x[0],
-       poly2(th, r18), x[1], poly2(th, r19), x[2], poly2(th, r20), x[3], poly2(th, r21), x[4],
-       poly2(th, r22), x[5], poly2(th, r23), x[6], poly2(th, r24), x[7], poly2(th, r25), x[8],
-       poly2(th, r26), x[9]);
+  w[0] = poly1(th, r1);
+  w[1] = poly2(th, r08); w[2] = poly2(th, r09); w[3] = poly2(th, r10); w[4] = poly2(th, r11);
+  w[5] = poly2(th, r12); w[6] = poly2(th, r13); w[7] = poly2(th, r14); w[8] = poly2(th, r15);
+  w[9] = poly2(th, r17); w[10] = poly2(th, r18); w[11] = poly2(th, r19);
+  w[12] = poly2(th, r20); w[13] = poly2(th, r21); w[14] = poly2(th, r22);
+  w[15] = poly2(th, r23); w[16] = poly2(th, r24); w[17] = poly2(th, r25);
+  w[18] = poly2(th, r26);
+}
+
+template <class T, int N>
+__device__ __forceinline__ void vern9_combine(T (&out)[N], const T (&uprev)[N], T h,
+                                              const T (&w)[V9W], const V9S<T, N>& S,
+                                              const T (&x)[10][N]) {
+  comb(out, uprev, h, w[0], S.s[0], w[1], S.s[1], w[2], S.s[2], w[3], S.s[3], w[4], S.s[4],
+       w[5], S.s[5], w[6], S.s[6], w[7], S.s[7], w[8], S.s[8], w[9], x[0], w[10], x[1],
+       w[11], x[2], w[12], x[3], w[13], x[4], w[14], x[5], w[15], x[6], w[16], x[7],
+       w[17], x[8], w[18], x[9]);
+}
+
+template <class T, int N>
+__device__ __forceinline__ void vern9_dense(T (&out)[N], const T (&uprev)[N], T h, T th,
+                                            const V9S<T, N>& S, const T (&x)[10][N]) {
+  T w[V9W];
+  vern9_weights(th, w);
+  vern9_combine(out, uprev, h, w, S, x);
 }
 #undef V9
 #undef V9X
@@ -907,6 +934,9 @@ __device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
   constexpr int P = R::P;
   constexpr int SLOTS = 64 / R::G;  // trajectories per wave
   constexpr bool V9 = (ALG == ALG_AVERN9);
+  // LDS exchange per wave: the error norm's components, and (AVern9 with saveat) the
+  // dense-output weights of R::G points per trajectory.
+  constexpr int NWL = (V9 && SAVE == SAVE_AT) ? SLOTS * R::G * V9W : 256;
   constexpr double DTMIN = V9 ? (double)1.0e-7f : 1.0e-14;  // gpuvern9.jl:466
   constexpr double SNAP = DTMIN;                            // gpuvern9.jl:591
   // build_adaptive_controller_cache(T), src/SimpleDiffEq.jl:67-77
@@ -1025,7 +1055,7 @@ __device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
     }
 #pragma unroll
     for (int c = 0; c < N; ++c) err[c] = dt * err[c];
-    const T ssq = err_sumsq<T, R>(err, uprev, u, abstol, reltol);
+    const T ssq = err_sumsq<T, R, NWL>(err, uprev, u, abstol, reltol);
     CtrlOut<T> co;
     if constexpr (Ops<T>::kFast) {
       co = controller<T, R, true>(ssq, dt, qold_pow, ck);
@@ -1081,14 +1111,34 @@ __device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
             T x[10][N];
             vern9_extra<T, R>(uprev, S9, x, pl, ps, told, dtold);  // told + c*dtold
             if (saves) {
+              // The weights depend on the point only: the G lanes of a trajectory take
+              // G of its points at a time, lane q the weights of point cur + q, and
+              // share them through LDS; every lane then forms its components of each
+              // point in order.  Same values as vern9_dense point by point.
+              constexpr int PB = R::G;
+              T* wl = wave_lds<T, NWL>() + (lane & (SLOTS - 1)) * (PB * V9W);
+              const int q = lane_comp<R>();
               while (cur < a.n_saveat && ts_next <= t) {
-                const double ts_after = save_time(a, cur + 1);
-                const T th = (T)((ts_next - told) / (double)dtold);
-                T o[N];
-                vern9_dense(o, uprev, dtold, th, S9, x);
-                store_save<T, R>(a, a.u_saveat, cur, a.n_saveat, traj, o);
-                ++cur;
-                ts_next = ts_after;
+                const int64_t pq = cur + q;
+                const double tq = pq < a.n_saveat ? a.saveat[pq] : ts_next;
+                T w[V9W];
+                vern9_weights((T)((tq - told) / (double)dtold), w);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();  // the previous batch's reads are done
+#pragma unroll
+                for (int i = 0; i < V9W; ++i) wl[q * V9W + i] = w[i];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (int m = 0; m < PB && cur < a.n_saveat && ts_next <= t; ++m) {
+                  // the weights are read from LDS where the combination uses them
+                  const T(&wm)[V9W] = *reinterpret_cast<const T(*)[V9W]>(wl + m * V9W);
+                  T o[N];
+                  vern9_combine(o, uprev, dtold, wm, S9, x);
+                  store_save<T, R>(a, a.u_saveat, cur, a.n_saveat, traj, o);
+                  ++cur;
+                  ts_next = save_time(a, cur);
+                }
               }
             }
           } else {
