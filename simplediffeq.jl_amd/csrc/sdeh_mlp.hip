@@ -427,18 +427,18 @@ __global__ void mlp_adam(float* p, const float* __restrict__ g, float* m, float*
 // partial last round of blocks.
 template <class Src, class Sink>
 static int64_t forward_grid(int64_t n) {
-  static int resident = 0;  // per process; every MI355X has the same count
-  if (resident == 0) {
+  // per process (initialised once, thread-safe); every MI355X has the same count
+  static const int resident = [] {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void*>(&mlp_forward_mfma<Src, Sink>), 256, 0) !=
-            hipSuccess ||
-        cus <= 0 || per_cu <= 0)
-      return -1;
-    resident = cus * per_cu;
-  }
+            hipSuccess)
+      return 0;
+    return cus > 0 && per_cu > 0 ? cus * per_cu : 0;
+  }();
+  if (resident <= 0) return -1;
   const int64_t want = ((n + 31) / 32 + 3) / 4;  // four waves of 32-sample tiles per block
   return want < resident ? want : resident;
 }

@@ -1329,6 +1329,7 @@ int sdeh_autotune(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, sdeh
 int sdeh_mlp_forward(sdeh_ctx* c, const float* params, const float* x, float* y, int64_t n,
                      void* stream) {
   if (!c || !params || !x || !y || n < 0) return SDEH_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
   int r = check_dev(c, params, SDEH_MLP_NPARAM * 4, "params");
   if (!r) r = check_dev(c, x, (size_t)n * SDEH_MLP_NIN * 4, "x");
   if (!r) r = check_dev(c, y, (size_t)n * SDEH_MLP_NOUT * 4, "y");
@@ -1345,6 +1346,7 @@ int sdeh_mlp_plan_lorenz(sdeh_ctx* c, const float* params, const double* p, cons
     return SDEH_E_INVALID;
   for (int k = 0; k < 8; k += 2)
     if (!(feat[k + 1] != 0.0)) return fail(c, SDEH_E_INVALID, "feature scale %d is zero", k / 2);
+  HIPCHK(c, hipSetDevice(c->device));
   int r = check_dev(c, params, SDEH_MLP_NPARAM * 4, "params");
   if (!r) r = check_dev(c, p, (size_t)n * 3 * 8, "p");
   if (!r) r = check_dev(c, u0, (size_t)n * 3 * 8, "u0");
@@ -1363,6 +1365,7 @@ int sdeh_mlp_plan_lorenz(sdeh_ctx* c, const float* params, const double* p, cons
 int sdeh_mlp_loss_grad(sdeh_ctx* c, const float* params, const float* x, const float* y,
                        int64_t n, float* loss, float* grad, void* stream) {
   if (!c || !params || !x || !y || !loss || !grad || n < 1) return SDEH_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
   int r = check_dev(c, params, SDEH_MLP_NPARAM * 4, "params");
   if (!r) r = check_dev(c, x, (size_t)n * SDEH_MLP_NIN * 4, "x");
   if (!r) r = check_dev(c, y, (size_t)n * SDEH_MLP_NOUT * 4, "y");
@@ -1384,6 +1387,7 @@ int sdeh_mlp_loss_grad(sdeh_ctx* c, const float* params, const float* x, const f
 int sdeh_mlp_adam(sdeh_ctx* c, float* params, const float* grad, float* m, float* v, float lr,
                   float beta1, float beta2, float eps, int32_t t, void* stream) {
   if (!c || !params || !grad || !m || !v || t < 1) return SDEH_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
   int r = check_dev(c, params, SDEH_MLP_NPARAM * 4, "params");
   if (!r) r = check_dev(c, grad, SDEH_MLP_NPARAM * 4, "grad");
   if (!r) r = check_dev(c, m, SDEH_MLP_NPARAM * 4, "m");
