@@ -797,17 +797,31 @@ __device__ __forceinline__ void store_save_time(const sdeh_kargs& a, int64_t row
 // save_everystep output of one trajectory.  Fixed rows: point k goes to row k of
 // the trajectory's max_saves rows.  Packed (a.packed): point k goes to row
 // base + k of the shared output while k < lim, where base = steps_off[traj] and lim
-// = n_steps[traj] come from the counting pass and the offsets scan.
+// = n_steps[traj] come from the counting pass and the offsets scan.  Packed rows
+// are held back in registers and written PK_ROWS at a time (consecutive rows:
+// contiguous bytes, so L2 sees whole runs of a trajectory's lines instead of one
+// 24-byte row per step, lines it would otherwise write back partly filled); the
+// trajectory's end writes the rest.
+#ifndef SDEH_PK_ROWS
+#define SDEH_PK_ROWS 4
+#endif
+constexpr int PK_ROWS = SDEH_PK_ROWS;
+
+template <class T, int N>
 struct StepSink {
   int64_t base;  // packed: first row of the trajectory
   int32_t lim;   // rows available (packed) or max_saves
   int32_t n;     // points offered
-  int32_t kept;  // points stored
+  int32_t kept;  // points stored (written or held back)
+  int32_t held;  // packed: rows held back, the last `held` of bt/bu
+  double bt[PK_ROWS];
+  T bu[PK_ROWS][N];
 };
 
-template <class R>
-__device__ __forceinline__ void sink_start(const sdeh_kargs& a, StepSink& s, int64_t traj) {
-  s.n = s.kept = 0;
+template <class R, class T, int N>
+__device__ __forceinline__ void sink_start(const sdeh_kargs& a, StepSink<T, N>& s,
+                                           int64_t traj) {
+  s.n = s.kept = s.held = 0;
   if (a.packed) {
     s.base = a.steps_off[traj];
     s.lim = a.n_steps[traj];
@@ -817,28 +831,54 @@ __device__ __forceinline__ void sink_start(const sdeh_kargs& a, StepSink& s, int
   }
 }
 
+// Packed: write the rows held back (rows kept - held .. kept - 1).
 template <class T, class R>
-__device__ __forceinline__ void save_step(const sdeh_kargs& a, StepSink& s, int64_t traj,
-                                          double t, const T (&u)[R::N]) {
+__device__ __forceinline__ void flush_steps(const sdeh_kargs& a, StepSink<T, R::N>& s) {
+  if (!a.packed || s.held == 0) return;
+  constexpr int FULL = R::N * R::G;
+  const bool lead = lane_comp<R>() == 0;
+#pragma unroll
+  for (int k = 0; k < PK_ROWS; ++k) {
+    if (k >= PK_ROWS - s.held) {
+      const int64_t row = s.base + s.kept - (PK_ROWS - k);
+      T* o = (T*)a.u_steps + row * FULL;
+#pragma unroll
+      for (int c = 0; c < R::N; ++c) o[comp_idx<R>(c)] = s.bu[k][c];
+      if (lead) a.t_steps[row] = s.bt[k];
+    }
+  }
+  s.held = 0;
+}
+
+template <class T, class R>
+__device__ __forceinline__ void save_step(const sdeh_kargs& a, StepSink<T, R::N>& s,
+                                          int64_t traj, double t, const T (&u)[R::N]) {
   const bool lead = lane_comp<R>() == 0;
   ++s.n;
   if (s.kept >= s.lim) return;
   if (a.packed) {
-    constexpr int FULL = R::N * R::G;
-    const int64_t row = s.base + s.kept;
-    T* o = (T*)a.u_steps + row * FULL;
 #pragma unroll
-    for (int c = 0; c < R::N; ++c) o[comp_idx<R>(c)] = u[c];
-    if (lead) a.t_steps[row] = t;
-  } else {
-    store_save<T, R>(a, a.u_steps, s.kept, a.max_saves, traj, u);
-    if (lead) store_save_time(a, s.kept, traj, t);
+    for (int k = 0; k + 1 < PK_ROWS; ++k) {  // shift: static register indices
+      s.bt[k] = s.bt[k + 1];
+#pragma unroll
+      for (int c = 0; c < R::N; ++c) s.bu[k][c] = s.bu[k + 1][c];
+    }
+    s.bt[PK_ROWS - 1] = t;
+#pragma unroll
+    for (int c = 0; c < R::N; ++c) s.bu[PK_ROWS - 1][c] = u[c];
+    ++s.kept;
+    if (++s.held == PK_ROWS) flush_steps<T, R>(a, s);
+    return;
   }
+  store_save<T, R>(a, a.u_steps, s.kept, a.max_saves, traj, u);
+  if (lead) store_save_time(a, s.kept, traj, t);
   ++s.kept;
 }
 
-// Leader's end-of-trajectory write of the save_everystep count.
-__device__ __forceinline__ void finish_steps(const sdeh_kargs& a, const StepSink& s,
+// Leader's end-of-trajectory write of the save_everystep count (after
+// flush_steps on every lane of the trajectory).
+template <class T, int N>
+__device__ __forceinline__ void finish_steps(const sdeh_kargs& a, const StepSink<T, N>& s,
                                              int64_t traj) {
   a.n_steps[traj] = s.kept;
 }
@@ -961,7 +1001,7 @@ __device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
   double t = 0.0, tf = 0.0;
   T dt = T(1), qold = qoldinit, qold_pow = qoldinit_pow;
   int32_t nacc = 0, nrej = 0, cur = 0;
-  StepSink sk{0, 0, 0, 0};
+  StepSink<T, N> sk{};
   double ts_next = 0.0;  // save_time(cur)
   int64_t attempts = 0, traj = -1;
   bool active = false;
@@ -1171,6 +1211,7 @@ __device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
     if (ret >= 0) {  // trajectory ends: write outputs (gpuatsit5.jl:322-329)
       if (SAVE == SAVE_STEPS && ret == RET_DEFAULT && sk.kept < sk.n) ret = RET_SAVE_OVERFLOW;
       store_state<T, R>(a.u_end, 0, n, traj, uprev);
+      if constexpr (SAVE == SAVE_STEPS) flush_steps<T, R>(a, sk);
       if (lead) {
         a.t_end[traj] = t;
         a.retcode[traj] = ret;
@@ -1209,7 +1250,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
   double t = 0.0, tf = 0.0;
   T dt = T(0), qold = T(0), qold_pow = T(0);
   int32_t nacc = 0, nrej = 0, cur = 0;
-  StepSink sk{0, 0, 0, 0};
+  StepSink<T, N> sk{};
   double ts_next = 0.0;  // save_time(cur)
   int64_t attempts = 0, traj = -1;
   bool active = false;
@@ -1388,6 +1429,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
     if (ret >= 0) {  // trajectory ends: write outputs (gpuatsit5.jl:322-329)
       if (SAVE == SAVE_STEPS && ret == RET_DEFAULT && sk.kept < sk.n) ret = RET_SAVE_OVERFLOW;
       store_state<T, R>(a.u_end, 0, n, traj, uprev);
+      if constexpr (SAVE == SAVE_STEPS) flush_steps<T, R>(a, sk);
       if (lead) {
         a.t_end[traj] = t;
         a.retcode[traj] = ret;
@@ -1433,7 +1475,7 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
   const T dt = (T)a.dt;
   double t = a.t0;
   int32_t cur = 0;
-  StepSink sk{0, 0, 0, 0};
+  StepSink<T, N> sk{};
   if constexpr (SAVE == SAVE_STEPS) sink_start<R>(a, sk, traj);
   double ts_next = 0.0;  // save_time(cur)
   if constexpr (SAVE == SAVE_AT) {
@@ -1537,6 +1579,7 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
     if constexpr (SAVE == SAVE_STEPS) save_step<T, R>(a, sk, traj, t, u);
   }
   store_state<T, R>(a.u_end, 0, n, traj, u);
+  if constexpr (SAVE == SAVE_STEPS) flush_steps<T, R>(a, sk);
   if (!lead) return;
   a.t_end[traj] = t;
   int ret = RET_DEFAULT;
