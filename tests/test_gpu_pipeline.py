@@ -107,3 +107,25 @@ def test_host_counts_optional(ctx):
     g = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
     ctx.solve(d, g)
     _same(h, g, ("u_end", "t_end", "retcode"))
+
+
+def test_pinned_and_pageable_host_arrays(ctx):
+    """Pinned host inputs (page-locked by torch) and pageable host outputs in one
+    pipelined solve: same bits as the device-memory solve."""
+    torch = pytest.importorskip("torch")
+    u0, p, ts = inputs.lorenz_uneven(N, seed=14)
+    ts[1] = np.minimum(ts[1], 2.0)
+    pins = []
+    for a in (u0, p, ts):
+        t = torch.empty(a.shape, dtype=torch.float64, pin_memory=True)
+        t.numpy()[...] = a
+        pins.append(t)
+    d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=N, dt=0.1,
+                          abstol=1e-6, reltol=1e-4)
+    h = runtime.EnsembleBuffers(d, 3, pins[0].numpy(), pins[1].numpy(),
+                                tspan=pins[2].numpy())
+    assert h.io.u0 == pins[0].data_ptr()  # the pinned memory itself, not a copy
+    ctx.solve(d, h)
+    g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device="cuda:0")
+    ctx.solve(d, g)
+    _same(h, g, BASE)
