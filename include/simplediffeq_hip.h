@@ -124,7 +124,15 @@ typedef enum sdeh_save_layout {
 typedef enum sdeh_schedule {
   SDEH_SCHED_AUTO = 0,       /* library choice (autotuned) */
   SDEH_SCHED_ONE_LAUNCH = 1, /* one lane per trajectory, grid covers the ensemble */
-  SDEH_SCHED_PERSISTENT = 2  /* resident grid; lanes refill from a device atomic queue */
+  SDEH_SCHED_PERSISTENT = 2, /* resident grid; lanes refill from a device atomic queue */
+  SDEH_SCHED_STAGED = 3      /* adaptive solvers with endpoint output, one lane per
+                                trajectory, no caller order: a one-launch pass of 32
+                                attempts per trajectory, then the trajectories still
+                                running, largest expected remainder first, on the
+                                persistent grid (same results as the other schedules);
+                                elsewhere (other outputs or layouts, io->order, graph
+                                capture, chunked host-memory solves) it runs as
+                                SDEH_SCHED_PERSISTENT */
 } sdeh_schedule;
 
 /* Precision mode (north-star precision work). */

@@ -51,6 +51,9 @@ enum { ALG_TSIT5 = 0, ALG_ATSIT5 = 1, ALG_VERN7 = 2, ALG_AVERN7 = 3, ALG_VERN9 =
        ALG_AVERN9 = 5, ALG_RK4 = 6, ALG_EULER = 7 };
 enum { SAVE_END = 0, SAVE_AT = 1, SAVE_STEPS = 2 };
 enum { RET_DEFAULT = 0, RET_DTMIN = 1, RET_NAN = 2, RET_MAXITERS = 3, RET_SAVE_OVERFLOW = 4 };
+// A staged solve's first launch parks trajectories still running after
+// stage_cap attempts; the second launch resumes them and overwrites the code.
+enum { RET_PARKED = -2 };
 
 template <class T>
 struct Id {
@@ -1266,21 +1269,52 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
     }
   }
   bool fresh = active;
+  // Staging (stage_cap / resume) applies to one lane per trajectory with endpoint
+  // output.
+  constexpr bool STAGED = SAVE == SAVE_END && R::G == 1;
+  const int64_t att_lim = (STAGED && a.stage_cap > 0 && a.stage_cap < a.maxiters)
+                              ? (int64_t)a.stage_cap
+                              : a.maxiters;
 
   for (;;) {
     if (a.persistent) {
-      int64_t slot = wave_claim(a, !active && lead, qbase, qlim, drained);
-      if constexpr (R::G > 1)  // the group leader claimed; its group follows
-        slot = (int64_t)__shfl((long long)slot, (int)(__lane_id() & ~(unsigned)(R::G - 1)));
-      if (slot >= 0) {
-        traj = a.order ? a.order[slot] : slot;
-        active = true;
-        fresh = true;
+      bool want = !active && lead;
+      for (;;) {
+        int64_t slot = wave_claim(a, want, qbase, qlim, drained);
+        if constexpr (R::G > 1)  // the group leader claimed; its group follows
+          slot = (int64_t)__shfl((long long)slot, (int)(__lane_id() & ~(unsigned)(R::G - 1)));
+        if (slot >= 0) {
+          traj = a.order ? a.order[slot] : slot;
+          active = true;
+          fresh = true;
+          // a resumed launch skips the trajectories that ended in the first stage
+          if (STAGED && a.resume && a.retcode[traj] != RET_PARKED) active = fresh = false;
+        }
+        want = !active && lead;
+        if (!STAGED || !a.resume || drained || __ballot(want && slot >= 0) == 0ull) break;
       }
     }
     if (__ballot(active) == 0ull) break;
     if (!active) continue;
 
+    if (fresh && STAGED && a.resume) {  // continue a parked trajectory
+      fresh = false;
+      load_state<T, R>(uprev, a.u_end, n, traj);
+#pragma unroll
+      for (int c = 0; c < N; ++c) comp[c] = T(0);
+      if constexpr (KAHAN) load_state<T, R>(comp, a.st_comp, n, traj);
+      load_params<T, R>(pl, a, traj);
+      t = a.t_end[traj];
+      tf = a.tspan ? a.tspan[n + traj] : a.tf;
+      dt = ((const T*)a.st_dt)[traj];
+      qold = qoldinit;  // written before it is read again
+      qold_pow = ((const T*)a.st_qpow)[traj];
+      nacc = a.naccept[traj];
+      nrej = a.nreject[traj];
+      attempts = (int64_t)nacc + nrej;
+      cur = 0;
+      if constexpr (ALG == ALG_ATSIT5) load_state<T, R>(fsal, a.st_fsal, n, traj);
+    }
     if (fresh) {  // start trajectory `traj` (gpuatsit5.jl:221-241)
       fresh = false;
       load_state<T, R>(uprev, a.u0, n, traj);
@@ -1314,8 +1348,8 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
       ret = RET_DEFAULT;
     } else if ((double)dt < DTMIN) {  // :256
       ret = RET_DTMIN;
-    } else if (attempts >= a.maxiters) {
-      ret = RET_MAXITERS;
+    } else if (attempts >= att_lim) {  // maxiters, or the first stage's cap (parked)
+      ret = attempts >= a.maxiters ? RET_MAXITERS : RET_PARKED;
     } else {
       ++attempts;
       T u[N], err[N], comp_new[N];
@@ -1426,7 +1460,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
       }
     }
 
-    if (ret >= 0) {  // trajectory ends: write outputs (gpuatsit5.jl:322-329)
+    if (ret != -1) {  // trajectory ends or parks: write outputs (gpuatsit5.jl:322-329)
       if (SAVE == SAVE_STEPS && ret == RET_DEFAULT && sk.kept < sk.n) ret = RET_SAVE_OVERFLOW;
       store_state<T, R>(a.u_end, 0, n, traj, uprev);
       if constexpr (SAVE == SAVE_STEPS) flush_steps<T, R>(a, sk);
@@ -1436,6 +1470,19 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
         if (a.naccept) a.naccept[traj] = nacc;
         if (a.nreject) a.nreject[traj] = nrej;
         if constexpr (SAVE == SAVE_STEPS) finish_steps(a, sk, traj);
+      }
+      if (STAGED && a.stage_cap > 0) {
+        double key = 0.0;
+        if (ret == RET_PARKED) {
+          if constexpr (ALG == ALG_ATSIT5) store_state<T, R>(a.st_fsal, 0, n, traj, fsal);
+          if constexpr (KAHAN) store_state<T, R>(a.st_comp, 0, n, traj, comp);
+          ((T*)a.st_dt)[traj] = dt;
+          ((T*)a.st_qpow)[traj] = qold_pow;
+          // remaining attempts extrapolated from the first stage's progress
+          const double t0 = a.tspan ? a.tspan[traj] : a.t0;
+          key = (double)a.stage_cap * ((tf - t) / (t - t0));
+        }
+        a.st_key[traj] = key;
       }
       active = false;
     }

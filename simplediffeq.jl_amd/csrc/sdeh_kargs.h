@@ -49,6 +49,21 @@ struct sdeh_kargs {
   // steps_off[i] + k of u_steps ([row][N]) / t_steps ([row]) while k < n_steps[i],
   // the row count a first pass and the offsets scan left there.
   int32_t packed;
-  int32_t pk_pad_;
+  // Staged endpoint solves (one lane per trajectory, adaptive, endpoint output).
+  // stage_cap > 0: a trajectory still running after stage_cap attempts stops with
+  // retcode RET_PARKED, leaving u in u_end, t in t_end, the counts in naccept /
+  // nreject, and fsal ([N][n], ATsit5), comp ([N][n], Kahan), dt and qold^beta2
+  // ([n] each, state type) in the stage buffers; every trajectory writes
+  // st_key[i]: the attempts it is expected still to need, stage_cap * (tf - t) /
+  // (t - t0), or 0 when it ended.  resume != 0: a trajectory handed out starts
+  // from that state if its retcode is RET_PARKED and is skipped otherwise.
+  int32_t stage_cap;
   const int64_t* steps_off;
+  void* st_fsal;
+  void* st_comp;
+  void* st_dt;
+  void* st_qpow;
+  double* st_key;
+  int32_t resume;
+  int32_t st_pad_;
 };

@@ -47,6 +47,8 @@ SDEH_HIDDEN hipError_t sdeh_util_check_order(const int64_t*, int64_t, int32_t*, 
                                              hipStream_t);
 SDEH_HIDDEN hipError_t sdeh_util_span_order(const double*, int64_t, unsigned*, int64_t*,
                                             hipStream_t);
+SDEH_HIDDEN hipError_t sdeh_util_key_order(const double*, int64_t, unsigned*, int64_t*,
+                                           hipStream_t);
 SDEH_HIDDEN size_t sdeh_util_span_scratch_bytes();
 SDEH_HIDDEN hipError_t sdeh_util_pack_offsets(const int32_t*, int32_t, int64_t, int64_t*, int64_t,
                                               int64_t*, int32_t*, unsigned long long*,
@@ -73,6 +75,11 @@ constexpr int64_t kDefaultMaxiters = 10000000;
 // the inputs of one chunk travel to HBM while the previous chunk integrates, and
 // its outputs travel back while the next one does.
 constexpr int64_t kPipeMin = 1 << 17;
+// Staged schedule: attempts of the first stage.  Its progress ranks the remaining
+// work: on the 2^20-trajectory Lorenz ATsit5 ensemble the extrapolated remaining
+// attempts after 32 attempts have a rank correlation of 0.98 with the true ones
+// (0.95 after 16, 0.74 after 8).
+constexpr int32_t kStageAttempts = 32;
 constexpr int kPipeChunks = 4;
 static_assert(8 + kPipeChunks <= 16, "chunk queues live in counters[8..16)");
 
@@ -471,6 +478,11 @@ struct Plan {
   bool span = false;
   int64_t* span_ord = nullptr;
   unsigned* span_hist[2] = {nullptr, nullptr};
+  // staged schedule (see run_plan): hand-out order of the second launch and its
+  // sort scratch; the parked state lives in the kargs' st_* buffers
+  bool staged = false;
+  int64_t* st_order = nullptr;
+  unsigned* st_hist = nullptr;
   KernelRef k_count;
   KernelRef k;
   Shape shape;
@@ -548,6 +560,8 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
     pl->pk_cap = io->steps_capacity;
     a.packed = 1;
   }
+  if (d->schedule < SDEH_SCHED_AUTO || d->schedule > SDEH_SCHED_STAGED)
+    return fail(c, SDEH_E_INVALID, "schedule %d out of range", d->schedule);
   if (d->span_order != 0 && d->span_order != 1)
     return fail(c, SDEH_E_INVALID, "span_order %d out of range", d->span_order);
   pl->span = d->span_order == 0 && adaptive && io->tspan && !io->order && d->n_traj > 1 &&
@@ -580,9 +594,10 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
     if (d->schedule != SDEH_SCHED_AUTO) sh.schedule = d->schedule;
     if (d->block) sh.block = d->block;
     if (d->chunk) sh.chunk = d->chunk;
-    if (sh.schedule == SDEH_SCHED_PERSISTENT) {
+    if (sh.schedule == SDEH_SCHED_PERSISTENT || sh.schedule == SDEH_SCHED_STAGED) {
       if (d->grid) sh.grid = d->grid;
-      else if (it == c->tuned.end() || d->block) sh.grid = c->cus * blocks_per_cu(c, pl->k, sh.block);
+      else if (it == c->tuned.end() || d->block || it->second.schedule != sh.schedule)
+        sh.grid = c->cus * blocks_per_cu(c, pl->k, sh.block);
       if (sh.grid < 1) sh.grid = 1;
     }
   } else {
@@ -594,7 +609,15 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
     if (g > 0x7fffffff) return fail(c, SDEH_E_INVALID, "ensemble too large for one launch");
     sh.grid = (int)g;
   }
-  a.persistent = sh.schedule == SDEH_SCHED_PERSISTENT;
+  if (sh.schedule == SDEH_SCHED_STAGED) {
+    // endpoint output, one lane per trajectory, the library's own order; the
+    // second launch is the persistent one, so a staged shape falls back to it
+    pl->staged = adaptive && pl->save == SAVE_END && !pl->packed && pl->k.lanes == 1 &&
+                 !io->order && d->n_traj > 1 && d->n_traj < (1ll << 32);
+    if (!pl->staged) sh.schedule = SDEH_SCHED_PERSISTENT;
+    if (pl->staged) pl->span = false;  // the second launch is ordered by the first stage
+  }
+  a.persistent = sh.schedule == SDEH_SCHED_PERSISTENT || sh.schedule == SDEH_SCHED_STAGED;
   a.chunk = sh.chunk;
   pl->shape = sh;
   return SDEH_OK;
@@ -605,6 +628,9 @@ struct Sizes {
   size_t u0, p, ps, tspan, dt0, order, u_end, t_end, i32, u_saveat, u_steps, t_steps, saveat;
   // packed layout: offsets output and scan scratch; span order and its histogram
   size_t pk_off, pk_tiles, span_order, span_hist;
+  // staged schedule: parked fsal / comp ([N][n]), dt and qold^beta2 ([n]), key and
+  // order ([n]), sort scratch
+  size_t st_fsal, st_comp, st_scalar, st_key, st_order, st_hist;
 };
 
 Sizes sizes_of(const sdeh_solve_desc* d, const sdeh_io* io, const Plan& pl) {
@@ -626,6 +652,14 @@ Sizes sizes_of(const sdeh_solve_desc* d, const sdeh_io* io, const Plan& pl) {
   if (pl.span) {
     s.span_order = n * 8;
     s.span_hist = sdeh_util_span_scratch_bytes();
+  }
+  if (pl.staged) {
+    s.st_fsal = d->alg == SDEH_ALG_ATSIT5 ? s.u0 : 0;
+    s.st_comp = pl.kahan ? s.u0 : 0;
+    s.st_scalar = n * pl.es;
+    s.st_key = n * 8;
+    s.st_order = n * 8;
+    s.st_hist = sdeh_util_span_scratch_bytes();
   }
   if (pl.packed) {
     const size_t cap = (size_t)pl.pk_cap;
@@ -672,8 +706,33 @@ int run_plan(sdeh_ctx* c, Plan& pl, hipStream_t s, unsigned long long* queue, bo
       HIPCHK(c, sdeh_util_pack_offsets(counted ? a.naccept : nullptr, add, a.n, pl.pk_tiles,
                                        pl.pk_cap, pl.pk_off, a.n_steps, c->counters + 6, s));
     }
-    if (a.persistent) HIPCHK(c, sdeh_util_zero_u64(queue, 1, s));
-    HIPCHK(c, launch(pl.k, pl.shape.grid, pl.shape.block, &a, s));
+    if (pl.staged) {
+      // Staged: every trajectory runs its first kStageAttempts attempts in a
+      // one-launch pass; those still running park their state and an estimate of
+      // the attempts they still need, the estimates are counting-sorted (largest
+      // first), and a persistent launch resumes the parked trajectories in that
+      // order, so the longest start first and the end of the launch drains less.
+      // Parking stores the whole per-trajectory state, so every trajectory takes
+      // the same steps as in one launch.
+      sdeh_kargs a1 = a;
+      a1.persistent = 0;
+      a1.order = nullptr;
+      a1.stage_cap = kStageAttempts;
+      a1.resume = 0;
+      const int64_t g1 = (a.n * pl.k.lanes + pl.shape.block - 1) / pl.shape.block;
+      HIPCHK(c, launch(pl.k, (int)g1, pl.shape.block, &a1, s));
+      HIPCHK(c, sdeh_util_key_order(a.st_key, a.n, pl.st_hist, pl.st_order, s));
+      sdeh_kargs a2 = a;
+      a2.persistent = 1;
+      a2.order = pl.st_order;
+      a2.stage_cap = 0;
+      a2.resume = 1;
+      HIPCHK(c, sdeh_util_zero_u64(queue, 1, s));
+      HIPCHK(c, launch(pl.k, pl.shape.grid, pl.shape.block, &a2, s));
+    } else {
+      if (a.persistent) HIPCHK(c, sdeh_util_zero_u64(queue, 1, s));
+      HIPCHK(c, launch(pl.k, pl.shape.grid, pl.shape.block, &a, s));
+    }
     if (timed) HIPCHK(c, hipEventRecord(c->ev1, s));
   }
   return SDEH_OK;
@@ -824,6 +883,10 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
     r = check_order_host(c, io->order, n);
     if (r) return r;
   }
+  if (pl.staged && host && n >= kPipeMin && !pl.packed) {  // the pipeline's chunks run persistent
+    pl.staged = false;
+    pl.shape.schedule = SDEH_SCHED_PERSISTENT;
+  }
 
   // Device layout: saveat copy + (host mode) all buffers + counters always.
   size_t need = round256(sz.saveat);
@@ -841,6 +904,9 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
     if (host) need += round256(sz.pk_off);
   }
   if (pl.span) need += round256(sz.span_order) + 2 * round256(sz.span_hist);
+  if (pl.staged)
+    need += round256(sz.st_fsal) + round256(sz.st_comp) + 2 * round256(sz.st_scalar) +
+            round256(sz.st_key) + round256(sz.st_order) + round256(sz.st_hist);
   r = ensure_arena(c, need);
   if (r) return r;
   Arena ar{c->arena};
@@ -931,6 +997,15 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
       HIPCHK(c, sdeh_util_span_order(a.tspan, n, pl.span_hist[0], pl.span_ord, s));
       a.order = pl.span_ord;
     }
+  }
+  if (pl.staged) {
+    a.st_fsal = ar.take(sz.st_fsal);
+    a.st_comp = ar.take(sz.st_comp);
+    a.st_dt = ar.take(sz.st_scalar);
+    a.st_qpow = ar.take(sz.st_scalar);
+    a.st_key = (double*)ar.take(sz.st_key);
+    pl.st_order = (int64_t*)ar.take(sz.st_order);
+    pl.st_hist = (unsigned*)ar.take(sz.st_hist);
   }
 
   if (!host && io->order && n > 0) {  // permutation check on the device
@@ -1172,6 +1247,10 @@ int sdeh_graph_capture(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io,
   int r = plan_solve(c, d, io, &pl);
   if (r) return r;
   if (io->order) return fail(c, SDEH_E_UNSUPPORTED, "graph capture takes no order permutation");
+  if (pl.staged) {  // a captured solve runs the staged shape as its persistent launch
+    pl.staged = false;
+    pl.shape.schedule = SDEH_SCHED_PERSISTENT;
+  }
   if (pl.packed)
     return fail(c, SDEH_E_UNSUPPORTED, "graph capture of the packed save layout is not built");
   const Sizes sz = sizes_of(d, io, pl);
@@ -1288,6 +1367,7 @@ int sdeh_autotune(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, sdeh
   for (int b : {64, 128, 256})
     for (int ch : {16, 64, 256}) cands.push_back({SDEH_SCHED_PERSISTENT, b, ch, 1});
   for (int ch : {64}) cands.push_back({SDEH_SCHED_PERSISTENT, 256, ch, 2});
+  for (int ch : {16, 64}) cands.push_back({SDEH_SCHED_STAGED, 256, ch, 1});
   double best_ms = 1e300;
   Shape best_shape;
   sdeh_stats best_stats{};
@@ -1297,10 +1377,11 @@ int sdeh_autotune(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, sdeh
     dd.block = k.block;
     dd.chunk = k.chunk;
     dd.grid = 0;
-    if (k.sched == SDEH_SCHED_PERSISTENT) {
+    if (k.sched != SDEH_SCHED_ONE_LAUNCH) {
       Plan pl;
       int r = plan_solve(c, &dd, io, &pl);
       if (r) return r;
+      if (pl.shape.schedule != k.sched) continue;  // staged does not apply to this solve
       dd.grid = std::max(1, pl.shape.grid / k.occ_div);
     }
     double ms = 1e300;

@@ -156,12 +156,15 @@ __global__ void write_i64(int64_t* p, int64_t v) {
 // decreasing order, blocks in index order), and span_scatter places the block's
 // trajectories, ranking them with LDS atomics.  No global atomics: the few buckets
 // a narrow span distribution fills would serialise them.  Within a (bucket, block)
-// the order is whatever the LDS atomics produce; no result depends on it.
+// the order is whatever the LDS atomics produce; no result depends on it.  With a
+// key array the same sort runs on key[i] in place of the span (a staged solve's
+// expected remaining attempts).
 constexpr int SPAN_BUCKETS = 1024;
 constexpr int SPAN_BLOCKS = 256;
 
-__device__ __forceinline__ int span_bucket(const double* tspan, int64_t n, int64_t i) {
-  const double span = tspan[n + i] - tspan[i];
+__device__ __forceinline__ int span_bucket(const double* tspan, const double* key, int64_t n,
+                                           int64_t i) {
+  const double span = key ? key[i] : tspan[n + i] - tspan[i];
   if (!(span > 0.0)) return 0;  // also NaN
   const unsigned long long bits = (unsigned long long)__double_as_longlong(span);
   int oct = (int)((bits >> 52) & 0x7ff) - (1023 - 30);
@@ -176,14 +179,15 @@ __device__ __forceinline__ void span_range(int64_t n, int64_t* lo, int64_t* hi) 
 }
 
 // bh[block][bucket] = trajectories of the block's range in the bucket.
-__global__ void __launch_bounds__(256) span_hist(const double* tspan, int64_t n, unsigned* bh) {
+__global__ void __launch_bounds__(256) span_hist(const double* tspan, const double* key, int64_t n,
+                                                 unsigned* bh) {
   __shared__ unsigned h[SPAN_BUCKETS];
   for (int k = threadIdx.x; k < SPAN_BUCKETS; k += blockDim.x) h[k] = 0u;
   __syncthreads();
   int64_t lo, hi;
   span_range(n, &lo, &hi);
   for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
-    atomicAdd(h + span_bucket(tspan, n, i), 1u);
+    atomicAdd(h + span_bucket(tspan, key, n, i), 1u);
   __syncthreads();
   for (int k = threadIdx.x; k < SPAN_BUCKETS; k += blockDim.x)
     bh[(size_t)blockIdx.x * SPAN_BUCKETS + k] = h[k];
@@ -226,8 +230,9 @@ __global__ void __launch_bounds__(1024) span_scan(unsigned* bh) {
   }
 }
 
-__global__ void __launch_bounds__(256) span_scatter(const double* tspan, int64_t n,
-                                                    const unsigned* bh, int64_t* order) {
+__global__ void __launch_bounds__(256) span_scatter(const double* tspan, const double* key,
+                                                    int64_t n, const unsigned* bh,
+                                                    int64_t* order) {
   __shared__ unsigned cur[SPAN_BUCKETS];
   for (int k = threadIdx.x; k < SPAN_BUCKETS; k += blockDim.x)
     cur[k] = bh[(size_t)blockIdx.x * SPAN_BUCKETS + k];
@@ -235,7 +240,7 @@ __global__ void __launch_bounds__(256) span_scatter(const double* tspan, int64_t
   int64_t lo, hi;
   span_range(n, &lo, &hi);
   for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
-    order[atomicAdd(cur + span_bucket(tspan, n, i), 1u)] = i;
+    order[atomicAdd(cur + span_bucket(tspan, key, n, i), 1u)] = i;
 }
 
 inline int grid_for(int64_t n) {
@@ -308,13 +313,25 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_pack_offse
 
 // order[0..n) = trajectories by decreasing span (see span_hist).  bh: scratch
 // of SPAN_BLOCKS x SPAN_BUCKETS uint32.  n must fit in 32 bits (the slot counters).
+static hipError_t bucket_order(const double* tspan, const double* key, int64_t n, unsigned* bh,
+                               int64_t* order, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(span_hist, dim3(SPAN_BLOCKS), dim3(256), 0, s, tspan, key, n, bh);
+  hipLaunchKernelGGL(span_scan, dim3(1), dim3(SPAN_BUCKETS), 0, s, bh);
+  hipLaunchKernelGGL(span_scatter, dim3(SPAN_BLOCKS), dim3(256), 0, s, tspan, key, n, bh, order);
+  return hipGetLastError();
+}
+
 extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_span_order(
     const double* tspan, int64_t n, unsigned* bh, int64_t* order, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(span_hist, dim3(SPAN_BLOCKS), dim3(256), 0, s, tspan, n, bh);
-  hipLaunchKernelGGL(span_scan, dim3(1), dim3(SPAN_BUCKETS), 0, s, bh);
-  hipLaunchKernelGGL(span_scatter, dim3(SPAN_BLOCKS), dim3(256), 0, s, tspan, n, bh, order);
-  return hipGetLastError();
+  return bucket_order(tspan, nullptr, n, bh, order, s);
+}
+
+// order[0..n) = trajectories by decreasing key[i] (keys <= 0 and NaN last), on the
+// same buckets and scratch as sdeh_util_span_order.
+extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_key_order(
+    const double* key, int64_t n, unsigned* bh, int64_t* order, hipStream_t s) {
+  return bucket_order(nullptr, key, n, bh, order, s);
 }
 
 extern "C" __attribute__((visibility("hidden"))) size_t sdeh_util_span_scratch_bytes() {

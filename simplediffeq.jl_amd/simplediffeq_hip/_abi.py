@@ -44,7 +44,7 @@ BUILTIN_RHS = {
 
 # sdeh_schedule / precision / memory
 SAVE_POINT_MAJOR, SAVE_TRAJ_MAJOR, SAVE_PACKED = 0, 1, 2
-SCHED_AUTO, SCHED_ONE_LAUNCH, SCHED_PERSISTENT = 0, 1, 2
+SCHED_AUTO, SCHED_ONE_LAUNCH, SCHED_PERSISTENT, SCHED_STAGED = 0, 1, 2, 3
 PREC_PLAIN, PREC_KAHAN = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
 

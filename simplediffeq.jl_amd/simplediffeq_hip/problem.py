@@ -159,7 +159,7 @@ class EnsembleHIP:
     def __init__(self, device=0, schedule="auto", precision="plain", memory="host"):
         self.device = device
         self.schedule = {"auto": _abi.SCHED_AUTO, "one_launch": _abi.SCHED_ONE_LAUNCH,
-                         "persistent": _abi.SCHED_PERSISTENT}[schedule]
+                         "persistent": _abi.SCHED_PERSISTENT, "staged": _abi.SCHED_STAGED}[schedule]
         self.precision = {"plain": _abi.PREC_PLAIN, "kahan": _abi.PREC_KAHAN}[precision]
         self.memory = memory
 
