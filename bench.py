@@ -39,12 +39,14 @@ N_TRAJ = 1 << 20
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch of the binary64 headline kernel at 2^20 trajectories, from
-# two rocprofv3 --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from
-# a 256 MiB device copy in the same run): 49,405 KiB read + 82,225 KiB written.
-# Counters cannot be read from inside a timed run, so the figure applies only to the
-# configuration it was measured on.
-MEASURED_TRAFFIC = {("f64", 1 << 20): 134.79e6}
+# HBM bytes per solve of the binary64 headline at 2^20 trajectories, by schedule,
+# from two rocprofv3 --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors
+# from a 256 MiB device copy in the same run).  Persistent: 49,405 KiB read + 82,225
+# KiB written.  Staged (first pass, key sort, resumed pass): 234,824 KiB read +
+# 389,626 KiB written, mostly the parked records and the resumed pass's endpoint
+# stores in hand-out order (partial lines).  Counters cannot be read from inside a
+# timed run, so each figure applies only to the configuration it was measured on.
+MEASURED_TRAFFIC = {("f64", 1 << 20, 2): 134.79e6, ("f64", 1 << 20, 3): 639.44e6}
 
 # Algorithmic FLOPs (per component per attempt, RHS calls per attempt) by solver.
 STAGE_FLOPS = {0: 48, 1: 67, 2: 83, 3: 117, 4: 167, 5: 211, 6: 13, 7: 2}
@@ -471,6 +473,9 @@ def main():
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="skip the autotuner (by default sdeh_autotune picks the launch shape "
                          "once per variant, before the warmup and the timed steps)")
+    ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="headline schedule: 0 library choice (autotuned), 1 one launch, "
+                         "2 persistent, 3 staged")
     ap.add_argument("--grid", type=int, default=0,
                     help="persistent schedule: workgroups of 256 (0 = library default, "
                          "CUs x occupancy)")
@@ -518,7 +523,8 @@ def main():
     for dtype_name, dtype in (("f64", _abi.F64), ("f32", _abi.F32)):
         if dtype_name == "f32" and args.no_f32:
             continue
-        d = runtime.make_desc(dtype=dtype, n_traj=n, grid=args.grid, **common)
+        d = runtime.make_desc(dtype=dtype, n_traj=n, grid=args.grid, schedule=args.schedule,
+                              **common)
         bufs = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device=dev)
         torch.cuda.synchronize()
         if args.tune:
@@ -586,8 +592,10 @@ def main():
                    "u0": "U(-1,1)^3", "launch": r64["shape"]},
         "roofline": {"bound": "valu", "achieved": r64["tflops"], "peak": PEAK_TFLOPS["f64"],
                      "unit": "TFLOP/s", "frac": r64["tflops"] / PEAK_TFLOPS["f64"],
-                     "traffic": MEASURED_TRAFFIC.get(("f64", n)),
-                     "kernel": "sdeh_adaptive_kernel<double, Lorenz, ATsit5, endpoint>",
+                     "traffic": MEASURED_TRAFFIC.get(("f64", n, r64["shape"]["schedule"])),
+                     "kernel": "sdeh_adaptive_kernel<double, Lorenz, ATsit5, endpoint>" + (
+                         " (staged: first pass, key sort, resumed pass)"
+                         if r64["shape"]["schedule"] == _abi.SCHED_STAGED else ""),
                      "kernel_ms": r64["kernel_ms"], "algorithmic_flops_per_launch": r64["flops"],
                      "attempts_per_launch": r64["attempts"],
                      "hbm_gbs": r64["gbs"], "hbm_frac": r64["gbs"] / HBM_PEAK_GBS},

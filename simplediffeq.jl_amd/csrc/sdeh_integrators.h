@@ -1227,6 +1227,74 @@ __device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
   }
 }
 
+// A staged solve's parked trajectory, one record per trajectory (layout and size
+// SDEH_PARK_BYTES in sdeh_kargs.h): t, tf, counts and flag, then u, fsal, comp,
+// p, dt and qold^beta2.
+template <class T, class R, bool F, bool K>
+struct ParkRec {
+  static constexpr int N = R::N, P = R::P > 0 ? R::P : 1, NP = R::P;
+  __device__ static char* at(const sdeh_kargs& a, int64_t traj) {
+    return (char*)a.st_rec + traj * a.st_stride;
+  }
+  __device__ static int32_t flag(const sdeh_kargs& a, int64_t traj) {
+    return ((const int32_t*)at(a, traj))[6];  // byte 24
+  }
+  __device__ static void set_flag(const sdeh_kargs& a, int64_t traj, int32_t f) {
+    ((int32_t*)at(a, traj))[6] = f;
+  }
+  __device__ static void store(const sdeh_kargs& a, int64_t traj, const T (&u)[N],
+                               const T (&fsal)[N], const T (&comp)[N], const T (&pl)[P],
+                               double t, double tf, T dt, T qpow, int32_t nacc, int32_t nrej) {
+    char* r = at(a, traj);
+    ((double*)r)[0] = t;
+    ((double*)r)[1] = tf;
+    ((int32_t*)r)[4] = nacc;
+    ((int32_t*)r)[5] = nrej;
+    ((int32_t*)r)[6] = 1;
+    T* v = (T*)(r + 32);
+#pragma unroll
+    for (int c = 0; c < N; ++c) *v++ = u[c];
+    if constexpr (F) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) *v++ = fsal[c];
+    }
+    if constexpr (K) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) *v++ = comp[c];
+    }
+#pragma unroll
+    for (int c = 0; c < NP; ++c) *v++ = pl[c];
+    v[0] = dt;
+    v[1] = qpow;
+  }
+  __device__ static void load(const sdeh_kargs& a, int64_t traj, T (&u)[N], T (&fsal)[N],
+                              T (&comp)[N], T (&pl)[P], double& t, double& tf, T& dt, T& qpow,
+                              int32_t& nacc, int32_t& nrej) {
+    const char* r = at(a, traj);
+    t = ((const double*)r)[0];
+    tf = ((const double*)r)[1];
+    nacc = ((const int32_t*)r)[4];
+    nrej = ((const int32_t*)r)[5];
+    const T* v = (const T*)(r + 32);
+#pragma unroll
+    for (int c = 0; c < N; ++c) u[c] = *v++;
+    if constexpr (F) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) fsal[c] = *v++;
+    }
+#pragma unroll
+    for (int c = 0; c < N; ++c) comp[c] = T(0);
+    if constexpr (K) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) comp[c] = *v++;
+    }
+#pragma unroll
+    for (int c = 0; c < NP; ++c) pl[c] = *v++;
+    dt = v[0];
+    qpow = v[1];
+  }
+};
+
 template <class T, class R, int ALG, int SAVE, bool KAHAN>
 __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
   if constexpr (IsWave<R>::value) {
@@ -1272,6 +1340,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
   // Staging (stage_cap / resume) applies to one lane per trajectory with endpoint
   // output.
   constexpr bool STAGED = SAVE == SAVE_END && R::G == 1;
+  using Park = ParkRec<T, R, ALG == ALG_ATSIT5, KAHAN>;
   const int64_t att_lim = (STAGED && a.stage_cap > 0 && a.stage_cap < a.maxiters)
                               ? (int64_t)a.stage_cap
                               : a.maxiters;
@@ -1288,7 +1357,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
           active = true;
           fresh = true;
           // a resumed launch skips the trajectories that ended in the first stage
-          if (STAGED && a.resume && a.retcode[traj] != RET_PARKED) active = fresh = false;
+          if (STAGED && a.resume && Park::flag(a, traj) == 0) active = fresh = false;
         }
         want = !active && lead;
         if (!STAGED || !a.resume || drained || __ballot(want && slot >= 0) == 0ull) break;
@@ -1299,21 +1368,10 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
 
     if (fresh && STAGED && a.resume) {  // continue a parked trajectory
       fresh = false;
-      load_state<T, R>(uprev, a.u_end, n, traj);
-#pragma unroll
-      for (int c = 0; c < N; ++c) comp[c] = T(0);
-      if constexpr (KAHAN) load_state<T, R>(comp, a.st_comp, n, traj);
-      load_params<T, R>(pl, a, traj);
-      t = a.t_end[traj];
-      tf = a.tspan ? a.tspan[n + traj] : a.tf;
-      dt = ((const T*)a.st_dt)[traj];
+      Park::load(a, traj, uprev, fsal, comp, pl, t, tf, dt, qold_pow, nacc, nrej);
       qold = qoldinit;  // written before it is read again
-      qold_pow = ((const T*)a.st_qpow)[traj];
-      nacc = a.naccept[traj];
-      nrej = a.nreject[traj];
       attempts = (int64_t)nacc + nrej;
       cur = 0;
-      if constexpr (ALG == ALG_ATSIT5) load_state<T, R>(fsal, a.st_fsal, n, traj);
     }
     if (fresh) {  // start trajectory `traj` (gpuatsit5.jl:221-241)
       fresh = false;
@@ -1462,25 +1520,27 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
 
     if (ret != -1) {  // trajectory ends or parks: write outputs (gpuatsit5.jl:322-329)
       if (SAVE == SAVE_STEPS && ret == RET_DEFAULT && sk.kept < sk.n) ret = RET_SAVE_OVERFLOW;
-      store_state<T, R>(a.u_end, 0, n, traj, uprev);
-      if constexpr (SAVE == SAVE_STEPS) flush_steps<T, R>(a, sk);
-      if (lead) {
-        a.t_end[traj] = t;
-        a.retcode[traj] = ret;
-        if (a.naccept) a.naccept[traj] = nacc;
-        if (a.nreject) a.nreject[traj] = nrej;
-        if constexpr (SAVE == SAVE_STEPS) finish_steps(a, sk, traj);
+      const bool parked = STAGED && ret == RET_PARKED;
+      if (!parked) {
+        store_state<T, R>(a.u_end, 0, n, traj, uprev);
+        if constexpr (SAVE == SAVE_STEPS) flush_steps<T, R>(a, sk);
+        if (lead) {
+          a.t_end[traj] = t;
+          a.retcode[traj] = ret;
+          if (a.naccept) a.naccept[traj] = nacc;
+          if (a.nreject) a.nreject[traj] = nrej;
+          if constexpr (SAVE == SAVE_STEPS) finish_steps(a, sk, traj);
+        }
       }
       if (STAGED && a.stage_cap > 0) {
         double key = 0.0;
-        if (ret == RET_PARKED) {
-          if constexpr (ALG == ALG_ATSIT5) store_state<T, R>(a.st_fsal, 0, n, traj, fsal);
-          if constexpr (KAHAN) store_state<T, R>(a.st_comp, 0, n, traj, comp);
-          ((T*)a.st_dt)[traj] = dt;
-          ((T*)a.st_qpow)[traj] = qold_pow;
+        if (parked) {
+          Park::store(a, traj, uprev, fsal, comp, pl, t, tf, dt, qold_pow, nacc, nrej);
           // remaining attempts extrapolated from the first stage's progress
           const double t0 = a.tspan ? a.tspan[traj] : a.t0;
           key = (double)a.stage_cap * ((tf - t) / (t - t0));
+        } else {
+          Park::set_flag(a, traj, 0);
         }
         a.st_key[traj] = key;
       }

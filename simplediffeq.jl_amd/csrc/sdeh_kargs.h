@@ -50,20 +50,24 @@ struct sdeh_kargs {
   // the row count a first pass and the offsets scan left there.
   int32_t packed;
   // Staged endpoint solves (one lane per trajectory, adaptive, endpoint output).
-  // stage_cap > 0: a trajectory still running after stage_cap attempts stops with
-  // retcode RET_PARKED, leaving u in u_end, t in t_end, the counts in naccept /
-  // nreject, and fsal ([N][n], ATsit5), comp ([N][n], Kahan), dt and qold^beta2
-  // ([n] each, state type) in the stage buffers; every trajectory writes
-  // st_key[i]: the attempts it is expected still to need, stage_cap * (tf - t) /
-  // (t - t0), or 0 when it ended.  resume != 0: a trajectory handed out starts
-  // from that state if its retcode is RET_PARKED and is skipped otherwise.
+  // stage_cap > 0: a trajectory still running after stage_cap attempts parks: its
+  // whole state goes to one record of st_stride bytes at st_rec + i * st_stride
+  // (layout below) instead of the outputs; every trajectory writes its record's
+  // flag (1 parked, 0 ended) and st_key[i]: the attempts it is expected still to
+  // need, stage_cap * (tf - t) / (t - t0), or 0 when it ended.  resume != 0: a
+  // trajectory handed out continues from its record if the flag is set and is
+  // skipped otherwise.  One record per trajectory keeps a resume to one or two
+  // cache lines however the hand-out order scatters the trajectories.
   int32_t stage_cap;
   const int64_t* steps_off;
-  void* st_fsal;
-  void* st_comp;
-  void* st_dt;
-  void* st_qpow;
+  void* st_rec;
   double* st_key;
+  int64_t st_stride;
   int32_t resume;
   int32_t st_pad_;
 };
+
+// Parked-trajectory record: double t, tf; int32 naccept, nreject, flag, pad; then
+// state-type arrays u[N], fsal[N] (F = 1 for ATsit5), comp[N] (K = 1 with Kahan
+// sums), p[P], dt, qold^beta2.  Bytes for element size ES:
+#define SDEH_PARK_BYTES(N, P, ES, F, K) (32 + (ES) * ((N) * (1 + (F) + (K)) + (P) + 2))

@@ -628,9 +628,9 @@ struct Sizes {
   size_t u0, p, ps, tspan, dt0, order, u_end, t_end, i32, u_saveat, u_steps, t_steps, saveat;
   // packed layout: offsets output and scan scratch; span order and its histogram
   size_t pk_off, pk_tiles, span_order, span_hist;
-  // staged schedule: parked fsal / comp ([N][n]), dt and qold^beta2 ([n]), key and
-  // order ([n]), sort scratch
-  size_t st_fsal, st_comp, st_scalar, st_key, st_order, st_hist;
+  // staged schedule: parked records (st_stride bytes each), key and order ([n]),
+  // sort scratch
+  size_t st_rec, st_stride, st_key, st_order, st_hist;
 };
 
 Sizes sizes_of(const sdeh_solve_desc* d, const sdeh_io* io, const Plan& pl) {
@@ -654,9 +654,10 @@ Sizes sizes_of(const sdeh_solve_desc* d, const sdeh_io* io, const Plan& pl) {
     s.span_hist = sdeh_util_span_scratch_bytes();
   }
   if (pl.staged) {
-    s.st_fsal = d->alg == SDEH_ALG_ATSIT5 ? s.u0 : 0;
-    s.st_comp = pl.kahan ? s.u0 : 0;
-    s.st_scalar = n * pl.es;
+    const size_t rec = SDEH_PARK_BYTES((size_t)pl.n_state, (size_t)pl.n_param, pl.es,
+                                       d->alg == SDEH_ALG_ATSIT5 ? 1 : 0, pl.kahan ? 1 : 0);
+    s.st_stride = (rec + 63) & ~size_t(63);  // whole 64-byte sectors per record
+    s.st_rec = n * s.st_stride;
     s.st_key = n * 8;
     s.st_order = n * 8;
     s.st_hist = sdeh_util_span_scratch_bytes();
@@ -712,8 +713,9 @@ int run_plan(sdeh_ctx* c, Plan& pl, hipStream_t s, unsigned long long* queue, bo
       // the attempts they still need, the estimates are counting-sorted (largest
       // first), and a persistent launch resumes the parked trajectories in that
       // order, so the longest start first and the end of the launch drains less.
-      // Parking stores the whole per-trajectory state, so every trajectory takes
-      // the same steps as in one launch.
+      // Parking stores the whole per-trajectory state (one record each, so the
+      // resumed pass reads one or two cache lines per trajectory whatever the
+      // order), and every trajectory takes the same steps as in one launch.
       sdeh_kargs a1 = a;
       a1.persistent = 0;
       a1.order = nullptr;
@@ -905,8 +907,8 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   }
   if (pl.span) need += round256(sz.span_order) + 2 * round256(sz.span_hist);
   if (pl.staged)
-    need += round256(sz.st_fsal) + round256(sz.st_comp) + 2 * round256(sz.st_scalar) +
-            round256(sz.st_key) + round256(sz.st_order) + round256(sz.st_hist);
+    need += round256(sz.st_rec) + round256(sz.st_key) + round256(sz.st_order) +
+            round256(sz.st_hist);
   r = ensure_arena(c, need);
   if (r) return r;
   Arena ar{c->arena};
@@ -999,10 +1001,8 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
     }
   }
   if (pl.staged) {
-    a.st_fsal = ar.take(sz.st_fsal);
-    a.st_comp = ar.take(sz.st_comp);
-    a.st_dt = ar.take(sz.st_scalar);
-    a.st_qpow = ar.take(sz.st_scalar);
+    a.st_rec = ar.take(sz.st_rec);
+    a.st_stride = (int64_t)sz.st_stride;
     a.st_key = (double*)ar.take(sz.st_key);
     pl.st_order = (int64_t*)ar.take(sz.st_order);
     pl.st_hist = (unsigned*)ar.take(sz.st_hist);
