@@ -615,7 +615,8 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
     pl->staged = adaptive && pl->save == SAVE_END && !pl->packed && pl->k.lanes == 1 &&
                  !io->order && d->n_traj > 1 && d->n_traj < (1ll << 32);
     if (!pl->staged) sh.schedule = SDEH_SCHED_PERSISTENT;
-    if (pl->staged) pl->span = false;  // the second launch is ordered by the first stage
+    // (a staged solve orders its second launch itself; pl->span stays for the paths
+    // that run the shape as persistent: graph capture, the host pipeline)
   }
   a.persistent = sh.schedule == SDEH_SCHED_PERSISTENT || sh.schedule == SDEH_SCHED_STAGED;
   a.chunk = sh.chunk;
@@ -991,7 +992,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
       }
     }
   }
-  if (pl.span) {  // longest span first (the pipeline orders each chunk itself)
+  if (pl.span && !pl.staged) {  // longest span first (the pipeline orders each chunk itself)
     pl.span_ord = (int64_t*)ar.take(sz.span_order);
     pl.span_hist[0] = (unsigned*)ar.take(sz.span_hist);
     pl.span_hist[1] = (unsigned*)ar.take(sz.span_hist);

@@ -155,3 +155,28 @@ def test_staged_headline_shape(ctx):
         assert ctx.stats()["schedule"] == sched
         res[sched] = b
     assert_bit_identical(res[_abi.SCHED_STAGED], res[_abi.SCHED_PERSISTENT], FIELDS)
+
+
+def test_staged_shape_in_graph_capture(ctx):
+    """Graph capture runs a staged shape as the persistent schedule, still handing
+    out trajectories longest span first: every replay gives the eager staged
+    solve's bits."""
+    import torch
+    n = 5000
+    u0, p, ts = _uneven(n, seed=36)
+    d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=n, dt=0.1,
+                          abstol=1e-6, reltol=1e-4, schedule=_abi.SCHED_STAGED, lanes=1)
+    ref = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device="cuda:0")
+    assert ctx.solve(d, ref) == 0
+    assert ctx.stats()["schedule"] == _abi.SCHED_STAGED
+    g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device="cuda:0")
+    torch.cuda.synchronize()
+    h = ctx.graph_capture(d, g)
+    for _ in range(2):
+        ctx.graph_launch(h)
+        ctx.synchronize()
+        assert_bit_identical(g, ref, FIELDS)
+        for f in ("u_end", "t_end"):
+            getattr(g, f).fill_(float("nan"))
+        torch.cuda.synchronize()
+    ctx.graph_destroy(h)
