@@ -141,9 +141,10 @@ __device__ void damped(T* du, const T* u, const T* p, const T* __restrict__ ps, 
     assert_bit_identical(res[_abi.SCHED_STAGED], res[_abi.SCHED_PERSISTENT], FIELDS)
 
 
-def test_staged_headline_shape(ctx):
-    """The headline workload's shape at 2^16 trajectories: the staged solve matches
-    the persistent one bit for bit (uniform tspan, no per-trajectory spans)."""
+def test_staged_wide_lorenz(ctx):
+    """The wide Lorenz ensemble (rho in U(0,21), tspan (0, 10), 1f-6 / 1f-3) at 2^16
+    trajectories: the staged solve matches the persistent one bit for bit (one
+    tspan for all, no per-trajectory spans)."""
     n = 1 << 16
     u0, p = inputs.lorenz_wide(n, seed=1)
     res = {}
