@@ -310,6 +310,16 @@ int sdeh_mlp_forward(sdeh_ctx* ctx, const float* params, const float* x, float* 
 int sdeh_mlp_plan_lorenz(sdeh_ctx* ctx, const float* params, const double* p, const double* u0,
                          const double* tspan, int64_t n, const double* feat, const float* y_norm,
                          float* key, double* dt0, float* pred, void* stream);
+/* The hand-out order from a plan's keys (no counterpart in the reference; it turns
+ * sdeh_mlp_plan_lorenz's `key` into the permutation sdeh_io.order takes, with no
+ * sort outside the library): order[0..n) lists the trajectories by increasing key,
+ * i.e. by decreasing predicted cost exp(-key), counting-sorted on a grid of 1/16
+ * octave of that cost (16 buckets per doubling, evenly spaced in log2, covering
+ * 2^-30 .. 2^34; costs outside go to the end buckets).  Trajectories that share a
+ * bucket keep an unspecified relative order; NaN keys come last.  Device buffers;
+ * key [n] FP32, order [n] int64, n < 2^32; ordered on `stream` (NULL = the
+ * context's), using the context's device scratch like sdeh_solve. */
+int sdeh_plan_order(sdeh_ctx* ctx, const float* key, int64_t n, int64_t* order, void* stream);
 /* Loss (mean squared error over n x 2 outputs) and its gradient with respect
  * to the parameters, on device buffers.  grad has SDEH_MLP_NPARAM floats;
  * loss is one float. */

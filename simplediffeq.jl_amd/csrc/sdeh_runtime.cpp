@@ -49,6 +49,8 @@ SDEH_HIDDEN hipError_t sdeh_util_span_order(const double*, int64_t, unsigned*, i
                                             hipStream_t);
 SDEH_HIDDEN hipError_t sdeh_util_key_order(const double*, int64_t, unsigned*, int64_t*,
                                            hipStream_t);
+SDEH_HIDDEN hipError_t sdeh_util_logkey_order(const float*, int64_t, unsigned*, int64_t*,
+                                              hipStream_t);
 SDEH_HIDDEN size_t sdeh_util_span_scratch_bytes();
 SDEH_HIDDEN hipError_t sdeh_util_pack_offsets(const int32_t*, int32_t, int64_t, int64_t*, int64_t,
                                               int64_t*, int32_t*, unsigned long long*,
@@ -1441,6 +1443,24 @@ int sdeh_mlp_plan_lorenz(sdeh_ctx* c, const float* params, const double* p, cons
                                 y_norm[3], key, dt0, pred,
                                 stream ? (hipStream_t)stream : c->stream);
   if (r) return fail(c, r, "mlp plan launch failed");
+  return SDEH_OK;
+}
+
+int sdeh_plan_order(sdeh_ctx* c, const float* key, int64_t n, int64_t* order, void* stream) {
+  if (!c || !key || !order || n < 0) return SDEH_E_INVALID;
+  if (n >= (1ll << 32)) return fail(c, SDEH_E_INVALID, "n must be below 2^32");
+  if (n == 0) return SDEH_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  int r = check_dev(c, key, (size_t)n * 4, "key");
+  if (!r) r = check_dev(c, order, (size_t)n * 8, "order");
+  if (r) return r;
+  // the bucket counts live in the arena, which an earlier call may still be using
+  const hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  r = after_pending(c, s);
+  if (!r) r = ensure_arena(c, sdeh_util_span_scratch_bytes());
+  if (r) return r;
+  HIPCHK(c, sdeh_util_logkey_order(key, n, (unsigned*)c->arena, order, s));
+  note_pending(c, s, false);
   return SDEH_OK;
 }
 

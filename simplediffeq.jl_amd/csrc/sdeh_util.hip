@@ -156,21 +156,41 @@ __global__ void write_i64(int64_t* p, int64_t v) {
 // decreasing order, blocks in index order), and span_scatter places the block's
 // trajectories, ranking them with LDS atomics.  No global atomics: the few buckets
 // a narrow span distribution fills would serialise them.  Within a (bucket, block)
-// the order is whatever the LDS atomics produce; no result depends on it.  With a
-// key array the same sort runs on key[i] in place of the span (a staged solve's
-// expected remaining attempts).
+// the order is whatever the LDS atomics produce; no result depends on it.  The same
+// sort runs on other weights (BucketSrc): a staged solve's expected remaining
+// attempts (binary64 keys), and the cost model's predicted step counts, given as
+// the plan's keys -log(count) in FP32.
 constexpr int SPAN_BUCKETS = 1024;
 constexpr int SPAN_BLOCKS = 256;
 
-__device__ __forceinline__ int span_bucket(const double* tspan, const double* key, int64_t n,
-                                           int64_t i) {
-  const double span = key ? key[i] : tspan[n + i] - tspan[i];
-  if (!(span > 0.0)) return 0;  // also NaN
-  const unsigned long long bits = (unsigned long long)__double_as_longlong(span);
+// Bucket of a positive weight w: 16 per octave, octave 0 = [2^-30, 2^-29).
+__device__ __forceinline__ int weight_bucket(double w) {
+  if (!(w > 0.0)) return 0;  // also NaN
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(w);
   int oct = (int)((bits >> 52) & 0x7ff) - (1023 - 30);
   oct = oct < 0 ? 0 : (oct > 63 ? 63 : oct);
   return oct * 16 + (int)((bits >> 48) & 15);
 }
+
+struct BucketSrc {
+  const double* tspan;  // [2][n]: weight tf - t0
+  const double* key;    // [n]: weight key[i]
+  const float* lkey;    // [n]: weight exp(-lkey[i]), bucketed from lkey directly
+  int64_t n;
+  __device__ __forceinline__ int operator()(int64_t i) const {
+    if (lkey) {
+      // 16 * log2(exp(-k)) = -k * 16 / ln 2: 16 buckets per octave, evenly spaced
+      // in log2 (weight_bucket spaces them evenly in the mantissa), octave 0 at
+      // 2^-30 as there (bucket 480 starts at a count of 1)
+      const float v = -lkey[i] * 23.083120654223414f;
+      if (!(v == v)) return 0;  // NaN: last
+      const float c = v < -480.0f ? -480.0f : (v > 543.0f ? 543.0f : v);
+      const int b = (int)__builtin_floorf(c) + 480;
+      return b < 1 ? 1 : b;
+    }
+    return weight_bucket(key ? key[i] : tspan[n + i] - tspan[i]);
+  }
+};
 
 __device__ __forceinline__ void span_range(int64_t n, int64_t* lo, int64_t* hi) {
   const int64_t per = (n + SPAN_BLOCKS - 1) / SPAN_BLOCKS;
@@ -179,15 +199,13 @@ __device__ __forceinline__ void span_range(int64_t n, int64_t* lo, int64_t* hi) 
 }
 
 // bh[block][bucket] = trajectories of the block's range in the bucket.
-__global__ void __launch_bounds__(256) span_hist(const double* tspan, const double* key, int64_t n,
-                                                 unsigned* bh) {
+__global__ void __launch_bounds__(256) span_hist(BucketSrc src, unsigned* bh) {
   __shared__ unsigned h[SPAN_BUCKETS];
   for (int k = threadIdx.x; k < SPAN_BUCKETS; k += blockDim.x) h[k] = 0u;
   __syncthreads();
   int64_t lo, hi;
-  span_range(n, &lo, &hi);
-  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
-    atomicAdd(h + span_bucket(tspan, key, n, i), 1u);
+  span_range(src.n, &lo, &hi);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(h + src(i), 1u);
   __syncthreads();
   for (int k = threadIdx.x; k < SPAN_BUCKETS; k += blockDim.x)
     bh[(size_t)blockIdx.x * SPAN_BUCKETS + k] = h[k];
@@ -230,17 +248,16 @@ __global__ void __launch_bounds__(1024) span_scan(unsigned* bh) {
   }
 }
 
-__global__ void __launch_bounds__(256) span_scatter(const double* tspan, const double* key,
-                                                    int64_t n, const unsigned* bh,
+__global__ void __launch_bounds__(256) span_scatter(BucketSrc src, const unsigned* bh,
                                                     int64_t* order) {
   __shared__ unsigned cur[SPAN_BUCKETS];
   for (int k = threadIdx.x; k < SPAN_BUCKETS; k += blockDim.x)
     cur[k] = bh[(size_t)blockIdx.x * SPAN_BUCKETS + k];
   __syncthreads();
   int64_t lo, hi;
-  span_range(n, &lo, &hi);
+  span_range(src.n, &lo, &hi);
   for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
-    order[atomicAdd(cur + span_bucket(tspan, key, n, i), 1u)] = i;
+    order[atomicAdd(cur + src(i), 1u)] = i;
 }
 
 inline int grid_for(int64_t n) {
@@ -311,27 +328,34 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_pack_offse
   return hipGetLastError();
 }
 
-// order[0..n) = trajectories by decreasing span (see span_hist).  bh: scratch
+// order[0..n) = trajectories by decreasing weight (see span_hist).  bh: scratch
 // of SPAN_BLOCKS x SPAN_BUCKETS uint32.  n must fit in 32 bits (the slot counters).
-static hipError_t bucket_order(const double* tspan, const double* key, int64_t n, unsigned* bh,
-                               int64_t* order, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(span_hist, dim3(SPAN_BLOCKS), dim3(256), 0, s, tspan, key, n, bh);
+static hipError_t bucket_order(BucketSrc src, unsigned* bh, int64_t* order, hipStream_t s) {
+  if (src.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(span_hist, dim3(SPAN_BLOCKS), dim3(256), 0, s, src, bh);
   hipLaunchKernelGGL(span_scan, dim3(1), dim3(SPAN_BUCKETS), 0, s, bh);
-  hipLaunchKernelGGL(span_scatter, dim3(SPAN_BLOCKS), dim3(256), 0, s, tspan, key, n, bh, order);
+  hipLaunchKernelGGL(span_scatter, dim3(SPAN_BLOCKS), dim3(256), 0, s, src, bh, order);
   return hipGetLastError();
 }
 
 extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_span_order(
     const double* tspan, int64_t n, unsigned* bh, int64_t* order, hipStream_t s) {
-  return bucket_order(tspan, nullptr, n, bh, order, s);
+  return bucket_order(BucketSrc{tspan, nullptr, nullptr, n}, bh, order, s);
 }
 
 // order[0..n) = trajectories by decreasing key[i] (keys <= 0 and NaN last), on the
 // same buckets and scratch as sdeh_util_span_order.
 extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_key_order(
     const double* key, int64_t n, unsigned* bh, int64_t* order, hipStream_t s) {
-  return bucket_order(nullptr, key, n, bh, order, s);
+  return bucket_order(BucketSrc{nullptr, key, nullptr, n}, bh, order, s);
+}
+
+// order[0..n) = trajectories by increasing lkey[i] (the cost model's -log predicted
+// count: largest predicted cost first), on the 1/16-octave grid of exp(-lkey); NaN
+// keys last.  Same scratch as sdeh_util_span_order.
+extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_logkey_order(
+    const float* lkey, int64_t n, unsigned* bh, int64_t* order, hipStream_t s) {
+  return bucket_order(BucketSrc{nullptr, nullptr, lkey, n}, bh, order, s);
 }
 
 extern "C" __attribute__((visibility("hidden"))) size_t sdeh_util_span_scratch_bytes() {

@@ -154,8 +154,8 @@ EXPORTS = [
     "sdeh_desc_init", "sdeh_abi_version", "sdeh_create", "sdeh_destroy", "sdeh_last_error",
     "sdeh_rhs_info", "sdeh_compile_rhs", "sdeh_solve", "sdeh_get_stats", "sdeh_graph_capture",
     "sdeh_graph_launch", "sdeh_graph_destroy", "sdeh_autotune", "sdeh_synchronize",
-    "sdeh_mlp_forward", "sdeh_mlp_plan_lorenz", "sdeh_mlp_loss_grad", "sdeh_mlp_adam",
-    "sdeh_fixed_grid",
+    "sdeh_mlp_forward", "sdeh_mlp_plan_lorenz", "sdeh_plan_order", "sdeh_mlp_loss_grad",
+    "sdeh_mlp_adam", "sdeh_fixed_grid",
 ]
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -220,6 +220,8 @@ def load_library(path=None):
     lib.sdeh_mlp_forward.restype = ctypes.c_int
     lib.sdeh_mlp_plan_lorenz.argtypes = [vp, vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp]
     lib.sdeh_mlp_plan_lorenz.restype = ctypes.c_int
+    lib.sdeh_plan_order.argtypes = [vp, vp, i64, vp, vp]
+    lib.sdeh_plan_order.restype = ctypes.c_int
     lib.sdeh_mlp_loss_grad.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp]
     lib.sdeh_mlp_loss_grad.restype = ctypes.c_int
     lib.sdeh_mlp_adam.argtypes = [vp, vp, vp, vp, vp, ctypes.c_float, ctypes.c_float,

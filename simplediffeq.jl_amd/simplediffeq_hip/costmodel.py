@@ -178,24 +178,27 @@ class CostModel:
             dt0 = torch.exp(pred[:, 1]).to(torch.float64)
         return pred, order, dt0
 
-    def plan_lorenz(self, p, u0, tspan, stream):
-        """plan_device on the raw binary64 inputs of a Lorenz ensemble (device tensors
-        p (3, n), u0 (3, n), tspan (2, n)): features, forward pass, de-normalisation,
-        sort key and dt0 in one library launch (sdeh_mlp_plan_lorenz), then the stable
-        device sort.  The same (pred, order, dt0) as
-        plan_device(features_device(p, u0, tspan), stream)."""
+    def plan_lorenz(self, p, u0, tspan, stream, pred=True):
+        """The binning step on the raw binary64 inputs of a Lorenz ensemble (device
+        tensors p (3, n), u0 (3, n), tspan (2, n)), entirely in the library:
+        features, forward pass, de-normalisation, sort key and dt0 in one launch
+        (sdeh_mlp_plan_lorenz), then the hand-out order from the keys
+        (sdeh_plan_order: counting sort at 1/16 octave of the predicted cost).
+        Returns (pred (n, 2) or None, order int64, dt0 float64), device tensors."""
         torch = self.torch
         n = p.shape[1]
         with torch.cuda.stream(stream):
             key = torch.empty(n, dtype=torch.float32, device=self.dev)
             dt0 = torch.empty(n, dtype=torch.float64, device=self.dev)
-            pred = torch.empty((n, NOUT), dtype=torch.float32, device=self.dev)
+            order = torch.empty(n, dtype=torch.int64, device=self.dev)
+            pr = torch.empty((n, NOUT), dtype=torch.float32, device=self.dev) if pred else None
             norm = (self.y_mean[0], self.y_mean[1], self.y_std[0], self.y_std[1])
             self.ctx.mlp_plan_lorenz(self.params.data_ptr(), p.data_ptr(), u0.data_ptr(),
                                      tspan.data_ptr(), n, FEAT_LORENZ, norm, key.data_ptr(),
-                                     dt0.data_ptr(), pred.data_ptr(), stream=stream.cuda_stream)
-            order = torch.argsort(key, stable=True)
-        return pred, order, dt0
+                                     dt0.data_ptr(), pr.data_ptr() if pred else None,
+                                     stream=stream.cuda_stream)
+            self.ctx.plan_order(key.data_ptr(), n, order.data_ptr(), stream=stream.cuda_stream)
+        return pr, order, dt0
 
 
 def wave_efficiency(attempts, order=None, wave=64):
@@ -209,3 +212,33 @@ def wave_efficiency(attempts, order=None, wave=64):
         a = np.concatenate([a, np.zeros(pad)])
     w = a.reshape(-1, wave)
     return float(a.sum() / (wave * w.max(axis=1)).sum())
+
+
+def train_lorenz(ctx, n=1 << 20, seed=40, steps=1500, batch=65536, lr=3e-3, abstol=1e-7,
+                 reltol=1e-5, model_seed=0, device="cuda:0", log_every=200):
+    """Step logs and training of the Lorenz cost model, all on the GPU: n plain
+    binary64 GPUSimpleATsit5 solves of the training distribution (sigma ~ U(8,12),
+    rho ~ U(0,28), beta ~ U(2,3), u0 ~ U(-1,1)^3, t_end ~ LogUniform(0.5,10); seed
+    `seed`) give the accepted-step counts, and `steps` Adam steps of the library's
+    MLP kernels fit the network to them.  Returns (model, loss history, seconds of
+    training, seconds of the step-log solve)."""
+    import time
+    import torch
+    from . import inputs, runtime
+    u0, p, ts = inputs.lorenz_costmodel(n, seed=seed)
+    d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=n,
+                          dt=float(np.float32(0.1)), abstol=abstol, reltol=reltol)
+    g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device=device)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.solve(d, g, sync=True)
+    t_logs = time.perf_counter() - t0
+    if not (g.numpy("retcode") == 0).all():
+        raise RuntimeError("step-log solves did not all finish")
+    x = features(p, u0, ts)
+    y = targets(g.numpy("naccept"), ts)
+    del g
+    cm = CostModel(ctx, device=device, seed=model_seed)
+    t0 = time.perf_counter()
+    hist = cm.fit(x, y, steps=steps, batch=batch, lr=lr, log_every=log_every)
+    return cm, hist, time.perf_counter() - t0, t_logs

@@ -283,6 +283,12 @@ class Context:
             self.h, params, p, u0, tspan, int(n), ctypes.cast(fa, ctypes.c_void_p),
             ctypes.cast(ya, ctypes.c_void_p), key, dt0, pred, ctypes.c_void_p(stream)))
 
+    def plan_order(self, key, n, order, stream=None):
+        """sdeh_plan_order on device pointers: order (int64 [n]) by increasing FP32
+        key (decreasing predicted cost), bucketed at 1/16 octave of exp(-key)."""
+        self._check(self.lib.sdeh_plan_order(self.h, key, int(n), order,
+                                             ctypes.c_void_p(stream)))
+
     def mlp_loss_grad(self, params, x, y, n, loss, grad, stream=None):
         self._check(self.lib.sdeh_mlp_loss_grad(self.h, params, x, y, int(n), loss, grad,
                                                 ctypes.c_void_p(stream)))

@@ -1,0 +1,165 @@
+"""The learned-binning workload at full size on the GPU: the cost model trained on
+2^20 plain binary64 ATsit5 step logs (Lorenz, rho ~ U(0,28), t_end ~
+LogUniform(0.5,10); seed 40), then 4,194,304 fresh trajectories (seed 4) solved
+with binary32 Kahan-compensated ATsit5 at abstol 1e-7, reltol 1e-5, binned by the
+model's predicted cost (planning on the device: sdeh_mlp_plan_lorenz then
+sdeh_plan_order), against
+
+  * the plain unsorted binary64 one-launch kernel (index order): the binned
+    persistent binary32 solve, planning included, must run at >= 3x its
+    trajectories/s, with median endpoint error <= 1e-5 relative to binary64 AVern9
+    at 1e-14 (on the first 65,536 trajectories);
+  * the same solve without the trained network (ablation): untrained weights and no
+    order at all, measured by the one-launch kernel's wave efficiency (fraction of
+    lane-iterations doing work, from the deterministic attempt counts) and by time;
+    the model-free longest-span-first order is measured beside them.
+
+Times are HIP-event device times on one stream (median of several solves).  The
+numbers are printed as one JSON line so the test log carries them."""
+import json
+
+import numpy as np
+import pytest
+
+from simplediffeq_hip import _abi, costmodel, inputs, runtime
+
+pytestmark = pytest.mark.gpu
+
+N_TRAIN, N_EVAL, N_ERR = 1 << 20, 1 << 22, 1 << 16
+TOL = dict(abstol=1e-7, reltol=1e-5)
+F32_01 = float(np.float32(0.1))
+
+
+@pytest.fixture(scope="module")
+def setup(ctx):
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    print("costmodel: training on %d step logs" % N_TRAIN, flush=True)
+    cm, hist, t_train, _ = costmodel.train_lorenz(ctx, n=N_TRAIN, seed=40, steps=1500)
+    u0, p, ts = inputs.lorenz_costmodel(N_EVAL, seed=4)
+    pd, u0d, tsd = (torch.as_tensor(a, device=dev) for a in (p, u0, ts))
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.synchronize()
+    return dict(torch=torch, dev=dev, cm=cm, hist=hist, t_train=t_train, u0=u0, p=p, ts=ts,
+                pd=pd, u0d=u0d, tsd=tsd, stream=stream, rec={})
+
+
+def _events(torch, stream, fn, reps=5):
+    """Median device time (ms) of fn() on `stream` over `reps` runs after one warm-up."""
+    fn()
+    torch.cuda.synchronize()
+    out = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out.append(e0.elapsed_time(e1))
+    return float(np.median(out))
+
+
+def _solve(ctx, S, dtype, sched, precision=_abi.PREC_PLAIN, order=None, span=False):
+    d = runtime.make_desc(_abi.ALG_ATSIT5, dtype=dtype, rhs=_abi.RHS_LORENZ, n_traj=N_EVAL,
+                          dt=F32_01, schedule=sched, precision=precision,
+                          span_order=0 if span else 1, **TOL)
+    g = runtime.EnsembleBuffers(d, 3, S["u0d"], S["pd"], tspan=S["tsd"], order=order,
+                                backend="torch", device=S["dev"])
+    ctx.solve(d, g, sync=True)
+    st = ctx.stats()
+    assert (g.numpy("retcode") == 0).all()
+    return d, g, st
+
+
+def _time(ctx, S, d, g):
+    s = S["stream"]
+    return _events(S["torch"], s, lambda: ctx.solve(d, g, stream=s.cuda_stream, sync=False))
+
+
+def _plan(S, cm):
+    return cm.plan_lorenz(S["pd"], S["u0d"], S["tsd"], S["stream"], pred=False)
+
+
+def test_trained_binning_meets_target(ctx, setup):
+    S = setup
+    torch, cm = S["torch"], S["cm"]
+    assert S["hist"][-1][1] < 0.5 * S["hist"][0][1]  # training reduced the loss
+    # baseline: plain binary64, one launch, index order
+    d0, g0, st0 = _solve(ctx, S, _abi.F64, _abi.SCHED_ONE_LAUNCH)
+    ms_base = _time(ctx, S, d0, g0)
+    # planning (features, forward, key, dt0; then the library's bucket order)
+    plan_ms = _events(torch, S["stream"], lambda: _plan(S, cm))
+    _, order, _ = _plan(S, cm)
+    torch.cuda.synchronize()
+    o = order.cpu().numpy()
+    assert np.array_equal(np.sort(o), np.arange(N_EVAL))
+    res = {}
+    for name, sched in (("persistent", _abi.SCHED_PERSISTENT), ("one_launch", _abi.SCHED_ONE_LAUNCH)):
+        d, g, st = _solve(ctx, S, _abi.F32, sched, _abi.PREC_KAHAN, order=order)
+        res[name] = (_time(ctx, S, d, g), g)
+    # binning changes no trajectory's result
+    du, gu, _ = _solve(ctx, S, _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN)
+    for f in ("u_end", "t_end", "naccept", "nreject"):
+        assert np.array_equal(res["persistent"][1].numpy(f), gu.numpy(f)), f
+    # error against binary64 AVern9 at 1e-14 on the first N_ERR trajectories
+    dr = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_LORENZ, n_traj=N_ERR, dt=F32_01,
+                           abstol=1e-14, reltol=1e-14)
+    gr = runtime.EnsembleBuffers(dr, 3, S["u0"][:, :N_ERR], S["p"][:, :N_ERR],
+                                 tspan=S["ts"][:, :N_ERR], backend="torch", device=S["dev"])
+    ctx.solve(dr, gr, sync=True)
+    uref = gr.numpy("u_end")
+    ok = gr.numpy("retcode") == 0
+    assert ok.mean() > 0.999
+
+    def err(g):
+        ue = g.numpy("u_end")[:, :N_ERR].astype(np.float64)
+        e = np.max(np.abs(ue - uref), axis=0) / np.max(np.abs(uref), axis=0)
+        return float(np.median(e[ok]))
+
+    e32 = err(res["persistent"][1])
+    e64 = err(g0)
+    speed_p = ms_base / (plan_ms + res["persistent"][0])
+    speed_o = ms_base / (plan_ms + res["one_launch"][0])
+    S["rec"].update(base_ms=ms_base, plan_ms=plan_ms, order=order)
+    print(json.dumps({"costmodel_target": {
+        "f64_one_launch_index_order_ms": ms_base, "plan_ms": plan_ms,
+        "f32k_persistent_binned_ms": res["persistent"][0],
+        "f32k_one_launch_binned_ms": res["one_launch"][0],
+        "speedup_persistent_with_plan": speed_p, "speedup_one_launch_with_plan": speed_o,
+        "median_err_f32k_binned": e32, "median_err_f64": e64,
+        "train_s": S["t_train"], "loss": [S["hist"][0][1], S["hist"][-1][1]]}}), flush=True)
+    assert speed_p >= 3.0, speed_p
+    assert e32 <= 1e-5, e32
+
+
+def test_network_gain_ablation(ctx, setup):
+    S = setup
+    torch, cm = S["torch"], S["cm"]
+    # attempt counts of every trajectory (identical under every order)
+    _, g64, _ = _solve(ctx, S, _abi.F64, _abi.SCHED_ONE_LAUNCH)
+    att = g64.numpy("naccept").astype(np.int64) + g64.numpy("nreject")
+    untrained = costmodel.CostModel(ctx, seed=0)
+    untrained.y_mean, untrained.y_std = cm.y_mean, cm.y_std
+    orders = {"trained": _plan(S, cm)[1], "untrained": _plan(S, untrained)[1], "none": None}
+    torch.cuda.synchronize()
+    plan_ms = {"trained": _events(torch, S["stream"], lambda: _plan(S, cm)),
+               "untrained": _events(torch, S["stream"], lambda: _plan(S, untrained)), "none": 0.0}
+    eff, ms = {}, {}
+    for k, od in orders.items():
+        eff[k] = costmodel.wave_efficiency(att, None if od is None else od.cpu().numpy())
+        d, g, _ = _solve(ctx, S, _abi.F32, _abi.SCHED_ONE_LAUNCH, _abi.PREC_KAHAN, order=od)
+        ms[k] = _time(ctx, S, d, g)
+    # model-free references: longest span first (one launch) and the persistent queue
+    d, g, _ = _solve(ctx, S, _abi.F32, _abi.SCHED_ONE_LAUNCH, _abi.PREC_KAHAN, span=True)
+    ms["span_order"] = _time(ctx, S, d, g)
+    d, g, _ = _solve(ctx, S, _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN)
+    ms["persistent_index_order"] = _time(ctx, S, d, g)
+    print(json.dumps({"costmodel_ablation": {
+        "wave_efficiency_one_launch": eff, "f32k_one_launch_ms": ms, "plan_ms": plan_ms,
+        "trained_with_plan_ms": ms["trained"] + plan_ms["trained"]}}), flush=True)
+    # the trained network is what makes the binning work: far better waves than
+    # untrained weights or no order, and a faster solve than either, planning included
+    assert eff["trained"] > eff["untrained"] + 0.2 and eff["trained"] > eff["none"] + 0.2
+    assert ms["trained"] + plan_ms["trained"] < ms["untrained"] + plan_ms["untrained"]
+    assert ms["trained"] + plan_ms["trained"] < ms["none"]
