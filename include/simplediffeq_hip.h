@@ -261,6 +261,22 @@ int sdeh_compile_rhs(sdeh_ctx* ctx, const char* src, const char* fn_name, int32_
 int sdeh_solve(sdeh_ctx* ctx, const sdeh_solve_desc* d, const sdeh_io* io, int32_t memory,
                void* stream, int32_t sync);
 
+/* sdeh_solve without any device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Device buffers only (as
+ * SDEH_MEM_DEVICE); the caller passes every input and output, naccept and nreject
+ * included.  Output: endpoint, or saveat with at most 1024 times (more:
+ * SDEH_E_INVALID); save_everystep is SDEH_E_UNSUPPORTED, and so is io->order.
+ * Trajectories are handed to lanes in index order (no longest-span-first order),
+ * and a staged launch shape runs as the persistent one; results are the same bits as
+ * sdeh_solve's.  The library's scratch for it is one static device array per GPU
+ * (8,256 bytes: the work-queue counter, the counts, the saveat times), shared by
+ * every sdeh_solve_static on that GPU: calls that overlap on one GPU (other contexts,
+ * other streams, other threads) race on it and must be serialised by the caller.
+ * `stream` and `sync` as in sdeh_solve; stats are filled when sync != 0. */
+int sdeh_solve_static(sdeh_ctx* ctx, const sdeh_solve_desc* d, const sdeh_io* io, void* stream,
+                      int32_t sync);
+
 int sdeh_get_stats(const sdeh_ctx* ctx, sdeh_stats* out);
 
 /* Repeated ensembles: capture one device-memory solve into a HIP graph and

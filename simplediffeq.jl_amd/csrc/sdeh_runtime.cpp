@@ -52,6 +52,8 @@ SDEH_HIDDEN hipError_t sdeh_util_key_order(const double*, int64_t, unsigned*, in
 SDEH_HIDDEN hipError_t sdeh_util_logkey_order(const float*, int64_t, unsigned*, int64_t*,
                                               hipStream_t);
 SDEH_HIDDEN size_t sdeh_util_span_scratch_bytes();
+SDEH_HIDDEN hipError_t sdeh_util_static_scratch(void**, size_t*);
+SDEH_HIDDEN int sdeh_util_static_saveat_max();
 SDEH_HIDDEN hipError_t sdeh_util_pack_offsets(const int32_t*, int32_t, int64_t, int64_t*, int64_t,
                                               int64_t*, int32_t*, unsigned long long*,
                                               hipStream_t);
@@ -177,6 +179,10 @@ struct sdeh_ctx {
   std::map<TuneKey, Shape> tuned;
   std::map<int64_t, GraphRec> graphs;
   int64_t next_graph = 1;
+  // The device address of the library's static scratch on this context's device
+  // (sdeh_solve_static), looked up at creation.
+  unsigned long long* static_scratch = nullptr;
+  size_t static_bytes = 0;
 };
 
 namespace {
@@ -1091,6 +1097,113 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   return SDEH_OK;
 }
 
+// The allocation-free solve (sdeh_solve_static): device buffers, endpoint or saveat
+// output, one-launch or persistent schedule.  Its only scratch is the library's
+// static device array: words 0..5 (queue counter, counts), then the saveat times.
+// No arena, no hipMalloc: what would need per-ensemble scratch (span order, staged
+// records, the permutation check of io->order, internal count arrays, packed
+// save_everystep) is refused or left out.
+int solve_static_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, hipStream_t s,
+                      int32_t sync) {
+  const auto t_start = std::chrono::steady_clock::now();
+  Plan pl;
+  int r = plan_solve(c, d, io, &pl);
+  if (r) return r;
+  if (pl.save == SAVE_STEPS)
+    return fail(c, SDEH_E_UNSUPPORTED,
+                "the allocation-free solve writes endpoint or saveat output (set "
+                "save_everystep = 0 or give saveat)");
+  const int64_t max_saveat = sdeh_util_static_saveat_max();
+  if (pl.save == SAVE_AT && d->n_saveat > max_saveat)
+    return fail(c, SDEH_E_INVALID,
+                "the allocation-free solve holds at most %lld saveat times (got %lld)",
+                (long long)max_saveat, (long long)d->n_saveat);
+  if (io->order)
+    return fail(c, SDEH_E_UNSUPPORTED,
+                "the allocation-free solve takes no order (its check needs per-trajectory "
+                "scratch); use sdeh_solve");
+  const int64_t n = d->n_traj;
+  if (n > 0 && (!io->naccept || !io->nreject))
+    return fail(c, SDEH_E_INVALID, "the allocation-free solve needs naccept and nreject buffers");
+  // index order (the span order needs per-trajectory scratch); a staged shape runs
+  // as the persistent launch
+  pl.span = false;
+  if (pl.staged || pl.shape.schedule == SDEH_SCHED_STAGED) {
+    pl.staged = false;
+    pl.shape.schedule = SDEH_SCHED_PERSISTENT;
+  }
+  const Sizes sz = sizes_of(d, io, pl);
+  if (n > 0) {
+    if (!io->u0 || !io->u_end || !io->t_end || !io->retcode)
+      return fail(c, SDEH_E_INVALID, "u0, u_end, t_end and retcode are required");
+    if (pl.n_param > 0 && !io->p) return fail(c, SDEH_E_INVALID, "p is required (n_param > 0)");
+    if (pl.n_shared > 0 && !io->p_shared)
+      return fail(c, SDEH_E_INVALID, "p_shared is required for this rhs");
+    if (pl.save == SAVE_AT && !io->u_saveat)
+      return fail(c, SDEH_E_INVALID, "u_saveat is required with saveat");
+    r = check_dev(c, io->u0, sz.u0, "u0");
+    if (!r) r = check_dev(c, io->p, sz.p, "p");
+    if (!r) r = check_dev(c, io->p_shared, sz.ps, "p_shared");
+    if (!r && io->tspan) r = check_dev(c, io->tspan, sz.tspan, "tspan");
+    if (!r && io->dt0) r = check_dev(c, io->dt0, sz.dt0, "dt0");
+    if (!r) r = check_dev(c, io->u_end, sz.u_end, "u_end");
+    if (!r) r = check_dev(c, io->t_end, sz.t_end, "t_end");
+    if (!r) r = check_dev(c, io->retcode, sz.i32, "retcode");
+    if (!r) r = check_dev(c, io->naccept, sz.i32, "naccept");
+    if (!r) r = check_dev(c, io->nreject, sz.i32, "nreject");
+    if (!r && pl.save == SAVE_AT) r = check_dev(c, io->u_saveat, sz.u_saveat, "u_saveat");
+    if (r) return r;
+  }
+  // this call's stream follows the context's earlier ones (their kernels may still
+  // read the context's events and the static scratch)
+  r = after_pending(c, s);
+  if (r) return r;
+  unsigned long long* const words = c->static_scratch;  // the device address, never the symbol
+  double* const d_saveat = (double*)(words + 8);
+  sdeh_kargs& a = pl.a;
+  if (sz.saveat) HIPCHK(c, hipMemcpyAsync(d_saveat, d->saveat, sz.saveat, hipMemcpyHostToDevice, s));
+  a.saveat = d_saveat;
+  a.u0 = io->u0;
+  a.p = io->p;
+  a.ps = io->p_shared;
+  a.tspan = io->tspan;
+  a.dt0 = io->dt0;
+  a.order = nullptr;
+  a.u_end = io->u_end;
+  a.t_end = io->t_end;
+  a.retcode = io->retcode;
+  a.naccept = io->naccept;
+  a.nreject = io->nreject;
+  a.u_saveat = io->u_saveat;
+  r = run_plan(c, pl, s, words, true);  // run_plan clears the queue word before a persistent launch
+  if (r) return r;
+  note_pending(c, s, false);
+  c->stats = sdeh_stats{};
+  c->stats.block = pl.shape.block;
+  c->stats.grid = pl.shape.grid;
+  c->stats.chunk = pl.shape.chunk;
+  c->stats.schedule = pl.shape.schedule;
+  c->stats.lanes = pl.k.lanes;
+  if (!sync) return SDEH_OK;
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  if (n > 0) {
+    HIPCHK(c, sdeh_util_zero_u64(words + 1, 4, s));
+    HIPCHK(c, sdeh_util_count(a.naccept, a.nreject, a.retcode, n, words + 1, s));
+    HIPCHK(c, hipMemcpyAsync(cnt, words + 1, sizeof cnt, hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(c, hipStreamSynchronize(s));
+  note_pending(c, s, true);
+  float ms = 0.f;
+  if (n > 0) HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->stats.kernel_ms = ms;
+  c->stats.attempts = (int64_t)cnt[0];
+  c->stats.accepts = (int64_t)cnt[1];
+  c->stats.total_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  if (cnt[2]) return fail(c, SDEH_E_DTMIN, "%llu trajectories stopped at dt < dtmin", cnt[2]);
+  return SDEH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1137,6 +1250,16 @@ int sdeh_create(sdeh_ctx** out, int device) {
       sdeh_destroy(c);
       return SDEH_E_HIP;
     }
+  {
+    void* q = nullptr;
+    size_t b = 0;
+    if (sdeh_util_static_scratch(&q, &b) != hipSuccess || !q) {
+      sdeh_destroy(c);
+      return SDEH_E_HIP;
+    }
+    c->static_scratch = (unsigned long long*)q;
+    c->static_bytes = b;
+  }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
   if (c->cus <= 0) c->cus = 256;
@@ -1218,6 +1341,13 @@ int sdeh_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   if (!c) return SDEH_E_INVALID;
   HIPCHK(c, hipSetDevice(c->device));
   return solve_impl(c, d, io, memory, stream ? (hipStream_t)stream : c->stream, sync);
+}
+
+int sdeh_solve_static(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, void* stream,
+                      int32_t sync) {
+  if (!c || !d || !io) return SDEH_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
+  return solve_static_impl(c, d, io, stream ? (hipStream_t)stream : c->stream, sync);
 }
 
 int sdeh_get_stats(const sdeh_ctx* c, sdeh_stats* out) {

@@ -4,6 +4,19 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
+// Device scratch of the allocation-free solve (sdeh_solve_static): words [0, 8) are
+// the work-queue counter and the per-solve counts, the rest holds up to
+// SDEH_STATIC_SAVEAT saveat times.  One copy per GPU, alive until the process ends;
+// every sdeh_solve_static on a GPU shares it (calls that overlap on one GPU race).
+// Host code never passes `scratch` itself to a kernel (in host code the name is a
+// stand-in, not a device address): sdeh_util_static_scratch() looks the address up
+// with hipGetSymbolAddress once per device, and callers use that pointer.
+#define SDEH_STATIC_WORDS 8
+#define SDEH_STATIC_SAVEAT 1024
+static __device__ __align__(256) unsigned long long scratch[SDEH_STATIC_WORDS + SDEH_STATIC_SAVEAT];
+
 namespace {
 
 __global__ void fill_f64(double* p, double v, int64_t n) {
@@ -360,4 +373,32 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_logkey_ord
 
 extern "C" __attribute__((visibility("hidden"))) size_t sdeh_util_span_scratch_bytes() {
   return (size_t)SPAN_BLOCKS * SPAN_BUCKETS * sizeof(unsigned);
+}
+
+// The device address of `scratch` on the current device, looked up once per device
+// (with that device current) and kept; *bytes = its size.  The first call on a
+// device must come from the library's set-up (sdeh_create), before any solve uses it.
+extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_static_scratch(
+    void** ptr, size_t* bytes) {
+  constexpr int kMaxDev = 64;
+  static std::mutex mu;
+  static void* addr[kMaxDev] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDev) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* q = nullptr;
+    e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = q;
+  }
+  *ptr = addr[dev];
+  *bytes = sizeof(scratch);
+  return hipSuccess;
+}
+
+extern "C" __attribute__((visibility("hidden"))) int sdeh_util_static_saveat_max() {
+  return SDEH_STATIC_SAVEAT;
 }

@@ -152,7 +152,8 @@ def _f32(x):
 # Exported symbols of libsimplediffeq_hip (every function include/simplediffeq_hip.h declares).
 EXPORTS = [
     "sdeh_desc_init", "sdeh_abi_version", "sdeh_create", "sdeh_destroy", "sdeh_last_error",
-    "sdeh_rhs_info", "sdeh_compile_rhs", "sdeh_solve", "sdeh_get_stats", "sdeh_graph_capture",
+    "sdeh_rhs_info", "sdeh_compile_rhs", "sdeh_solve", "sdeh_solve_static", "sdeh_get_stats",
+    "sdeh_graph_capture",
     "sdeh_graph_launch", "sdeh_graph_destroy", "sdeh_autotune", "sdeh_synchronize",
     "sdeh_mlp_forward", "sdeh_mlp_plan_lorenz", "sdeh_plan_order", "sdeh_mlp_loss_grad",
     "sdeh_mlp_adam", "sdeh_fixed_grid",
@@ -202,6 +203,9 @@ def load_library(path=None):
     lib.sdeh_solve.argtypes = [vp, ctypes.POINTER(SolveDesc), ctypes.POINTER(SolveIO), i32, vp,
                                i32]
     lib.sdeh_solve.restype = ctypes.c_int
+    lib.sdeh_solve_static.argtypes = [vp, ctypes.POINTER(SolveDesc), ctypes.POINTER(SolveIO),
+                                      vp, i32]
+    lib.sdeh_solve_static.restype = ctypes.c_int
     lib.sdeh_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     lib.sdeh_get_stats.restype = ctypes.c_int
     lib.sdeh_graph_capture.argtypes = [vp, ctypes.POINTER(SolveDesc), ctypes.POINTER(SolveIO),

@@ -237,6 +237,17 @@ class Context:
                                  ctypes.c_void_p(stream), 1 if sync else 0)
         return self._check(st, allow)
 
+    def solve_static(self, desc, bufs, stream=None, sync=True, allow=(_abi.E_DTMIN,)):
+        """sdeh_solve_static (no device allocation) on device (torch) EnsembleBuffers;
+        returns the status."""
+        if bufs.backend != "torch":
+            raise ValueError("the allocation-free solve takes device buffers")
+        if stream is None:
+            stream = self._torch_stream(bufs)
+        st = self.lib.sdeh_solve_static(self.h, ctypes.byref(desc), ctypes.byref(bufs.io),
+                                        ctypes.c_void_p(stream), 1 if sync else 0)
+        return self._check(st, allow)
+
     def stats(self):
         s = _abi.Stats()
         self._check(self.lib.sdeh_get_stats(self.h, ctypes.byref(s)))
