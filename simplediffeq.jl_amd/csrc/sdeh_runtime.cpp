@@ -161,6 +161,7 @@ struct sdeh_ctx {
   hipStream_t s_k2 = nullptr;  // the pipeline's second kernel stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_pipe[2 * kPipeChunks] = {};  // per chunk: inputs landed, kernel done
+  hipEvent_t ev_pipe_start = nullptr;         // the caller's stream at a pipeline's start
   std::string err;
   sdeh_stats stats{};
   int cus = 0;
@@ -786,6 +787,14 @@ int run_host_pipeline(sdeh_ctx* c, Plan& pl, const sdeh_io* io, hipStream_t s) {
   const size_t es = pl.es;
   const bool tm = base.traj_major != 0;
   const int K = kPipeChunks;
+  // Everything already queued on `s` comes first: an earlier device-memory solve on
+  // `s` that may still be running (its arena scratch is what the chunk uploads
+  // overwrite) and this call's own uploads of the saveat times and p_shared, which
+  // the chunks on s_k2 read.  The three pipeline streams wait for that point.
+  HIPCHK(c, hipEventRecord(c->ev_pipe_start, s));
+  HIPCHK(c, hipStreamWaitEvent(c->s_in, c->ev_pipe_start, 0));
+  HIPCHK(c, hipStreamWaitEvent(c->s_k2, c->ev_pipe_start, 0));
+  HIPCHK(c, hipStreamWaitEvent(c->s_out, c->ev_pipe_start, 0));
   auto offs = [](const void* p, int64_t rows, int64_t a, size_t e) -> void* {
     return p ? (void*)((char*)p + (size_t)rows * a * e) : nullptr;
   };
@@ -1250,6 +1259,10 @@ int sdeh_create(sdeh_ctx** out, int device) {
       sdeh_destroy(c);
       return SDEH_E_HIP;
     }
+  if (hipEventCreateWithFlags(&c->ev_pipe_start, hipEventDisableTiming) != hipSuccess) {
+    sdeh_destroy(c);
+    return SDEH_E_HIP;
+  }
   {
     void* q = nullptr;
     size_t b = 0;
@@ -1282,6 +1295,7 @@ void sdeh_destroy(sdeh_ctx* c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (hipEvent_t e : c->ev_pipe)
     if (e) (void)hipEventDestroy(e);
+  if (c->ev_pipe_start) (void)hipEventDestroy(c->ev_pipe_start);
   if (c->s_in) {
     (void)hipStreamSynchronize(c->s_in);
     (void)hipStreamDestroy(c->s_in);

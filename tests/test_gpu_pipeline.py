@@ -129,3 +129,42 @@ def test_pinned_and_pageable_host_arrays(ctx):
     g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device="cuda:0")
     ctx.solve(d, g)
     _same(h, g, BASE)
+
+
+def test_host_pipeline_after_async_device_solve(ctx):
+    """An asynchronous device-memory solve with per-trajectory spans on the context's
+    stream (its longest-span-first order lives in the context's scratch), then at once
+    a pipelined host-memory solve with saveat and p_shared, whose chunk uploads go to
+    that same scratch on the pipeline's own streams: the pipeline must wait for the
+    earlier solve.  Both against the oracle."""
+    import ctypes
+    from helpers import assert_bit_identical
+    from oracle import pyoracle
+    torch = pytest.importorskip("torch")
+    # grow the context's scratch first, so the second call reuses it in place
+    big = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=2 * N, tspan=(0.0, 0.5))
+    u0b, pb = inputs.lorenz_wide(2 * N, seed=21)
+    ctx.solve(big, runtime.EnsembleBuffers(big, 3, u0b, pb))
+    # 1. device solve, queued on the context's stream, not waited for
+    m = 1 << 17
+    u0, p, ts = inputs.lorenz_uneven(m, seed=22)
+    ts[1] = np.minimum(ts[1], 4.0)
+    d1 = runtime.make_desc(_abi.ALG_AVERN7, rhs=_abi.RHS_LORENZ, n_traj=m, dt=0.1,
+                           abstol=1e-8, reltol=1e-8)
+    g1 = runtime.EnsembleBuffers(d1, 3, u0, p, tspan=ts, backend="torch", device="cuda:0")
+    torch.cuda.synchronize()
+    st = ctx.lib.sdeh_solve(ctx.h, ctypes.byref(d1), ctypes.byref(g1.io), _abi.MEM_DEVICE, None, 0)
+    assert st == 0, ctx.last_error()
+    # 2. at once: host solve, pipelined (>= 2^17 trajectories), saveat and p_shared
+    u0g, r, A = inputs.glv16(N, seed=23)
+    d2 = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_GLV16, n_traj=N, tspan=(0.0, 5.0),
+                           dt=0.1, abstol=1e-6, reltol=1e-6, saveat=np.linspace(0.0, 5.0, 6))
+    h2 = runtime.EnsembleBuffers(d2, 16, u0g, r, A)
+    ctx.solve(d2, h2)
+    ctx.synchronize()
+    c1 = runtime.EnsembleBuffers(d1, 3, u0, p, tspan=ts)
+    assert pyoracle.solve(d1, c1, nthreads=8) == 0
+    assert_bit_identical(g1, c1)
+    c2 = runtime.EnsembleBuffers(d2, 16, u0g, r, A)
+    assert pyoracle.solve(d2, c2, nthreads=8) == 0
+    assert_bit_identical(h2, c2, ("u_end", "t_end", "retcode", "naccept", "nreject", "u_saveat"))
