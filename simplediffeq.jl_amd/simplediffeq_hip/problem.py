@@ -271,13 +271,21 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=None, batch_size=None, dt
     (u, converged) folds the batch in, and converged stops the loop early."""
     if not isinstance(alg, _Alg):
         raise TypeError("alg must be one of the GPUSimple* algorithms")
+    # The six Tsit5/Verner solvers keep their step and end times in a vector of
+    # dt's element type, ts = Vector{eltype(dt)} (gpuatsit5.jl:72,226; gpuvern7.jl:72,
+    # 321; gpuvern9.jl:72,432), and push! rounds each binary64 t into it
+    # (gpuatsit5.jl:114,303,325).  The default dt is the Float32 literal 0.1f0, so
+    # their sol.t is binary32 unless dt is given in binary64 (a Python float); a
+    # numpy float32 dt is the Float32 case.  saveat times are the caller's array, and
+    # RK4/Euler return their Float64 range (gpurk4.jl:66).
+    t_f32 = dt is None or isinstance(dt, np.float32)
     if dt is None:
         if alg.dt_required:
             raise ValueError("dt is required for this algorithm")
         dt = _f32(0.1)
-    kw = dict(dt=dt, saveat=saveat, save_everystep=save_everystep, abstol=abstol,
+    kw = dict(dt=float(dt), saveat=saveat, save_everystep=save_everystep, abstol=abstol,
               reltol=reltol, max_saves=max_saves, dtype=dtype, maxiters=maxiters,
-              quirks=quirks)
+              quirks=quirks, t_f32=t_f32)
     if isinstance(prob, ODEProblem):
         sols, _ = _solve_many(EnsembleProblem(prob), alg, ensemblealg or EnsembleHIP(), [0],
                               [1], dt0s, order, kw)
@@ -427,6 +435,9 @@ def _solve_many(eprob, alg, ens, idx, repeats, dt0s, order, kw):
         raise DtLessThanMin("dt<dtmin")
 
     t0s = tspans[0] if tspans is not None else np.full(n, base.tspan[0])
+    # ts = Vector{eltype(dt)} for the six Tsit5/Verner solvers without saveat
+    t_type = (np.float32 if kw["t_f32"] and saveat_arr is None
+              and alg.code not in _abi.FIXED_GRID_ONLY_ALGS else np.float64)
     sols = []
     for i in range(n):
         rc = _abi.RETCODE_NAMES[int(bufs.retcode[i])]
@@ -435,9 +446,10 @@ def _solve_many(eprob, alg, ens, idx, repeats, dt0s, order, kw):
             u = list(bufs.u_saveat[i])
         elif everystep:
             t, us = bufs.steps_of(i)
+            t = t.astype(t_type)
             u = list(us)
         else:
-            t = np.array([t0s[i], bufs.t_end[i]])
+            t = np.array([t0s[i], bufs.t_end[i]]).astype(t_type)
             u = [u0s[:, i].astype(bufs.u_end.dtype), bufs.u_end[:, i]]
         if base.scalar_u:
             u = [x[0] for x in u]

@@ -168,3 +168,29 @@ def test_scalar_problem_keeps_scalar_states():
     sol = sd.solve(prob, sd.GPUSimpleTsit5(), dt=0.01, save_everystep=False)
     assert len(sol.u) == 2 and np.ndim(sol.u[-1]) == 0
     assert abs(float(sol.u[-1]) - np.exp(-1.0)) < 1e-9
+
+
+def test_solution_times_take_dt_element_type():
+    # ts = Vector{eltype(dt)} and push!(ts, t) (gpuatsit5.jl:72,226,303,325;
+    # gpuvern7.jl:72,321; gpuvern9.jl:72,432): with the default dt = 0.1f0 the six
+    # Tsit5/Verner solvers return Float32 times, the binary64 t rounded; a binary64
+    # dt keeps binary64 times; saveat returns the caller's times; RK4/Euler return
+    # their Float64 range (gpurk4.jl:66).
+    prob = sd.ODEProblem(sd.rhs.lorenz, np.ones(3), (0.0, 0.7), [10.0, 28.0, 8 / 3])
+    for alg in (sd.GPUSimpleATsit5(), sd.GPUSimpleAVern7(), sd.GPUSimpleAVern9()):
+        s32 = sd.solve(prob, alg, save_everystep=False)
+        s64 = sd.solve(prob, alg, dt=float(F(0.1)), save_everystep=False)
+        assert s32.t.dtype == np.float32 and s64.t.dtype == np.float64
+        assert s64.t[-1] == 0.7 and s32.t[-1] == F(0.7) and float(s32.t[-1]) != 0.7
+        assert bits_equal(s32.u[-1], s64.u[-1])  # the same solve, only ts differs
+        e32 = sd.solve(prob, alg)  # save_everystep = true
+        e64 = sd.solve(prob, alg, dt=float(F(0.1)))
+        assert e32.t.dtype == np.float32 and np.array_equal(e32.t, e64.t.astype(np.float32))
+        sf = sd.solve(prob, alg, dt=F(0.1), save_everystep=False)  # a Float32 dt given
+        assert sf.t.dtype == np.float32
+    sa = sd.solve(prob, sd.GPUSimpleATsit5(), saveat=[0.0, 0.35, 0.7])
+    assert sa.t.dtype == np.float64 and list(sa.t) == [0.0, 0.35, 0.7]
+    for alg in (sd.GPUSimpleTsit5(), sd.GPUSimpleVern9()):
+        assert sd.solve(prob, alg, dt=F(0.1), save_everystep=False).t.dtype == np.float32
+    r = sd.solve(prob, sd.GPUSimpleRK4(), dt=F(0.1))
+    assert r.t.dtype == np.float64
