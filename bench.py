@@ -1,29 +1,38 @@
 #!/usr/bin/env python3
 """Benchmark: adaptive ensemble throughput on one MI355X.
 
-Workload (one "step" = one ensemble solve): Lorenz, 1,048,576 trajectories,
-sigma ~ U(8,12), rho ~ U(0,21), beta ~ U(2,3), u0 ~ U(-1,1)^3 (Philox seed 1),
-tspan (0, 10), GPUSimpleATsit5 with abstol 1e-6, reltol 1e-3 and the reference
-default dt0 = 0.1f0, endpoint output only.  Inputs are resident in HBM before
-the timed region; each step runs the integrator kernel over the whole ensemble.
+Default line (one "step" = one ensemble solve): the learned-binning evaluation
+ensemble, Lorenz with 4,194,304 trajectories, sigma ~ U(8,12), rho ~ U(0,28),
+beta ~ U(2,3), u0 ~ U(-1,1)^3, t_end ~ LogUniform(0.5,10) (Philox seed 4),
+GPUSimpleATsit5 at abstol 1e-7, reltol 1e-5, dt0 = 0.1f0, endpoint output.
+Inputs are resident in HBM before the timed region.
 
-value = trajectories/s over the K timed steps (binary64 state, the headline);
-the binary32 run of the same ensemble is reported beside it.  Before the warmup,
-the library's autotuner picks each variant's launch shape (schedule, block, grid,
-queue chunk) by timing candidate launches of the same solve; the chosen shape is
-in config.launch.
+value = trajectories/s of the binary64 solve (the reference's precision) over the
+K timed steps, in the launch shape the library's autotuner picks (schedule,
+block, grid, queue chunk; config.launch).  Beside it:
+  * learned: the cost model (6-64-64-2 MLP) trained on the GPU from 2^20 plain
+    binary64 step logs (seed 40), then the binned binary32 Kahan-compensated
+    solve of the same ensemble, planning (features, forward pass, hand-out order)
+    included, against the plain binary64 one-launch kernel in index order, with an
+    ablation (untrained weights, no order, the model-free orders);
+  * error of every reported dtype against binary64 AVern9 at 1e-14 on the first
+    65,536 trajectories (median and 90th percentile of the normwise relative
+    endpoint error);
+  * lorenz_wide: the 1,048,576-trajectory Lorenz ensemble (rho ~ U(0,21), seed 1,
+    tspan (0,10), abstol 1e-6, reltol 1e-3), binary64 and binary32.
 
 roofline: the integrator kernel is bound by vector (VALU) binary64 throughput:
-achieved = algorithmic FLOPs per launch / average launch time (HIP events on
-the launch stream over the timed region), peak = 78.6 TFLOP/s.  Algorithmic
-FLOPs per launch = sum over trajectories of (accepted + rejected attempts) x
-(3 * 67 + 6 * 8 + 12) + 8 (the initial FSAL evaluation): 67 FLOPs of stage,
-error and scaling arithmetic per component per attempt, 6 Lorenz evaluations of
-8 FLOPs, 2 for the norm and 10 for the controller; FMA = 2, add/mul/div/sqrt/pow = 1.
+achieved = algorithmic FLOPs per solve / average device time per solve (HIP
+events on the launch stream over the timed region), peak = 78.6 TFLOP/s.
+Algorithmic FLOPs = sum over trajectories of (accepted + rejected attempts) x
+(3 * 67 + 6 * 8 + 12) + 8 per trajectory (the initial FSAL evaluation): 67 FLOPs
+of stage, error and scaling arithmetic per component per attempt, 6 Lorenz
+evaluations of 8 FLOPs, 2 for the norm and 10 for the controller; FMA = 2,
+add/mul/div/sqrt/pow = 1.
 
 cpu_baseline: the CPU restatement of the same solver (oracle/, one thread per
-host core, contiguous chunks as EnsembleThreads does) on a bounded sample of the
-same ensemble.
+host core of the GPU's share, contiguous chunks as EnsembleThreads does) on the
+first 65,536 trajectories of the same ensemble.
 """
 import argparse
 import json
@@ -35,7 +44,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "simplediffeq.jl_amd"))
 
-N_TRAJ = 1 << 20
+N_HEADLINE = 1 << 22  # the learned-binning evaluation ensemble
+N_WIDE = 1 << 20      # the wide Lorenz ensemble
+N_TRAIN = 1 << 20     # step logs the cost model is trained on
+N_ERR = 1 << 16       # trajectories checked against binary64 AVern9 at 1e-14
+N_TRAJ = N_WIDE       # default size of the single-workload runs
+# The host threads of the CPU baseline: the GPU's share of the host is 16 threads
+# (the CPU count the process sees may be larger), capped at the CPUs it may use.
+CPU_SHARE = 16
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 HBM_PEAK_GBS = 8000.0
 
@@ -97,22 +113,21 @@ def cpu_time(desc, n_state, u0, p, ps=None, tspan=None, threads=1, min_seconds=1
     return n * reps / el, el, reps
 
 
+def cpu_threads():
+    """(threads used, CPUs the process may use, CPUs of the host)."""
+    allowed = len(os.sched_getaffinity(0))
+    return min(CPU_SHARE, allowed), allowed, os.cpu_count()
+
+
 def cpu_leg(desc, n_state, u0, p, ps=None, tspan=None, min_seconds=3.0):
     """The `cpu_baseline` record of a workload (oracle on the host cores)."""
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads, allowed, nproc = cpu_threads()
     rate, el, reps = cpu_time(desc, n_state, u0, p, ps, tspan, threads, min_seconds)
     return {"value": rate, "unit": "trajectories/s", "cores": threads, "kind": "port",
-            "sample": "first %d trajectories, solved %d times in %.1f s on %s"
-                      % (int(desc.n_traj), reps, el, cpu_model())}
-
-
-def cpu_baseline(u0, p, desc_kwargs, sample, threads, min_seconds=10.0, max_reps=400):
-    """Oracle solves of the first `sample` trajectories, repeated until at least
-    `min_seconds` of CPU time have elapsed; returns (trajectories/s, seconds, reps)."""
-    from simplediffeq_hip import runtime
-    d = runtime.make_desc(n_traj=sample, **desc_kwargs)
-    return cpu_time(d, 3, u0[:, :sample], p[:, :sample], threads=threads,
-                    min_seconds=min_seconds, max_reps=max_reps)
+            "nproc": nproc, "affinity": allowed,
+            "sample": "first %d trajectories, solved %d times in %.1f s on %s, %d threads "
+                      "(the GPU's share of the host; %d CPUs allowed, %d in the host)"
+                      % (int(desc.n_traj), reps, el, cpu_model(), threads, allowed, nproc)}
 
 
 def _time_solves(ctx, d, bufs, stream, steps, warmup):
@@ -149,7 +164,7 @@ def workload_small(args):
         g = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
         ms, wall = _time_solves(ctx, d, g, stream, args.steps, args.warmup)
         c = runtime.EnsembleBuffers(d, 3, u0, p)
-        threads = min(16, len(os.sched_getaffinity(0)))
+        threads = cpu_threads()[0]
         t0 = time.perf_counter()
         for _ in range(5):
             pyoracle.solve(d, c, nthreads=threads)
@@ -310,151 +325,328 @@ def workload_everystep(args):
     return out
 
 
-def workload_costmodel(args):
-    """Learned cost binning + dt0.  Train the 6-64-64-2 MLP on the GPU from the
-    accepted-step counts of 2^20 plain binary64 ATsit5 solves (Lorenz, sigma~U(8,12),
-    rho~U(0,28), beta~U(2,3), u0~U(-1,1)^3, t_end~LogUniform(0.5,10); seed 40) at
-    abstol 1e-7, reltol 1e-5; evaluate on 2^22 fresh trajectories (seed 4):
-    binary32 + compensated update, persistent schedule, binned by predicted cost and
-    dt0-seeded, against the plain unsorted binary64 one-launch kernel.  Endpoint error
-    of both against binary64 AVern9 at 1e-14 on a 65,536-trajectory subset."""
+def _timed_steps(ctx, d, bufs, stream, steps, warmup):
+    """The driver's timed region: `warmup` untimed solves, then exactly `steps`
+    solves bracketed by synchronisation.  Returns (device ms per solve from HIP
+    events on the launch stream, wall seconds of the K solves)."""
+    import torch
+    for _ in range(warmup):
+        ctx.solve(d, bufs, stream=stream.cuda_stream, sync=False)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        ctx.solve(d, bufs, stream=stream.cuda_stream, sync=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall
+
+
+def _events(stream, fn, reps=5):
+    """Median device time (ms) of fn() on `stream` over `reps` runs after one warm-up."""
     import numpy as np
     import torch
-    from simplediffeq_hip import _abi, costmodel, inputs, runtime
-    n_train = min(args.n, 1 << 20) if args.n != N_TRAJ else 1 << 20
-    n_eval = 1 << 22 if args.n == N_TRAJ else args.n
-    ctx = runtime.Context(0)
-    stream = torch.cuda.Stream()
-    dev = "cuda:0"
-    out = {"workload": "costmodel", "train": n_train, "eval": n_eval}
-    tol = dict(abstol=1e-7, reltol=1e-5)
-    f32_01 = float(np.float32(0.1))
-
-    # 1. step logs of the plain solver
-    u0, p, ts = inputs.lorenz_costmodel(n_train, seed=40)
-    d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=n_train, dt=f32_01, **tol)
-    g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device=dev)
-    ctx.solve(d, g, sync=True)
-    x = costmodel.features(p, u0, ts)
-    y = costmodel.targets(g.numpy("naccept"), ts)
-    del g
-    # 2. train on the GPU (forward/backward/Adam kernels)
-    cm = costmodel.CostModel(ctx, seed=0)
-    t0 = time.perf_counter()
-    hist = cm.fit(x, y, steps=args.train_steps, batch=65536, lr=3e-3, log_every=200)
-    out["train_s"] = time.perf_counter() - t0
-    out["loss"] = [hist[0][1], hist[-1][1]]
-
-    # 3. evaluation ensemble, resident in HBM.  The binning step (features, MLP
-    # predictions, stable sort by decreasing predicted cost, seeded dt0) runs on the
-    # device on the bench stream; its time counts against the binned solves below.
-    # One library launch (sdeh_mlp_plan_lorenz) does all but the sort; the same step
-    # from torch-built features and the plain forward pass is timed beside it.
-    u0, p, ts = inputs.lorenz_costmodel(n_eval, seed=4)
-    u0d, pd, tsd = (torch.as_tensor(a, device=dev) for a in (u0, p, ts))
+    fn()
     torch.cuda.synchronize()
-
-    def plan_unfused():
-        with torch.cuda.stream(stream):
-            xd = costmodel.features_device(pd, u0d, tsd)
-        return cm.plan_device(xd, stream)
-
-    def plan():
-        return cm.plan_lorenz(pd, u0d, tsd, stream)
-
-    def time_plan(fn):
-        fn()
-        torch.cuda.synchronize()
+    out = []
+    for _ in range(reps):
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        for _ in range(5):
-            res = fn()
+        fn()
         e1.record(stream)
         torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / 5, res
+        out.append(e0.elapsed_time(e1))
+    return float(np.median(out))
 
-    plan_ms_unfused, (pred_u, order_u, dt0_u) = time_plan(plan_unfused)
-    plan_ms, (pred_f, order, dt0) = time_plan(plan)
-    out["plan_on_device_ms"] = plan_ms
-    out["plan_on_device_unfused_ms"] = plan_ms_unfused
-    out["fused_plan_equals_unfused"] = bool(
-        torch.equal(pred_f, pred_u) and torch.equal(order, order_u) and torch.equal(dt0, dt0_u))
-    # the same step with the predictions sorted on the host, for comparison
-    t0 = time.perf_counter()
-    pred_h = cm.predict(costmodel.features(p, u0, ts))
-    order_h = cm.order_by_cost(pred_h)
-    out["plan_on_host_s"] = time.perf_counter() - t0
-    out["device_and_host_orders_equal"] = bool(np.array_equal(order.cpu().numpy(), order_h))
 
-    def run(dtype, sched, precision, order_=None, dt0_=None, span=False):
-        # "unsorted" runs hand trajectories out in index order (span_order 1); with
-        # span=True the library's longest-span-first order applies (no model)
-        dd = runtime.make_desc(_abi.ALG_ATSIT5, dtype=dtype, rhs=_abi.RHS_LORENZ, n_traj=n_eval,
-                               dt=f32_01, schedule=sched, precision=precision,
-                               span_order=0 if span else 1, **tol)
-        gg = runtime.EnsembleBuffers(dd, 3, u0, p, tspan=ts, dt0=dt0_, order=order_,
-                                     backend="torch", device=dev)
-        ctx.solve(dd, gg, sync=True)
+class _Reference:
+    """Binary64 AVern9 at abstol = reltol = 1e-14 on the first `m` trajectories of
+    an ensemble (the oracle runs of the error figures), solved on the GPU."""
+
+    def __init__(self, ctx, u0, p, tspan=None, t=None, m=N_ERR):
+        import numpy as np
+        from simplediffeq_hip import _abi, runtime
+        m = min(m, u0.shape[1])
+        kw = dict(tspan=t) if t is not None else {}
+        d = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_LORENZ, n_traj=m,
+                              dt=float(np.float32(0.1)), abstol=1e-14, reltol=1e-14, **kw)
+        g = runtime.EnsembleBuffers(d, 3, u0[:, :m], p[:, :m],
+                                    tspan=None if tspan is None else tspan[:, :m],
+                                    backend="torch", device="cuda:0")
+        ctx.solve(d, g, sync=True)
+        self.m = m
+        self.u = g.numpy("u_end").astype(np.float64)
+        self.ok = g.numpy("retcode") == 0
+
+    def error(self, bufs):
+        """Median and 90th percentile of max_c |u - u_ref| / max_c |u_ref| over the
+        trajectories the reference finished."""
+        import numpy as np
+        ue = bufs.numpy("u_end")[:, :self.m].astype(np.float64)
+        e = np.max(np.abs(ue - self.u), axis=0) / np.max(np.abs(self.u), axis=0)
+        e = e[self.ok]
+        return {"median": float(np.median(e)), "p90": float(np.percentile(e, 90)),
+                "trajectories": int(self.ok.sum())}
+
+
+def _roofline(alg_flops, kernel_ms, peak, kernel, traffic=None, extra=None):
+    ach = alg_flops / (kernel_ms * 1e-3) / 1e12
+    r = {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+         "traffic": traffic, "kernel": kernel, "kernel_ms": kernel_ms,
+         "algorithmic_flops_per_launch": alg_flops}
+    if extra:
+        r.update(extra)
+    return r
+
+
+_KERNEL = "sdeh_adaptive_kernel<%s, Lorenz, ATsit5, endpoint%s>"
+_SCHED = {1: "one launch", 2: "persistent", 3: "staged: 32-attempt first pass, key sort, resumed "
+          "persistent pass"}
+
+
+def _kernel_name(dtype, sched, kahan=False):
+    return (_KERNEL % ("double" if dtype == "f64" else "float", ", Kahan" if kahan else "")
+            + " (" + _SCHED.get(sched, str(sched)) + ")")
+
+
+def lorenz_wide(args, ctx, stream, steps, warmup):
+    """The 1,048,576-trajectory Lorenz ensemble (seed 1, rho ~ U(0,21), tspan
+    (0,10), ATsit5 1e-6/1e-3, dt0 0.1f0, endpoint): binary64 and binary32, each
+    autotuned, K timed solves, error against AVern9 at 1e-14 on 65,536."""
+    import numpy as np
+    import torch
+    from simplediffeq_hip import _abi, inputs, runtime
+    n = N_WIDE if args.n in (N_TRAJ, N_HEADLINE) else args.n
+    u0, p = inputs.lorenz_wide(n, seed=1)
+    ref = _Reference(ctx, u0, p, t=(0.0, 10.0))
+    out = {"workload": "Lorenz ensemble, GPUSimpleATsit5, abstol 1e-6, reltol 1e-3, tspan "
+                       "(0,10), dt0 0.1f0, endpoint only",
+           "trajectories": n, "sigma": "U(8,12)", "rho": "U(0,21)", "beta": "U(2,3)",
+           "u0": "U(-1,1)^3", "seed": 1}
+    for name, dtype in (("f64", _abi.F64), ("f32", _abi.F32)):
+        d = runtime.make_desc(_abi.ALG_ATSIT5, dtype=dtype, rhs=_abi.RHS_LORENZ, n_traj=n,
+                              tspan=(0.0, 10.0), dt=float(np.float32(0.1)), abstol=1e-6,
+                              reltol=1e-3, schedule=args.schedule if name == "f64" else 0)
+        g = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
+        torch.cuda.synchronize()
+        if args.tune:
+            ctx.autotune(d, g)
+        ctx.solve(d, g, stream=stream.cuda_stream, sync=True)
         st = ctx.stats()
-        ms, _ = _time_solves(ctx, dd, gg, stream, args.steps, 1)
-        return gg, ms, st
+        if not (g.numpy("retcode") == 0).all():
+            raise RuntimeError("wide ensemble: trajectories did not finish")
+        ms, wall = _timed_steps(ctx, d, g, stream, steps, warmup)
+        flops = algorithmic_flops(_abi.ALG_ATSIT5, 3, LORENZ_FLOPS, st["attempts"], st["accepts"], n)
+        sched = st["schedule"]
+        out[name] = {"value": n * steps / wall, "unit": "trajectories/s", "kernel_ms": ms,
+                     "ms_per_step": wall * 1e3 / steps,
+                     "roofline": _roofline(flops, ms, PEAK_TFLOPS[name], _kernel_name(name, sched),
+                                           MEASURED_TRAFFIC.get((name, n, sched)),
+                                           {"attempts_per_launch": st["attempts"]}),
+                     "error_vs_avern9_1e-14": ref.error(g),
+                     "launch": {k: st[k] for k in ("block", "grid", "chunk", "schedule")}}
+        if name == "f32":
+            out[name]["roofline"]["frac_of_78.6_unpacked"] = out[name]["roofline"]["achieved"] / 78.6
+        del g
+    return out
 
-    base, ms_base, st_base = run(_abi.F64, _abi.SCHED_ONE_LAUNCH, _abi.PREC_PLAIN)
-    out["f64_one_launch_unsorted"] = {"kernel_ms": ms_base, "traj_per_s": n_eval / (ms_base * 1e-3),
-                                      "attempts": st_base["attempts"]}
+
+def headline(args):
+    """The default line: the learned-binning evaluation ensemble (see the module
+    docstring), binary64 `value`, the learned block, errors, the wide ensemble and
+    the CPU baseline."""
+    import numpy as np
+    import torch
+    from simplediffeq_hip import _abi, costmodel, inputs, runtime
+    n = N_HEADLINE if args.n == N_TRAJ else args.n
+    steps, warmup = args.steps, args.warmup
+    ctx = runtime.Context(0)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)  # default priority
+    f32_01 = float(np.float32(0.1))
+    tol = dict(abstol=1e-7, reltol=1e-5)
+
+    # 1. the cost model: step logs of the plain solver and training, on the GPU
+    cm, hist, t_train, t_logs = costmodel.train_lorenz(ctx, n=min(N_TRAIN, n), seed=40,
+                                                       steps=args.train_steps, **tol)
+    # 2. the evaluation ensemble, resident in HBM
+    u0, p, ts = inputs.lorenz_costmodel(n, seed=4)
+    u0d, pd, tsd = (torch.as_tensor(a, device=dev) for a in (u0, p, ts))
+    torch.cuda.synchronize()
+    ref = _Reference(ctx, u0, p, tspan=ts)
+
+    def bufs(dtype=_abi.F64, sched=0, precision=_abi.PREC_PLAIN, order=None, span=True):
+        d = runtime.make_desc(_abi.ALG_ATSIT5, dtype=dtype, rhs=_abi.RHS_LORENZ, n_traj=n,
+                              dt=f32_01, schedule=sched, precision=precision,
+                              span_order=0 if span else 1, **tol)
+        g = runtime.EnsembleBuffers(d, 3, u0d, pd, tspan=tsd, order=order, backend="torch",
+                                    device=dev)
+        return d, g
+
+    def solve_once(d, g):
+        ctx.solve(d, g, stream=stream.cuda_stream, sync=True)
+        st = ctx.stats()
+        if not (g.numpy("retcode") == 0).all():
+            raise RuntimeError("trajectories did not finish")
+        return st
+
+    def time_of(d, g, reps):
+        return _events(stream, lambda: ctx.solve(d, g, stream=stream.cuda_stream, sync=False),
+                       reps)
+
+    # 3. value: binary64 in the autotuned shape, the driver's timed region
+    d64, g64 = bufs(sched=args.schedule)
+    if args.tune:
+        ctx.autotune(d64, g64)
+    st64 = solve_once(d64, g64)
+    ms64, wall64 = _timed_steps(ctx, d64, g64, stream, steps, warmup)
+    flops64 = algorithmic_flops(_abi.ALG_ATSIT5, 3, LORENZ_FLOPS, st64["attempts"],
+                                st64["accepts"], n)
+    att = g64.numpy("naccept").astype(np.int64) + g64.numpy("nreject")
+    err64 = ref.error(g64)
+    sched64 = st64["schedule"]
+    launch64 = {k: st64[k] for k in ("block", "grid", "chunk", "schedule")}
+    del g64
+
+    # 4. binary32 (plain) in its autotuned shape, for its error and rate
+    d32, g32 = bufs(dtype=_abi.F32)
+    if args.tune:
+        ctx.autotune(d32, g32)
+    st32 = solve_once(d32, g32)
+    ms32 = time_of(d32, g32, 5)
+    f32 = {"kernel_ms": ms32, "value": n / (ms32 * 1e-3), "unit": "trajectories/s",
+           "error_vs_avern9_1e-14": ref.error(g32),
+           "roofline": _roofline(algorithmic_flops(_abi.ALG_ATSIT5, 3, LORENZ_FLOPS,
+                                                   st32["attempts"], st32["accepts"], n),
+                                 ms32, PEAK_TFLOPS["f32"], _kernel_name("f32", st32["schedule"])),
+           "launch": {k: st32[k] for k in ("block", "grid", "chunk", "schedule")}}
+    del g32
+
+    # 5. learned: planning on the device, binned binary32 Kahan solves, baselines
+    def plan(model):
+        return model.plan_lorenz(pd, u0d, tsd, stream, pred=False)
+
+    plan_ms = _events(stream, lambda: plan(cm))
+    _, order, _ = plan(cm)
+    untrained = costmodel.CostModel(ctx, seed=0)
+    untrained.y_mean, untrained.y_std = cm.y_mean, cm.y_std
+    plan_ms_untrained = _events(stream, lambda: plan(untrained))
+    _, order_untrained, _ = plan(untrained)
+    torch.cuda.synchronize()
+    reps = max(5, steps // 2)
+    runs = {}
+    for name, dtype, sched, prec, od, span in (
+            ("f64_one_launch_index_order", _abi.F64, 1, _abi.PREC_PLAIN, None, False),
+            ("f32k_persistent_binned", _abi.F32, 2, _abi.PREC_KAHAN, order, False),
+            ("f32k_one_launch_binned", _abi.F32, 1, _abi.PREC_KAHAN, order, False),
+            ("f32k_one_launch_binned_untrained", _abi.F32, 1, _abi.PREC_KAHAN, order_untrained,
+             False),
+            ("f32k_one_launch_index_order", _abi.F32, 1, _abi.PREC_KAHAN, None, False),
+            ("f32k_one_launch_span_order", _abi.F32, 1, _abi.PREC_KAHAN, None, True),
+            ("f32k_persistent_index_order", _abi.F32, 2, _abi.PREC_KAHAN, None, False),
+            ("f32k_persistent_span_order", _abi.F32, 2, _abi.PREC_KAHAN, None, True),
+            ("f32k_staged", _abi.F32, 3, _abi.PREC_KAHAN, None, True)):
+        d, g = bufs(dtype, sched, prec, od, span)
+        st = solve_once(d, g)
+        runs[name] = {"kernel_ms": time_of(d, g, reps), "attempts": st["attempts"],
+                      "schedule": st["schedule"]}
+        if name == "f32k_persistent_binned":
+            err32k = ref.error(g)
+            ach = algorithmic_flops(_abi.ALG_ATSIT5, 3, LORENZ_FLOPS, st["attempts"],
+                                    st["accepts"], n)
+            roof32k = _roofline(ach, runs[name]["kernel_ms"], PEAK_TFLOPS["f32"],
+                                _kernel_name("f32", 2, kahan=True))
+        del g
+    base_ms = runs["f64_one_launch_index_order"]["kernel_ms"]
+    binned_ms = runs["f32k_persistent_binned"]["kernel_ms"] + plan_ms
+    best_free = min(("f32k_one_launch_index_order", "f32k_one_launch_span_order",
+                     "f32k_persistent_index_order", "f32k_persistent_span_order", "f32k_staged"),
+                    key=lambda k: runs[k]["kernel_ms"])
+    best_binned = min(("f32k_persistent_binned", "f32k_one_launch_binned"),
+                      key=lambda k: runs[k]["kernel_ms"])
+    learned = {
+        "value": n / (binned_ms * 1e-3), "unit": "trajectories/s",
+        "dtype": "f32 state, Kahan-compensated update, f64 time",
+        "workload": "persistent, binned by the trained model; planning included",
+        "plan_ms": plan_ms, "solve_ms": runs["f32k_persistent_binned"]["kernel_ms"],
+        "speedup_vs_f64_one_launch_index_order": base_ms / binned_ms,
+        "target_3x_met": base_ms / binned_ms >= 3.0,
+        "error_vs_avern9_1e-14": err32k, "target_median_1e-5_met": err32k["median"] <= 1e-5,
+        "roofline": roof32k,
+        "solves_ms": {k: v["kernel_ms"] for k, v in runs.items()},
+        "gain": {
+            "wave_efficiency_one_launch": {
+                "trained": costmodel.wave_efficiency(att, order.cpu().numpy()),
+                "untrained": costmodel.wave_efficiency(att, order_untrained.cpu().numpy()),
+                "index_order": costmodel.wave_efficiency(att)},
+            "plan_ms_untrained": plan_ms_untrained,
+            "best_binned_with_plan_ms": runs[best_binned]["kernel_ms"] + plan_ms,
+            "best_binned": best_binned,
+            "best_model_free_ms": runs[best_free]["kernel_ms"], "best_model_free": best_free,
+            "net_gain_over_best_model_free_ms":
+                runs[best_free]["kernel_ms"] - (runs[best_binned]["kernel_ms"] + plan_ms)},
+        "train": {"step_logs": min(N_TRAIN, n), "step_log_solve_s": t_logs, "adam_steps":
+                  args.train_steps, "seconds": t_train, "loss": [hist[0][1], hist[-1][1]]},
+    }
+
+    out = {
+        "metric": "adaptive trajectories/s at stated tol vs FP64 oracle; % FP32/FP64 VALU peak",
+        "value": n * steps / wall64,
+        "unit": "trajectories/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": wall64 * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Philox seed 4; cost model trained on seed 40)",
+        "config": {"workload": "Lorenz ensemble, GPUSimpleATsit5, abstol 1e-7, reltol 1e-5, "
+                               "dt0 0.1f0, endpoint only, per-trajectory t_end (the learned-"
+                               "binning evaluation ensemble)",
+                   "trajectories": n, "sigma": "U(8,12)", "rho": "U(0,28)", "beta": "U(2,3)",
+                   "u0": "U(-1,1)^3", "t_end": "LogUniform(0.5,10)", "launch": launch64},
+        "roofline": _roofline(flops64, ms64, PEAK_TFLOPS["f64"], _kernel_name("f64", sched64),
+                              MEASURED_TRAFFIC.get(("f64", n, sched64, "costmodel")),
+                              {"attempts_per_launch": st64["attempts"],
+                               "algorithmic_bytes_gbs": n * (3 * 8 * 3 + 16 + 8 + 12) /
+                               (ms64 * 1e-3) / 1e9}),
+        "error_vs_avern9_1e-14": err64,
+        "f32": f32,
+        "learned": learned,
+    }
+    if not args.no_wide:
+        out["lorenz_wide"] = lorenz_wide(args, ctx, stream, steps, warmup)
     if not args.no_cpu:
-        k = min(n_eval, 1 << 16)
+        k = min(n, 1 << 16)
         dc = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=k, dt=f32_01, **tol)
-        out["cpu_baseline"] = cpu_leg(dc, 3, u0[:, :k], p[:, :k], tspan=ts[:, :k])
-    rec = {}
-    for name, dtype, sched, prec, od, dz, sp in (
-            ("f64_persistent_unsorted", _abi.F64, _abi.SCHED_PERSISTENT, _abi.PREC_PLAIN, None, None, False),
-            ("f64_one_launch_span_order", _abi.F64, _abi.SCHED_ONE_LAUNCH, _abi.PREC_PLAIN, None, None, True),
-            ("f32k_one_launch_span_order", _abi.F32, _abi.SCHED_ONE_LAUNCH, _abi.PREC_KAHAN, None, None, True),
-            ("f64_persistent_binned", _abi.F64, _abi.SCHED_PERSISTENT, _abi.PREC_PLAIN, order, None, False),
-            ("f32k_persistent_unsorted", _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN, None, None, False),
-            ("f32k_one_launch_binned", _abi.F32, _abi.SCHED_ONE_LAUNCH, _abi.PREC_KAHAN, order, None, False),
-            ("f32k_persistent_binned", _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN, order, None, False),
-            ("f32k_persistent_binned_dt0", _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN, order, dt0, False)):
-        gg, ms, st = run(dtype, sched, prec, od, dz, sp)
-        rec[name] = gg
-        out[name] = {"kernel_ms": ms, "traj_per_s": n_eval / (ms * 1e-3), "attempts": st["attempts"],
-                     "speedup_vs_f64_one_launch": ms_base / ms}
-        if od is not None:  # the binning step is part of the binned solve's cost
-            out[name]["with_plan_ms"] = ms + plan_ms
-            out[name]["speedup_with_plan"] = ms_base / (ms + plan_ms)
-    # binning leaves every trajectory's result unchanged
-    out["binned_bit_identical"] = bool(
-        np.array_equal(rec["f64_persistent_binned"].numpy("u_end"), base.numpy("u_end")) and
-        np.array_equal(rec["f32k_persistent_binned"].numpy("u_end"),
-                       rec["f32k_persistent_unsorted"].numpy("u_end")))
-    # 4. error against binary64 AVern9 at 1e-14 on a subset
-    ns = min(65536, n_eval)
-    dref = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_LORENZ, n_traj=ns, dt=f32_01,
-                             abstol=1e-14, reltol=1e-14)
-    gref = runtime.EnsembleBuffers(dref, 3, u0[:, :ns], p[:, :ns], tspan=ts[:, :ns],
-                                   backend="torch", device=dev)
-    ctx.solve(dref, gref, sync=True)
-    uref = gref.numpy("u_end").astype(np.float64)
-    ok = gref.numpy("retcode") == 0
+        out["cpu_baseline"] = cpu_leg(dc, 3, u0[:, :k], p[:, :k], tspan=ts[:, :k],
+                                      min_seconds=10.0)
+    ctx.close()
+    return out
 
-    def err(gg):
-        ue = gg.numpy("u_end")[:, :ns].astype(np.float64)
-        e = np.max(np.abs(ue - uref), axis=0) / np.max(np.abs(uref), axis=0)
-        e = e[ok]
-        return {"median": float(np.median(e)), "p90": float(np.percentile(e, 90))}
-    out["error_vs_avern9_1e-14"] = {"f64_one_launch_unsorted": err(base),
-                                    "f32k_persistent_binned": err(rec["f32k_persistent_binned"]),
-                                    "f32k_persistent_binned_dt0": err(rec["f32k_persistent_binned_dt0"]),
-                                    "reference_ok": int(ok.sum())}
-    a_base = np.asarray(base.numpy("naccept"), dtype=np.int64) + base.numpy("nreject")
-    out["wave_efficiency"] = {"unsorted": costmodel.wave_efficiency(a_base),
-                              "binned": costmodel.wave_efficiency(a_base, order.cpu().numpy())}
-    out["target_speedup_3x_met"] = out["f32k_persistent_binned_dt0"]["speedup_vs_f64_one_launch"] >= 3.0
-    out["target_speedup_3x_met_with_plan"] = (
-        max(v["speedup_with_plan"] for v in out.values()
-            if isinstance(v, dict) and "speedup_with_plan" in v) >= 3.0)
+
+def wide_line(args):
+    """--workload lorenz-wide: the wide ensemble alone, as its own JSON line."""
+    import torch
+    from simplediffeq_hip import runtime
+    ctx = runtime.Context(0)
+    stream = torch.cuda.Stream(device=torch.device("cuda:0"))
+    out = lorenz_wide(args, ctx, stream, args.steps, args.warmup)
+    if not args.no_cpu:
+        import numpy as np
+        from simplediffeq_hip import _abi, inputs
+        k = 1 << 16
+        u0, p = inputs.lorenz_wide(k, seed=1)
+        dc = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=k, tspan=(0.0, 10.0),
+                               dt=float(np.float32(0.1)), abstol=1e-6, reltol=1e-3)
+        out["cpu_baseline"] = cpu_leg(dc, 3, u0, p, min_seconds=10.0)
+    ctx.close()
     return out
 
 
@@ -463,10 +655,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=N_TRAJ)
+    ap.add_argument("--n", type=int, default=N_TRAJ,
+                    help="ensemble size (default: each workload's own)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-f32", action="store_true", help="skip the binary32 run")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 18)
+    ap.add_argument("--no-wide", action="store_true",
+                    help="default line: skip the wide-ensemble block")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for the HBM "
                          "byte counters)")
@@ -474,15 +667,13 @@ def main():
                     help="skip the autotuner (by default sdeh_autotune picks the launch shape "
                          "once per variant, before the warmup and the timed steps)")
     ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2, 3],
-                    help="headline schedule: 0 library choice (autotuned), 1 one launch, "
+                    help="binary64 schedule: 0 library choice (autotuned), 1 one launch, "
                          "2 persistent, 3 staged")
-    ap.add_argument("--grid", type=int, default=0,
-                    help="persistent schedule: workgroups of 256 (0 = library default, "
-                         "CUs x occupancy)")
-    ap.add_argument("--workload", default="lorenz-wide",
-                    choices=["lorenz-wide", "lorenz-small", "lorenz-uneven", "glv16", "costmodel",
-                             "everystep"],
-                    help="lorenz-wide is the headline line; the others report their own JSON")
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "costmodel", "lorenz-wide", "lorenz-small",
+                             "lorenz-uneven", "glv16", "everystep"],
+                    help="headline (= costmodel) is the driver's line; the others report their "
+                         "own JSON")
     ap.add_argument("--train-steps", type=int, default=1500)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -490,138 +681,17 @@ def main():
               "(trajectories are independent: an N-GPU run gives each GPU a contiguous slice)"
               % args.gpus, file=sys.stderr)
         return 2
-
-    extra = {"lorenz-small": workload_small, "lorenz-uneven": workload_uneven,
-             "glv16": workload_glv, "costmodel": workload_costmodel,
-             "everystep": workload_everystep}
-    if args.workload in extra:
-        print(json.dumps(extra[args.workload](args)))
-        return 0
-
-    import numpy as np
-    import torch
-    from simplediffeq_hip import _abi, inputs, runtime
-
     if args.copy_probe:
+        import torch
         src = torch.ones(32 << 20, dtype=torch.float64, device="cuda:0")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-
-    n = args.n
-    u0, p = inputs.lorenz_wide(n, seed=1)
-    dt0 = float(np.float32(0.1))
-    common = dict(alg=_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, tspan=(0.0, 10.0), dt=dt0,
-                  abstol=1e-6, reltol=1e-3, save_everystep=False)
-    dev = torch.device("cuda:0")
-    ctx = runtime.Context(0)
-    stream = torch.cuda.Stream(device=dev)  # default priority
-    sh = stream.cuda_stream
-
-    results = {}
-    for dtype_name, dtype in (("f64", _abi.F64), ("f32", _abi.F32)):
-        if dtype_name == "f32" and args.no_f32:
-            continue
-        d = runtime.make_desc(dtype=dtype, n_traj=n, grid=args.grid, schedule=args.schedule,
-                              **common)
-        bufs = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device=dev)
-        torch.cuda.synchronize()
-        if args.tune:
-            ctx.autotune(d, bufs)
-        ctx.solve(d, bufs, stream=sh, sync=True)  # counters of this (deterministic) workload
-        st = ctx.stats()
-        rc = bufs.numpy("retcode")
-        if not (rc == 0).all():
-            raise RuntimeError("%d trajectories did not finish" % int((rc != 0).sum()))
-        for _ in range(args.warmup):
-            ctx.solve(d, bufs, stream=sh, sync=False)
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        e0.record(stream)
-        for _ in range(args.steps):
-            ctx.solve(d, bufs, stream=sh, sync=False)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        dev_ms = e0.elapsed_time(e1) / args.steps
-        flops = algorithmic_flops(_abi.ALG_ATSIT5, 3, LORENZ_FLOPS, st["attempts"], st["accepts"], n)
-        es = 8 if dtype == _abi.F64 else 4
-        alg_bytes = n * (3 * es + 3 * es + 3 * es + 8 + 12)  # u0, p, u_end, t_end, 3 int32
-        host_ms = None
-        if dtype == _abi.F64:
-            # the same solve from host (pageable numpy) buffers: the library stages
-            # inputs to HBM and outputs back, so this rate includes both PCIe copies
-            hb = runtime.EnsembleBuffers(d, 3, u0, p)
-            ctx.solve(d, hb)
-            t1 = time.perf_counter()
-            reps = 3
-            for _ in range(reps):
-                ctx.solve(d, hb)
-            host_ms = (time.perf_counter() - t1) * 1e3 / reps
-            del hb
-        results[dtype_name] = dict(
-            wall_s=wall, ms_per_step=wall * 1e3 / args.steps, kernel_ms=dev_ms,
-            traj_per_s=n * args.steps / wall, attempts=st["attempts"], accepts=st["accepts"],
-            flops=flops, tflops=flops / (dev_ms * 1e-3) / 1e12, bytes=alg_bytes,
-            gbs=alg_bytes / (dev_ms * 1e-3) / 1e9, host_ms=host_ms,
-            shape=dict(block=st["block"], grid=st["grid"],
-                                                             chunk=st["chunk"],
-                                                             schedule=st["schedule"]))
-        del bufs
-
-    r64 = results["f64"]
-    out = {
-        "metric": "adaptive trajectories/s at stated tol vs FP64 oracle; % FP32/FP64 VALU peak",
-        "value": r64["traj_per_s"],
-        "unit": "trajectories/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r64["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (Philox seed 1)",
-        "config": {"workload": "Lorenz ensemble, GPUSimpleATsit5, abstol 1e-6, reltol 1e-3, "
-                               "tspan (0,10), dt0 0.1f0, endpoint only",
-                   "trajectories": n, "sigma": "U(8,12)", "rho": "U(0,21)", "beta": "U(2,3)",
-                   "u0": "U(-1,1)^3", "launch": r64["shape"]},
-        "roofline": {"bound": "valu", "achieved": r64["tflops"], "peak": PEAK_TFLOPS["f64"],
-                     "unit": "TFLOP/s", "frac": r64["tflops"] / PEAK_TFLOPS["f64"],
-                     "traffic": MEASURED_TRAFFIC.get(("f64", n, r64["shape"]["schedule"])),
-                     "kernel": "sdeh_adaptive_kernel<double, Lorenz, ATsit5, endpoint>" + (
-                         " (staged: first pass, key sort, resumed pass)"
-                         if r64["shape"]["schedule"] == _abi.SCHED_STAGED else ""),
-                     "kernel_ms": r64["kernel_ms"], "algorithmic_flops_per_launch": r64["flops"],
-                     "attempts_per_launch": r64["attempts"],
-                     "hbm_gbs": r64["gbs"], "hbm_frac": r64["gbs"] / HBM_PEAK_GBS},
-        # not `value`: the rate when the caller hands over host buffers (PCIe both ways)
-        "host_buffers": {"ms_per_solve": r64["host_ms"],
-                         "traj_per_s": n / (r64["host_ms"] * 1e-3)},
-    }
-    if "f32" in results:
-        r32 = results["f32"]
-        out["f32"] = {"value": r32["traj_per_s"], "unit": "trajectories/s",
-                      "kernel_ms": r32["kernel_ms"], "achieved_tflops": r32["tflops"],
-                      "frac_of_157.3": r32["tflops"] / 157.3,
-                      "frac_of_78.6_unpacked": r32["tflops"] / 78.6,
-                      "attempts_per_launch": r32["attempts"], "launch": r32["shape"]}
-    if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))
-        sample = min(args.cpu_sample, n)
-        kw = dict(common)
-        kw["dtype"] = _abi.F64
-        rate, el, reps = cpu_baseline(u0, p, kw, sample, threads)
-        out["cpu_baseline"] = {"value": rate, "unit": "trajectories/s", "cores": threads,
-                               "kind": "port",
-                               "sample": "first %d trajectories of the same ensemble, binary64, "
-                                         "solved %d times in %.1f s on %s"
-                                         % (sample, reps, el, cpu_model())}
-    print(json.dumps(out))
+    fn = {"headline": headline, "costmodel": headline, "lorenz-wide": wide_line,
+          "lorenz-small": workload_small, "lorenz-uneven": workload_uneven, "glv16": workload_glv,
+          "everystep": workload_everystep}[args.workload]
+    print(json.dumps(fn(args)))
     return 0
 
 
