@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define SDEH_ABI_VERSION 3
+#define SDEH_ABI_VERSION 4
 
 typedef enum sdeh_status {
   SDEH_OK = 0,
@@ -183,7 +183,12 @@ typedef struct sdeh_solve_desc {
                                hand trajectories to lanes longest span first (fewer
                                long trajectories start late; results are unchanged);
                                1: index order */
-  int32_t reserved;
+  int32_t waves;            /* register cap: 0 = tuned (else the kernel's default);
+                               4, 6 or 8 = the variant compiled for that many waves per
+                               SIMD (at most 512 / waves VGPRs per lane), built for the
+                               Lorenz right-hand side's adaptive ATsit5 and AVern7
+                               endpoint kernels (SDEH_E_UNSUPPORTED elsewhere).  Same
+                               arithmetic, same results.  (ABI 4; was reserved, 0.) */
 } sdeh_solve_desc;
 
 /* Solve buffers.  With memory == SDEH_MEM_HOST these are host pointers and the
@@ -223,7 +228,7 @@ typedef struct sdeh_stats {
   int64_t accepts;
   int32_t block, grid, chunk, schedule; /* launch shape actually used */
   int32_t lanes;           /* lanes per trajectory used */
-  int32_t reserved0;
+  int32_t waves;           /* register-cap variant used (0 = the default kernel) */
   int64_t steps_needed;    /* SDEH_SAVE_PACKED: total points offered by all
                               trajectories (the steps_capacity that holds them all) */
 } sdeh_stats;
@@ -287,8 +292,9 @@ int sdeh_graph_launch(sdeh_ctx* ctx, int64_t graph_handle, void* stream);
 int sdeh_graph_destroy(sdeh_ctx* ctx, int64_t graph_handle);
 
 /* Launch-shape autotuner: time candidate (block, grid, chunk, schedule) shapes
- * for the given problem on device buffers and store the best one for
- * (alg, rhs, dtype, save mode).  Later solves with tuned fields left 0 use it. */
+ * for the given problem on device buffers, then the register-cap variants (waves)
+ * of the best one where they are built, and store the best for (alg, rhs, dtype,
+ * save mode, precision, lanes).  Later solves with tuned fields left 0 use it. */
 int sdeh_autotune(sdeh_ctx* ctx, const sdeh_solve_desc* d, const sdeh_io* io,
                   sdeh_stats* best);
 

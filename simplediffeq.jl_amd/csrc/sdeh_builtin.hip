@@ -20,9 +20,28 @@ const void* fixed_ptr() {
   return reinterpret_cast<const void*>(&sdeh_fixed_kernel<T, R, ALG, SAVE>);
 }
 
-template <class T, int ALG, int SAVE, bool KAHAN>
+template <class T, int ALG, int SAVE, bool KAHAN, int WAVES = 0>
 const void* adaptive_ptr() {
-  return reinterpret_cast<const void*>(&sdeh_adaptive_kernel<T, R, ALG, SAVE, KAHAN>);
+  return reinterpret_cast<const void*>(&sdeh_adaptive_kernel<T, R, ALG, SAVE, KAHAN, WAVES>);
+}
+
+// Register-cap variants (SDEH_WAVE_VARIANTS, built for the Lorenz right-hand side):
+// the adaptive endpoint kernels of ATsit5 and AVern7 at 4, 6 and 8 waves per SIMD,
+// for the autotuner; waves = 0 is the default kernel.
+
+template <class T, int ALG, bool KAHAN>
+const void* waves_variant(int waves) {
+#if defined(SDEH_WAVE_VARIANTS)
+  if constexpr (ALG == sdeh::ALG_ATSIT5 || ALG == sdeh::ALG_AVERN7) {
+    switch (waves) {
+      case 4: return adaptive_ptr<T, ALG, sdeh::SAVE_END, KAHAN, 4>();
+      case 6: return adaptive_ptr<T, ALG, sdeh::SAVE_END, KAHAN, 6>();
+      case 8: return adaptive_ptr<T, ALG, sdeh::SAVE_END, KAHAN, 8>();
+    }
+  }
+#endif
+  (void)waves;
+  return nullptr;
 }
 
 // Which variants this translation unit instantiates.  The 16-species GLV
@@ -56,7 +75,7 @@ const void* adaptive_or_null() {
 }
 
 template <class T, int ALG>
-const void* by_save(int save, int kahan) {
+const void* by_save(int save, int kahan, int waves) {
   constexpr bool fixed = (ALG == sdeh::ALG_TSIT5 || ALG == sdeh::ALG_VERN7 ||
                           ALG == sdeh::ALG_VERN9 || ALG == sdeh::ALG_RK4 ||
                           ALG == sdeh::ALG_EULER);
@@ -73,6 +92,14 @@ const void* by_save(int save, int kahan) {
     }
     return nullptr;
   } else {
+    if (waves) {
+      if (save != sdeh::SAVE_END) return nullptr;
+      if (kahan) {
+        if constexpr (sizeof(T) == 4) return waves_variant<T, ALG, true>(waves);
+        return nullptr;
+      }
+      return waves_variant<T, ALG, false>(waves);
+    }
     if (kahan) {
       // The compensated update is a binary32 precision mode.
       if constexpr (sizeof(T) == 4) {
@@ -90,26 +117,28 @@ const void* by_save(int save, int kahan) {
 }
 
 template <class T>
-const void* by_alg(int alg, int save, int kahan) {
+const void* by_alg(int alg, int save, int kahan, int waves) {
   switch (alg) {
-    case sdeh::ALG_TSIT5: return by_save<T, sdeh::ALG_TSIT5>(save, kahan);
-    case sdeh::ALG_ATSIT5: return by_save<T, sdeh::ALG_ATSIT5>(save, kahan);
-    case sdeh::ALG_VERN7: return by_save<T, sdeh::ALG_VERN7>(save, kahan);
-    case sdeh::ALG_AVERN7: return by_save<T, sdeh::ALG_AVERN7>(save, kahan);
-    case sdeh::ALG_VERN9: return by_save<T, sdeh::ALG_VERN9>(save, kahan);
-    case sdeh::ALG_AVERN9: return by_save<T, sdeh::ALG_AVERN9>(save, kahan);
-    case sdeh::ALG_RK4: return by_save<T, sdeh::ALG_RK4>(save, kahan);
-    case sdeh::ALG_EULER: return by_save<T, sdeh::ALG_EULER>(save, kahan);
+    case sdeh::ALG_TSIT5: return by_save<T, sdeh::ALG_TSIT5>(save, kahan, waves);
+    case sdeh::ALG_ATSIT5: return by_save<T, sdeh::ALG_ATSIT5>(save, kahan, waves);
+    case sdeh::ALG_VERN7: return by_save<T, sdeh::ALG_VERN7>(save, kahan, waves);
+    case sdeh::ALG_AVERN7: return by_save<T, sdeh::ALG_AVERN7>(save, kahan, waves);
+    case sdeh::ALG_VERN9: return by_save<T, sdeh::ALG_VERN9>(save, kahan, waves);
+    case sdeh::ALG_AVERN9: return by_save<T, sdeh::ALG_AVERN9>(save, kahan, waves);
+    case sdeh::ALG_RK4: return by_save<T, sdeh::ALG_RK4>(save, kahan, waves);
+    case sdeh::ALG_EULER: return by_save<T, sdeh::ALG_EULER>(save, kahan, waves);
   }
   return nullptr;
 }
 
 }  // namespace
 
-// dtype: 0 binary64, 1 binary32.  Returns the kernel's host stub or null.
+// dtype: 0 binary64, 1 binary32; waves: 0 for the default kernel, else a
+// register-cap variant (null where not built).  Returns the kernel's host stub or null.
 extern "C" __attribute__((visibility("hidden"))) const void* SDEH_AOT_FN(int dtype, int alg,
-                                                                         int save, int kahan) {
-  if (dtype == 0) return by_alg<double>(alg, save, kahan);
-  if (dtype == 1) return by_alg<float>(alg, save, kahan);
+                                                                         int save, int kahan,
+                                                                         int waves) {
+  if (dtype == 0) return by_alg<double>(alg, save, kahan, waves);
+  if (dtype == 1) return by_alg<float>(alg, save, kahan, waves);
   return nullptr;
 }

@@ -1718,10 +1718,14 @@ struct WavesHint {
 }  // namespace sdeh
 
 // Kernel entry templates.  Instantiated with extern "C" names by the
-// ahead-of-time translation unit and by the hipRTC wrapper.
-template <class T, class R, int ALG, int SAVE, bool KAHAN>
+// ahead-of-time translation unit and by the hipRTC wrapper.  WAVES > 0 asks the
+// compiler for that many waves per SIMD instead of the WavesHint default (at most
+// 512 / WAVES VGPRs per lane, in steps of 8): the autotuner's register-cap
+// variants of the same arithmetic.
+template <class T, class R, int ALG, int SAVE, bool KAHAN, int WAVES = 0>
 __global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(sdeh::WavesHint<T, R, ALG, SAVE>::value)))
+    __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES
+                                                 : sdeh::WavesHint<T, R, ALG, SAVE>::value)))
     sdeh_adaptive_kernel(sdeh_kargs a) {
   sdeh::adaptive_body<T, R, ALG, SAVE, KAHAN>(a);
 }

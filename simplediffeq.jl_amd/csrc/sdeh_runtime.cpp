@@ -31,13 +31,13 @@
 #define SDEH_HIDDEN __attribute__((visibility("hidden")))
 
 extern "C" {
-SDEH_HIDDEN const void* sdeh_aot_Lorenz(int, int, int, int);
-SDEH_HIDDEN const void* sdeh_aot_GLV16(int, int, int, int);
-SDEH_HIDDEN const void* sdeh_aot_GLV16G(int, int, int, int);
-SDEH_HIDDEN const void* sdeh_aot_GLV16M(int, int, int, int);
-SDEH_HIDDEN const void* sdeh_aot_ExpDecay3(int, int, int, int);
-SDEH_HIDDEN const void* sdeh_aot_Affine1(int, int, int, int);
-SDEH_HIDDEN const void* sdeh_aot_Linear2(int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_Lorenz(int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_GLV16(int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_GLV16G(int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_GLV16M(int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_ExpDecay3(int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_Affine1(int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_Linear2(int, int, int, int, int);
 SDEH_HIDDEN hipError_t sdeh_util_fill_f64(double*, double, int64_t, hipStream_t);
 SDEH_HIDDEN hipError_t sdeh_util_fill_f32(float*, float, int64_t, hipStream_t);
 SDEH_HIDDEN hipError_t sdeh_util_zero_u64(unsigned long long*, int, hipStream_t);
@@ -85,6 +85,8 @@ constexpr int64_t kPipeMin = 1 << 17;
 // (0.95 after 16, 0.74 after 8).
 constexpr int32_t kStageAttempts = 32;
 constexpr int kPipeChunks = 4;
+// Register-cap variants (waves per SIMD) the autotuner tries after the launch shape.
+constexpr int kWaveVariants[] = {4, 6, 8};
 static_assert(8 + kPipeChunks <= 16, "chunk queues live in counters[8..16)");
 
 struct Builtin {
@@ -92,13 +94,13 @@ struct Builtin {
   const char* name;
   int32_t n, p;
   int64_t s;
-  const void* (*lookup)(int, int, int, int);
+  const void* (*lookup)(int, int, int, int, int);
   // Kernels of the group layout (`group` lanes per trajectory), if built.
   int32_t group;
-  const void* (*group_lookup)(int, int, int, int);
+  const void* (*group_lookup)(int, int, int, int, int);
   // Kernels of the matrix-core wave layout (`mgroup` lanes per trajectory), if built.
   int32_t mgroup;
-  const void* (*mgroup_lookup)(int, int, int, int);
+  const void* (*mgroup_lookup)(int, int, int, int, int);
 };
 
 const Builtin kBuiltins[] = {
@@ -126,10 +128,11 @@ struct KernelRef {
   const void* aot = nullptr;
   hipFunction_t fn = nullptr;
   int32_t lanes = 1;  // lanes per trajectory
+  int32_t waves = 0;  // register-cap variant (0 = the default kernel)
 };
 
 struct Shape {
-  int32_t schedule = SDEH_SCHED_ONE_LAUNCH, block = 256, grid = 0, chunk = 64;
+  int32_t schedule = SDEH_SCHED_ONE_LAUNCH, block = 256, grid = 0, chunk = 64, waves = 0;
 };
 
 using TuneKey = std::tuple<int64_t, int, int, int, int, int>;  // rhs, alg, dtype, save, kahan, lanes
@@ -232,13 +235,17 @@ const std::vector<std::string>& rtc_options() {
 
 // Compile (or fetch) the integrator kernel for a registered RHS.
 int user_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kahan,
-                KernelRef* out) {
+                KernelRef* out, int waves = 0) {
   const UserRhs& u = c->user[rhs - kUserBase];
+  if (waves && (!is_adaptive(alg) || save != SAVE_END))
+    return fail(c, SDEH_E_UNSUPPORTED, "register-cap variants are adaptive endpoint kernels");
   char key[128];
-  snprintf(key, sizeof key, "%lld/%d/%d/%d/%d", (long long)rhs, alg, dtype, save, kahan);
+  snprintf(key, sizeof key, "%lld/%d/%d/%d/%d/%d", (long long)rhs, alg, dtype, save, kahan,
+           waves);
   auto it = c->rtc.find(key);
   if (it != c->rtc.end()) {
     out->fn = it->second.second;
+    out->waves = waves;
     return SDEH_OK;
   }
   const char* T = dtype == SDEH_F64 ? "double" : "float";
@@ -256,7 +263,13 @@ int user_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kaha
            "};\n",
            u.n, u.p, (long long)u.s, u.fn.c_str());
   src += wrap;
-  if (is_adaptive(alg))
+  if (is_adaptive(alg) && waves)
+    snprintf(wrap, sizeof wrap,
+             "extern \"C\" __global__ void __launch_bounds__(256) "
+             "__attribute__((amdgpu_waves_per_eu(%d))) sdeh_user_kernel(sdeh_kargs a) "
+             "{ sdeh::adaptive_body<%s, sdeh_user_rhs, %d, %d, %s>(a); }\n",
+             waves, T, alg, save, kahan ? "true" : "false");
+  else if (is_adaptive(alg))
     snprintf(wrap, sizeof wrap,
              "extern \"C\" __global__ void __launch_bounds__(256) "
              "__attribute__((amdgpu_waves_per_eu(sdeh::WavesHint<%s, sdeh_user_rhs, %d, %d>::value))) "
@@ -329,6 +342,7 @@ int user_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kaha
   HIPCHK(c, hipModuleGetFunction(&fn, mod, "sdeh_user_kernel"));
   c->rtc[key] = {mod, fn};
   out->fn = fn;
+  out->waves = waves;
   return SDEH_OK;
 }
 
@@ -342,8 +356,21 @@ int user_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kaha
 // faster there than in the group layout, which repeats the controller on all 16
 // lanes.)
 int resolve_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kahan,
-                   int lanes, KernelRef* out) {
+                   int lanes, KernelRef* out, int waves = 0) {
+  out->waves = 0;
   if (const Builtin* b = find_builtin(rhs)) {
+    if (waves) {  // register-cap variants: one lane per trajectory only
+      if (lanes > 1 || (lanes == 0 && dtype == SDEH_F64 && (b->mgroup > 1 || b->group > 1)))
+        return fail(c, SDEH_E_UNSUPPORTED, "register-cap variants run one lane per trajectory");
+      out->lanes = 1;
+      out->aot = b->lookup(dtype, alg, save, kahan, waves);
+      if (!out->aot)
+        return fail(c, SDEH_E_UNSUPPORTED,
+                    "no register-cap variant (%d waves) for rhs %s, alg %d, dtype %d, save "
+                    "mode %d, precision %d", waves, b->name, alg, dtype, save, kahan);
+      out->waves = waves;
+      return SDEH_OK;
+    }
     if (lanes != 0 && lanes != 1 && lanes != b->group && !(b->mgroup > 1 && lanes == b->mgroup)) {
       if (b->mgroup > 1)
         return fail(c, SDEH_E_INVALID,
@@ -356,7 +383,7 @@ int resolve_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int k
                   lanes);
     }
     if (b->mgroup > 1 && (lanes == b->mgroup || (lanes == 0 && dtype == SDEH_F64))) {
-      out->aot = b->mgroup_lookup(dtype, alg, save, kahan);
+      out->aot = b->mgroup_lookup(dtype, alg, save, kahan, 0);
       out->lanes = b->mgroup;
       if (out->aot) return SDEH_OK;
       if (lanes == b->mgroup)
@@ -366,7 +393,7 @@ int resolve_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int k
     }
     if ((lanes == b->group || (lanes == 0 && dtype == SDEH_F64)) && b->group > 1 &&
         b->group_lookup) {
-      out->aot = b->group_lookup(dtype, alg, save, kahan);
+      out->aot = b->group_lookup(dtype, alg, save, kahan, 0);
       out->lanes = b->group;
       if (out->aot || lanes == b->group) {
         if (!out->aot)
@@ -377,7 +404,7 @@ int resolve_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int k
       }
     }
     out->lanes = 1;
-    out->aot = b->lookup(dtype, alg, save, kahan);
+    out->aot = b->lookup(dtype, alg, save, kahan, 0);
     if (!out->aot)
       return fail(c, SDEH_E_UNSUPPORTED,
                   "no built-in kernel for rhs %s, alg %d, dtype %d, save mode %d, precision %d",
@@ -387,7 +414,7 @@ int resolve_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int k
   if (lanes > 1)
     return fail(c, SDEH_E_INVALID, "registered right-hand sides run one lane per trajectory");
   out->lanes = 1;
-  return user_kernel(c, rhs, alg, dtype, save, kahan, out);
+  return user_kernel(c, rhs, alg, dtype, save, kahan, out, waves);
 }
 
 hipError_t launch(const KernelRef& k, int grid, int block, sdeh_kargs* a, hipStream_t s) {
@@ -596,16 +623,35 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
 
   // launch shape
   Shape sh;
+  if (d->waves != 0 && d->waves != 4 && d->waves != 6 && d->waves != 8)
+    return fail(c, SDEH_E_INVALID, "waves must be 0, 4, 6 or 8");
+  if (d->waves && !adaptive)
+    return fail(c, SDEH_E_UNSUPPORTED, "register-cap variants are adaptive endpoint kernels");
   if (adaptive) {
     auto it = c->tuned.find(TuneKey(d->rhs, d->alg, d->dtype, pl->save, pl->kahan, pl->k.lanes));
     if (it != c->tuned.end()) sh = it->second;
     else sh.schedule = SDEH_SCHED_PERSISTENT;
+    // the register-cap variant: the caller's, else the tuned one (when it exists
+    // for this solve), else the default kernel
+    const int waves = d->waves ? d->waves : sh.waves;
+    sh.waves = 0;
+    if (waves) {
+      KernelRef kv;
+      r = resolve_kernel(c, d->rhs, d->alg, d->dtype, pl->save, pl->kahan, pl->k.lanes, &kv,
+                         waves);
+      if (r && d->waves) return r;
+      if (!r) {
+        pl->k = kv;
+        sh.waves = waves;
+      }
+    }
     if (d->schedule != SDEH_SCHED_AUTO) sh.schedule = d->schedule;
     if (d->block) sh.block = d->block;
     if (d->chunk) sh.chunk = d->chunk;
     if (sh.schedule == SDEH_SCHED_PERSISTENT || sh.schedule == SDEH_SCHED_STAGED) {
       if (d->grid) sh.grid = d->grid;
-      else if (it == c->tuned.end() || d->block || it->second.schedule != sh.schedule)
+      else if (it == c->tuned.end() || d->block || it->second.schedule != sh.schedule ||
+               it->second.waves != sh.waves)
         sh.grid = c->cus * blocks_per_cu(c, pl->k, sh.block);
       if (sh.grid < 1) sh.grid = 1;
     }
@@ -1081,6 +1127,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   c->stats.chunk = pl.shape.chunk;
   c->stats.schedule = pl.shape.schedule;
   c->stats.lanes = pl.k.lanes;
+  c->stats.waves = pl.k.waves;
   if (do_sync) {
     if (piped) HIPCHK(c, hipStreamSynchronize(c->s_out));
     HIPCHK(c, hipStreamSynchronize(s));
@@ -1193,6 +1240,7 @@ int solve_static_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, 
   c->stats.chunk = pl.shape.chunk;
   c->stats.schedule = pl.shape.schedule;
   c->stats.lanes = pl.k.lanes;
+  c->stats.waves = pl.k.waves;
   if (!sync) return SDEH_OK;
   unsigned long long cnt[4] = {0, 0, 0, 0};
   if (n > 0) {
@@ -1518,6 +1566,14 @@ int sdeh_autotune(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, sdeh
   double best_ms = 1e300;
   Shape best_shape;
   sdeh_stats best_stats{};
+  TuneKey key;
+  {  // the shape search runs the caller's kernel choice, not an earlier tuning's
+    Plan pl;
+    int r = plan_solve(c, d, io, &pl);
+    if (r) return r;
+    key = TuneKey(d->rhs, d->alg, d->dtype, pl.save, pl.kahan, pl.k.lanes);
+    c->tuned.erase(key);
+  }
   for (const Cand& k : cands) {
     sdeh_solve_desc dd = *d;
     dd.schedule = k.sched;
@@ -1543,13 +1599,42 @@ int sdeh_autotune(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, sdeh
       best_shape.block = k.block;
       best_shape.chunk = k.chunk ? k.chunk : 64;
       best_shape.grid = c->stats.grid;
+      best_shape.waves = c->stats.waves;
       best_stats = c->stats;
     }
   }
-  Plan pl;
-  int r = plan_solve(c, d, io, &pl);
-  if (r) return r;
-  c->tuned[TuneKey(d->rhs, d->alg, d->dtype, pl.save, pl.kahan, pl.k.lanes)] = best_shape;
+  // Then the register cap: the best shape's schedule, block and chunk with each
+  // register-cap variant built for this solve (its own resident grid).
+  if (!d->waves) {
+    const Shape base = best_shape;
+    for (int w : kWaveVariants) {
+      sdeh_solve_desc dd = *d;
+      dd.schedule = base.schedule;
+      dd.block = base.block;
+      dd.chunk = base.chunk;
+      dd.grid = 0;
+      dd.waves = w;
+      Plan pl;
+      int r = plan_solve(c, &dd, io, &pl);
+      if (r == SDEH_E_UNSUPPORTED) continue;  // no such variant for this solve
+      if (r) return r;
+      double ms = 1e300;
+      for (int rep = 0; rep < 2; ++rep) {
+        r = sdeh_solve(c, &dd, io, SDEH_MEM_DEVICE, nullptr, 1);
+        if (r && r != SDEH_E_DTMIN) return r;
+        ms = c->stats.kernel_ms;
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_shape = base;
+        best_shape.grid = c->stats.grid;
+        best_shape.waves = w;
+        best_stats = c->stats;
+      }
+    }
+  }
+  c->err.clear();
+  c->tuned[key] = best_shape;
   if (best) *best = best_stats;
   return SDEH_OK;
 }

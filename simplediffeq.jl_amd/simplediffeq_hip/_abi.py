@@ -6,7 +6,7 @@ Only plain ctypes: the structs below must match the C header field for field.
 import ctypes
 import os
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # sdeh_status
 OK = 0
@@ -77,7 +77,7 @@ class SolveDesc(ctypes.Structure):
         ("lanes", ctypes.c_int32),
         ("save_layout", ctypes.c_int32),
         ("span_order", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("waves", ctypes.c_int32),
     ]
 
 
@@ -115,7 +115,7 @@ class Stats(ctypes.Structure):
         ("chunk", ctypes.c_int32),
         ("schedule", ctypes.c_int32),
         ("lanes", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("waves", ctypes.c_int32),
         ("steps_needed", ctypes.c_int64),
     ]
 

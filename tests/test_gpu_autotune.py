@@ -1,0 +1,94 @@
+"""The autotuner's register-cap dimension: kernels compiled for 4, 6 and 8 waves per
+SIMD (amdgpu_waves_per_eu, at most 512 / waves VGPRs per lane) run the same
+arithmetic as the default kernel, so every variant gives the same bits, and the
+binary64 ones equal the oracle; the tuner times them after the launch shape and
+later solves use its choice (reported in stats.waves)."""
+import numpy as np
+import pytest
+
+from helpers import assert_bit_identical
+from oracle import pyoracle
+from simplediffeq_hip import _abi, inputs, runtime
+
+pytestmark = pytest.mark.gpu
+
+F32_01 = float(np.float32(0.1))
+CASES = [(_abi.ALG_ATSIT5, _abi.F64, _abi.PREC_PLAIN), (_abi.ALG_ATSIT5, _abi.F32, _abi.PREC_PLAIN),
+         (_abi.ALG_ATSIT5, _abi.F32, _abi.PREC_KAHAN), (_abi.ALG_AVERN7, _abi.F64, _abi.PREC_PLAIN),
+         (_abi.ALG_AVERN7, _abi.F32, _abi.PREC_PLAIN)]
+IDS = ["atsit5-f64", "atsit5-f32", "atsit5-f32kahan", "avern7-f64", "avern7-f32"]
+
+
+def _run(ctx, alg, dtype, prec, waves, sched, n=30_001, seed=3):
+    u0, p, ts = inputs.lorenz_uneven(n, seed=seed)
+    ts[1] = np.minimum(ts[1], 5.0)
+    d = runtime.make_desc(alg, dtype=dtype, rhs=_abi.RHS_LORENZ, n_traj=n, dt=F32_01,
+                          abstol=1e-7, reltol=1e-6, precision=prec, schedule=sched, waves=waves)
+    g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device="cuda:0")
+    ctx.solve(d, g, sync=True)
+    return d, g, ctx.stats(), (u0, p, ts)
+
+
+@pytest.fixture(scope="module")
+def fresh_ctx():
+    c = runtime.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("alg,dtype,prec", CASES, ids=IDS)
+@pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT, _abi.SCHED_STAGED],
+                         ids=["one_launch", "persistent", "staged"])
+def test_register_cap_variants_give_the_same_bits(fresh_ctx, alg, dtype, prec, sched):
+    ctx = fresh_ctx  # no tuned shape: waves = 0 is the default kernel
+    _, ref, st0, inp = _run(ctx, alg, dtype, prec, 0, sched)
+    assert st0["waves"] == 0
+    for w in (4, 6, 8):
+        _, g, st, _ = _run(ctx, alg, dtype, prec, w, sched)
+        assert st["waves"] == w
+        assert_bit_identical(g, ref)
+    if dtype == _abi.F64 and sched == _abi.SCHED_PERSISTENT:
+        d, _, _, (u0, p, ts) = _run(ctx, alg, dtype, prec, 6, sched, n=4096, seed=4)
+        c = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts)
+        assert pyoracle.solve(d, c, nthreads=8) == 0
+        _, g, _, _ = _run(ctx, alg, dtype, prec, 6, sched, n=4096, seed=4)
+        assert_bit_identical(g, c)
+
+
+def test_register_cap_refusals(ctx):
+    n = 64
+    u0, p = inputs.lorenz_wide(n, seed=1)
+    with pytest.raises(runtime.SolveError, match="waves must be"):
+        d = runtime.make_desc(_abi.ALG_ATSIT5, n_traj=n, waves=5)
+        ctx.solve(d, runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0"))
+    # saveat and the fixed-step solvers have no variants
+    d = runtime.make_desc(_abi.ALG_ATSIT5, n_traj=n, waves=6, saveat=[0.0, 0.5])
+    with pytest.raises(runtime.SolveError, match="register-cap"):
+        ctx.solve(d, runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0"))
+    d = runtime.make_desc(_abi.ALG_TSIT5, n_traj=n, dt=0.01, waves=4)
+    with pytest.raises(runtime.SolveError, match="register-cap"):
+        ctx.solve(d, runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0"))
+    # nor the group and matrix-core layouts of GLV-16
+    u0g, r, A = inputs.glv16(n, seed=3)
+    d = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_GLV16, n_traj=n, tspan=(0.0, 1.0),
+                          waves=4)
+    with pytest.raises(runtime.SolveError, match="register-cap"):
+        ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0g, r, A, backend="torch", device="cuda:0"))
+
+
+def test_autotuner_reports_and_keeps_its_register_cap(ctx):
+    n = 100_000
+    u0, p = inputs.lorenz_wide(n, seed=1)
+    d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=n, tspan=(0.0, 10.0),
+                          dt=F32_01, abstol=1e-6, reltol=1e-3)
+    g = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
+    best = ctx.autotune(d, g)
+    assert best["waves"] in (0, 4, 6, 8)
+    ctx.solve(d, g, sync=True)
+    st = ctx.stats()
+    assert st["waves"] == best["waves"] and st["schedule"] == best["schedule"]
+    ref = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
+    d1 = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=n, tspan=(0.0, 10.0),
+                           dt=F32_01, abstol=1e-6, reltol=1e-3, schedule=_abi.SCHED_ONE_LAUNCH)
+    ctx.solve(d1, ref, sync=True)
+    assert_bit_identical(g, ref)
