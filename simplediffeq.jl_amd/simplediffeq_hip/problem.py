@@ -387,6 +387,15 @@ def _solve_many(eprob, alg, ens, idx, repeats, dt0s, order, kw):
         dtype = _abi.F32
     elif dtype in ("f64", np.float64):
         dtype = _abi.F64
+    # Precision policy: binary32 AVern9 below a tolerance of 1e-9 runs in binary64.
+    # Binary32 rounds at about 6e-8 relative, so the embedded error estimate of a
+    # step asked to stay below 1e-9 is mostly rounding (Verner 9's dense-output
+    # polynomials also carry coefficients up to ~1e4); the states come back in the
+    # caller's binary32.
+    out_np = None
+    if (dtype == _abi.F32 and alg.code == _abi.ALG_AVERN9
+            and min(kw["abstol"], kw["reltol"]) < 1e-9):
+        dtype, out_np = _abi.F64, np.float32
     cols = np.asarray(idx)
     if dt0s is not None:
         dt0s = np.asarray(dt0s)[cols]
@@ -451,6 +460,8 @@ def _solve_many(eprob, alg, ens, idx, repeats, dt0s, order, kw):
         else:
             t = np.array([t0s[i], bufs.t_end[i]]).astype(t_type)
             u = [u0s[:, i].astype(bufs.u_end.dtype), bufs.u_end[:, i]]
+        if out_np is not None:
+            u = [np.asarray(x).astype(out_np) for x in u]
         if base.scalar_u:
             u = [x[0] for x in u]
         sols.append(ODESolution(t, u, rc, bufs.naccept[i], bufs.nreject[i], alg))

@@ -99,6 +99,21 @@ def test_traced_rhs_matches_builtin_bitwise():
         assert bits_equal(a.raw.u_end, b.raw.u_end) and bits_equal(a.raw.naccept, b.raw.naccept)
 
 
+def oracle_ends(u0s, ps, tspans=None, tspan=(0.0, 1.0), alg=1, dt=0.1, abstol=1e-6,
+                reltol=1e-3):
+    """The CPU oracle's endpoints (n_state, n) of a Lorenz ensemble, the checker of
+    the ensemble surface's per-trajectory results."""
+    from oracle import pyoracle
+    from simplediffeq_hip import _abi, runtime
+    u0s = np.asarray(u0s, dtype=np.float64)
+    n = u0s.shape[1]
+    d = runtime.make_desc(alg, rhs=_abi.RHS_LORENZ, n_traj=n, tspan=tspan, dt=dt,
+                          abstol=abstol, reltol=reltol)
+    b = runtime.EnsembleBuffers(d, 3, u0s, np.asarray(ps, dtype=np.float64), tspan=tspans)
+    assert pyoracle.solve(d, b, nthreads=8) == 0
+    return b.u_end, b.t_end
+
+
 def test_ensemble_problem_prob_func_output_func_reduction():
     base = sd.ODEProblem(sd.rhs.lorenz, [1.0, 0.0, 0.0], (0.0, 1.0), [10.0, 0.0, 8 / 3])
     rhos = np.linspace(0.0, 21.0, 64)
@@ -111,11 +126,21 @@ def test_ensemble_problem_prob_func_output_func_reduction():
     single = [sd.solve(sd.remake(base, p=[10.0, r, 8 / 3]), sd.GPUSimpleATsit5(), dt=0.1,
                        abstol=1e-6, reltol=1e-3, save_everystep=False).u[-1][2] for r in rhos]
     assert esol.converged and esol.u == sum(single)
+    # each trajectory the reduction saw is the oracle's, summed in the same order
+    ps = np.stack([np.full(64, 10.0), rhos, np.full(64, 8 / 3)])
+    ue, _ = oracle_ends(np.tile([[1.0], [0.0], [0.0]], 64), ps)
+    assert esol.u == sum(float(z) for z in ue[2])
     # per-trajectory tspans from prob_func
     eprob = sd.EnsembleProblem(base, prob_func=lambda prob, i, r: sd.remake(prob, tspan=(0.0, 0.5 * i)))
     esol = sd.solve(eprob, sd.GPUSimpleATsit5(), sd.EnsembleHIP(), trajectories=4,
                     save_everystep=False)
     assert [s.t[-1] for s in esol.u] == [0.5, 1.0, 1.5, 2.0]
+    spans = np.array([[0.0] * 4, [0.5, 1.0, 1.5, 2.0]])
+    ue, te = oracle_ends(np.tile([[1.0], [0.0], [0.0]], 4), np.tile([[10.0], [0.0], [8 / 3]], 4),
+                         tspans=spans, dt=float(F(0.1)), abstol=float(F(1e-6)),
+                         reltol=float(F(1e-3)))
+    for i, s in enumerate(esol.u):
+        assert bits_equal(s.u[-1], ue[:, i]) and float(s.t[-1]) == float(F(te[i]))
 
 
 def test_ensemble_batches_and_reruns_match_one_solve():
@@ -133,12 +158,16 @@ def test_ensemble_batches_and_reruns_match_one_solve():
                     sd.EnsembleHIP(), batch_size=7, **kw)
     assert len(many.raw) == 15
     assert bits_equal(np.array([s.u[-1] for s in one.u]), np.array([s.u[-1] for s in many.u]))
+    ps = np.stack([np.full(100, 10.0), rhos, np.full(100, 8 / 3)])
+    ue, _ = oracle_ends(np.tile([[1.0], [0.0], [0.0]], 100), ps)
+    assert bits_equal(np.array([s.u[-1] for s in many.u]), ue.T)
     # output_func asks for reruns while z(1) > 5: those end with the rho that
     # brought z below 5, exactly as a direct solve with that rho
     ep = sd.EnsembleProblem(base, prob_func=prob_func,
                             output_func=lambda sol, i: (sol, bool(sol.u[-1][2] > 5.0)))
     res = sd.solve(ep, sd.GPUSimpleATsit5(), sd.EnsembleHIP(), batch_size=32, **kw)
     reran = 0
+    final_rho = []
     for i, s in enumerate(res.u):
         assert s.u[-1][2] <= 5.0
         rho = rhos[i]
@@ -150,7 +179,13 @@ def test_ensemble_batches_and_reruns_match_one_solve():
             rho /= 2
             reran += 1
         assert bits_equal(s.u[-1], d.u[-1])
+        final_rho.append(rho)
     assert reran > 0
+    # and every final solution is the oracle's for the rho its reruns ended with
+    n = len(final_rho)
+    ue, _ = oracle_ends(np.tile([[1.0], [0.0], [0.0]], n),
+                        np.stack([np.full(n, 10.0), final_rho, np.full(n, 8 / 3)]))
+    assert bits_equal(np.array([s.u[-1] for s in res.u]), ue.T)
 
 
 def test_dtmin_raises_like_the_reference():
@@ -194,3 +229,19 @@ def test_solution_times_take_dt_element_type():
         assert sd.solve(prob, alg, dt=F(0.1), save_everystep=False).t.dtype == np.float32
     r = sd.solve(prob, sd.GPUSimpleRK4(), dt=F(0.1))
     assert r.t.dtype == np.float64
+
+
+def test_binary32_avern9_below_1e9_runs_in_binary64():
+    # the precision policy: binary32 states with AVern9 at a tolerance below 1e-9 are
+    # integrated in binary64 and returned in binary32; at 1e-9 and above, binary32
+    prob = sd.ODEProblem(sd.rhs.lorenz, np.ones(3, dtype=F), (0.0, 2.0),
+                         np.array([10.0, 28.0, 8 / 3], dtype=F))
+    tight = sd.solve(prob, sd.GPUSimpleAVern9(), abstol=1e-11, reltol=1e-11, save_everystep=False)
+    p64 = sd.ODEProblem(sd.rhs.lorenz, np.ones(3), (0.0, 2.0),
+                        np.array([10.0, 28.0, 8 / 3], dtype=F).astype(np.float64))
+    ref = sd.solve(p64, sd.GPUSimpleAVern9(), abstol=1e-11, reltol=1e-11, save_everystep=False)
+    assert tight.u[-1].dtype == np.float32
+    assert np.array_equal(tight.u[-1], ref.u[-1].astype(np.float32))
+    assert tight.stats == ref.stats
+    loose = sd.solve(prob, sd.GPUSimpleAVern9(), abstol=1e-6, reltol=1e-6, save_everystep=False)
+    assert loose.u[-1].dtype == np.float32
