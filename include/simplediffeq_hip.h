@@ -217,6 +217,11 @@ typedef struct sdeh_io {
   int64_t* steps_offset;   /* SDEH_SAVE_PACKED: [n_traj + 1] first row of each
                               trajectory; steps_offset[n_traj] = rows used */
   int64_t steps_capacity;  /* SDEH_SAVE_PACKED: rows of u_steps and t_steps */
+  int32_t* visits;         /* optional [n_traj] debug counters, device memory only (ABI 4):
+                              each time a lane takes trajectory i up in a launch (its
+                              start, or a staged solve's resume) visits[i] gains 1, so a
+                              caller can check that every trajectory is handed out
+                              exactly once per launch.  NULL: no counting */
 } sdeh_io;
 
 /* Timing and work counters of the last sdeh_solve() on a context. */

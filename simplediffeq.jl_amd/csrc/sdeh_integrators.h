@@ -952,6 +952,11 @@ __device__ __forceinline__ int64_t wave_claim(const sdeh_kargs& a, bool need_me,
   return mine;
 }
 
+// The debug visit counter of trajectory `traj` (sdeh_kargs::visits), one per take-up.
+__device__ __forceinline__ void note_visit(const sdeh_kargs& a, int64_t traj) {
+  if (a.visits) atomicAdd(a.visits + traj, 1);
+}
+
 // Save time i, or 0 past the end (callers test i < n_saveat first).  The next save
 // time is kept in a register between steps and the one after it is loaded while
 // the current point's dense output is computed, so the save test on each step
@@ -1035,6 +1040,7 @@ __device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
     if (__ballot(active) == 0ull) break;
 
     if (fresh) {  // start trajectory `traj` (gpuatsit5.jl:221-241)
+      if (lead) note_visit(a, traj);
       load_state<T, R>(uprev, a.u0, n, traj);
 #pragma unroll
       for (int c = 0; c < N; ++c) comp[c] = T(0);
@@ -1368,6 +1374,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
 
     if (fresh && STAGED && a.resume) {  // continue a parked trajectory
       fresh = false;
+      note_visit(a, traj);
       Park::load(a, traj, uprev, fsal, comp, pl, t, tf, dt, qold_pow, nacc, nrej);
       qold = qoldinit;  // written before it is read again
       attempts = (int64_t)nacc + nrej;
@@ -1375,6 +1382,7 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
     }
     if (fresh) {  // start trajectory `traj` (gpuatsit5.jl:221-241)
       fresh = false;
+      if (lead) note_visit(a, traj);
       load_state<T, R>(uprev, a.u0, n, traj);
 #pragma unroll
       for (int c = 0; c < N; ++c) comp[c] = T(0);
@@ -1573,6 +1581,7 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
   const bool lead = lane_comp<R>() == 0;
   const int64_t n = a.n;
   const int64_t traj = a.order ? a.order[slot] : slot;
+  if (lead) note_visit(a, traj);
   const T* ps = (const T*)a.ps;
   T u[N], uprev[N], pl[P], zero[N], dummy[N], err[N];
   load_state<T, R>(u, a.u0, n, traj);

@@ -65,6 +65,9 @@ struct sdeh_kargs {
   int64_t st_stride;
   int32_t resume;
   int32_t st_pad_;
+  // Optional [n] debug counters: a lane that takes trajectory i up in a launch
+  // (a fresh start, or a staged solve resuming it) adds 1 to visits[i].
+  int32_t* visits;
 };
 
 // Parked-trajectory record: double t, tf; int32 naccept, nreject, flag, pad; then

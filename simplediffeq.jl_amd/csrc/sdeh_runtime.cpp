@@ -945,6 +945,8 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   sdeh_kargs& a = pl.a;
   const bool host = memory == SDEH_MEM_HOST;
   if (!host && memory != SDEH_MEM_DEVICE) return fail(c, SDEH_E_INVALID, "memory mode");
+  if (host && io->visits)
+    return fail(c, SDEH_E_INVALID, "visits counters take device memory (SDEH_MEM_DEVICE)");
   if (host && io->order && n > 0) {
     r = check_order_host(c, io->order, n);
     if (r) return r;
@@ -1015,6 +1017,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
     if (!r) r = check_dev(c, io->retcode, sz.i32, "retcode");
     if (!r && io->naccept) r = check_dev(c, io->naccept, sz.i32, "naccept");
     if (!r && io->nreject) r = check_dev(c, io->nreject, sz.i32, "nreject");
+    if (!r && io->visits) r = check_dev(c, io->visits, sz.i32, "visits");
     if (!r && pl.save == SAVE_AT) r = check_dev(c, io->u_saveat, sz.u_saveat, "u_saveat");
     if (!r && pl.save == SAVE_STEPS) {
       r = check_dev(c, io->u_steps, sz.u_steps, "u_steps");
@@ -1023,6 +1026,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
       if (!r && pl.packed) r = check_dev(c, io->steps_offset, sz.pk_off, "steps_offset");
     }
     if (r) return r;
+    a.visits = io->visits;
     a.u0 = io->u0;
     a.p = io->p;
     a.ps = io->p_shared;
@@ -1207,6 +1211,7 @@ int solve_static_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, 
     if (!r) r = check_dev(c, io->retcode, sz.i32, "retcode");
     if (!r) r = check_dev(c, io->naccept, sz.i32, "naccept");
     if (!r) r = check_dev(c, io->nreject, sz.i32, "nreject");
+    if (!r && io->visits) r = check_dev(c, io->visits, sz.i32, "visits");
     if (!r && pl.save == SAVE_AT) r = check_dev(c, io->u_saveat, sz.u_saveat, "u_saveat");
     if (r) return r;
   }
@@ -1231,6 +1236,7 @@ int solve_static_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, 
   a.naccept = io->naccept;
   a.nreject = io->nreject;
   a.u_saveat = io->u_saveat;
+  a.visits = io->visits;
   r = run_plan(c, pl, s, words, true);  // run_plan clears the queue word before a persistent launch
   if (r) return r;
   note_pending(c, s, false);
@@ -1466,6 +1472,7 @@ int sdeh_graph_capture(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io,
     if (!r) r = check_dev(c, io->t_steps, sz.t_steps, "t_steps");
     if (!r) r = check_dev(c, io->n_steps, sz.i32, "n_steps");
   }
+  if (!r && io->visits) r = check_dev(c, io->visits, sz.i32, "visits");
   if (r) return r;
   if (n == 0) return fail(c, SDEH_E_INVALID, "empty ensemble");
   GraphRec g;
@@ -1492,6 +1499,7 @@ int sdeh_graph_capture(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io,
   a.u_steps = io->u_steps;
   a.t_steps = io->t_steps;
   a.n_steps = io->n_steps;
+  a.visits = io->visits;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   if (pl.span) {  // the hand-out order is recomputed on every replay

@@ -101,6 +101,7 @@ class SolveIO(ctypes.Structure):
         ("n_steps", ctypes.c_void_p),
         ("steps_offset", ctypes.c_void_p),
         ("steps_capacity", ctypes.c_int64),
+        ("visits", ctypes.c_void_p),
     ]
 
 
