@@ -189,6 +189,15 @@ typedef struct sdeh_solve_desc {
                                Lorenz right-hand side's adaptive ATsit5 and AVern7
                                endpoint kernels (SDEH_E_UNSUPPORTED elsewhere).  Same
                                arithmetic, same results.  (ABI 4; was reserved, 0.) */
+  int32_t per_lane;         /* trajectories per lane: 0 = tuned (else 1); 2 = the variant
+                               whose lanes each integrate two trajectories, their
+                               stages interleaved (instruction-level parallelism in
+                               place of waves), built for the Lorenz right-hand side's
+                               adaptive ATsit5 (binary64, binary32, compensated) and
+                               AVern7 (binary64) endpoint kernels and for registered
+                               right-hand sides; one-launch and persistent schedules (a
+                               staged shape runs persistent).  Same results.  (ABI 4) */
+  int32_t reserved2;
 } sdeh_solve_desc;
 
 /* Solve buffers.  With memory == SDEH_MEM_HOST these are host pointers and the
@@ -236,6 +245,8 @@ typedef struct sdeh_stats {
   int32_t waves;           /* register-cap variant used (0 = the default kernel) */
   int64_t steps_needed;    /* SDEH_SAVE_PACKED: total points offered by all
                               trajectories (the steps_capacity that holds them all) */
+  int32_t per_lane;        /* trajectories per lane used */
+  int32_t reserved1;
 } sdeh_stats;
 
 typedef struct sdeh_ctx sdeh_ctx;
@@ -298,8 +309,9 @@ int sdeh_graph_destroy(sdeh_ctx* ctx, int64_t graph_handle);
 
 /* Launch-shape autotuner: time candidate (block, grid, chunk, schedule) shapes
  * for the given problem on device buffers, then the register-cap variants (waves)
- * of the best one where they are built, and store the best for (alg, rhs, dtype,
- * save mode, precision, lanes).  Later solves with tuned fields left 0 use it. */
+ * of the best one and the two-trajectories-per-lane variant (one launch and
+ * persistent) where they are built, and store the best for (alg, rhs, dtype, save
+ * mode, precision, lanes).  Later solves with tuned fields left 0 use it. */
 int sdeh_autotune(sdeh_ctx* ctx, const sdeh_solve_desc* d, const sdeh_io* io,
                   sdeh_stats* best);
 

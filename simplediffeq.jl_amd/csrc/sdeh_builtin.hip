@@ -44,6 +44,20 @@ const void* waves_variant(int waves) {
   return nullptr;
 }
 
+// Two trajectories per lane (SDEH_WAVE_VARIANTS): ATsit5 (binary64, binary32 plain
+// and compensated) and AVern7 (binary64), endpoint output.
+template <class T, int ALG, bool KAHAN>
+const void* multi_variant(int tpl) {
+#if defined(SDEH_WAVE_VARIANTS)
+  if constexpr ((ALG == sdeh::ALG_ATSIT5) || (ALG == sdeh::ALG_AVERN7 && sizeof(T) == 8 && !KAHAN)) {
+    if (tpl == 2)
+      return reinterpret_cast<const void*>(&sdeh_adaptive_multi_kernel<T, R, ALG, KAHAN, 2>);
+  }
+#endif
+  (void)tpl;
+  return nullptr;
+}
+
 // Which variants this translation unit instantiates.  The 16-species GLV
 // right-hand side is only built for the variants its benchmark and tests use
 // (each of its kernels is large); SDEH_LIMITED selects that subset.
@@ -75,14 +89,14 @@ const void* adaptive_or_null() {
 }
 
 template <class T, int ALG>
-const void* by_save(int save, int kahan, int waves) {
+const void* by_save(int save, int kahan, int waves, int tpl) {
   constexpr bool fixed = (ALG == sdeh::ALG_TSIT5 || ALG == sdeh::ALG_VERN7 ||
                           ALG == sdeh::ALG_VERN9 || ALG == sdeh::ALG_RK4 ||
                           ALG == sdeh::ALG_EULER);
   // GPUSimpleRK4 and GPUSimpleEuler have no interpolant: no saveat kernel.
   constexpr bool dense = !(ALG == sdeh::ALG_RK4 || ALG == sdeh::ALG_EULER);
   if constexpr (fixed) {
-    if (kahan) return nullptr;
+    if (kahan || waves || tpl > 1) return nullptr;
     switch (save) {
       case sdeh::SAVE_END: return fixed_or_null<T, ALG, sdeh::SAVE_END>();
       case sdeh::SAVE_AT:
@@ -92,6 +106,14 @@ const void* by_save(int save, int kahan, int waves) {
     }
     return nullptr;
   } else {
+    if (tpl > 1) {
+      if (save != sdeh::SAVE_END || waves) return nullptr;
+      if (kahan) {
+        if constexpr (sizeof(T) == 4) return multi_variant<T, ALG, true>(tpl);
+        return nullptr;
+      }
+      return multi_variant<T, ALG, false>(tpl);
+    }
     if (waves) {
       if (save != sdeh::SAVE_END) return nullptr;
       if (kahan) {
@@ -117,16 +139,16 @@ const void* by_save(int save, int kahan, int waves) {
 }
 
 template <class T>
-const void* by_alg(int alg, int save, int kahan, int waves) {
+const void* by_alg(int alg, int save, int kahan, int waves, int tpl) {
   switch (alg) {
-    case sdeh::ALG_TSIT5: return by_save<T, sdeh::ALG_TSIT5>(save, kahan, waves);
-    case sdeh::ALG_ATSIT5: return by_save<T, sdeh::ALG_ATSIT5>(save, kahan, waves);
-    case sdeh::ALG_VERN7: return by_save<T, sdeh::ALG_VERN7>(save, kahan, waves);
-    case sdeh::ALG_AVERN7: return by_save<T, sdeh::ALG_AVERN7>(save, kahan, waves);
-    case sdeh::ALG_VERN9: return by_save<T, sdeh::ALG_VERN9>(save, kahan, waves);
-    case sdeh::ALG_AVERN9: return by_save<T, sdeh::ALG_AVERN9>(save, kahan, waves);
-    case sdeh::ALG_RK4: return by_save<T, sdeh::ALG_RK4>(save, kahan, waves);
-    case sdeh::ALG_EULER: return by_save<T, sdeh::ALG_EULER>(save, kahan, waves);
+    case sdeh::ALG_TSIT5: return by_save<T, sdeh::ALG_TSIT5>(save, kahan, waves, tpl);
+    case sdeh::ALG_ATSIT5: return by_save<T, sdeh::ALG_ATSIT5>(save, kahan, waves, tpl);
+    case sdeh::ALG_VERN7: return by_save<T, sdeh::ALG_VERN7>(save, kahan, waves, tpl);
+    case sdeh::ALG_AVERN7: return by_save<T, sdeh::ALG_AVERN7>(save, kahan, waves, tpl);
+    case sdeh::ALG_VERN9: return by_save<T, sdeh::ALG_VERN9>(save, kahan, waves, tpl);
+    case sdeh::ALG_AVERN9: return by_save<T, sdeh::ALG_AVERN9>(save, kahan, waves, tpl);
+    case sdeh::ALG_RK4: return by_save<T, sdeh::ALG_RK4>(save, kahan, waves, tpl);
+    case sdeh::ALG_EULER: return by_save<T, sdeh::ALG_EULER>(save, kahan, waves, tpl);
   }
   return nullptr;
 }
@@ -134,11 +156,12 @@ const void* by_alg(int alg, int save, int kahan, int waves) {
 }  // namespace
 
 // dtype: 0 binary64, 1 binary32; waves: 0 for the default kernel, else a
-// register-cap variant (null where not built).  Returns the kernel's host stub or null.
+// register-cap variant; tpl: trajectories per lane (1, or 2 for the multi-slot
+// variant).  Returns the kernel's host stub, or null where not built.
 extern "C" __attribute__((visibility("hidden"))) const void* SDEH_AOT_FN(int dtype, int alg,
                                                                          int save, int kahan,
-                                                                         int waves) {
-  if (dtype == 0) return by_alg<double>(alg, save, kahan, waves);
-  if (dtype == 1) return by_alg<float>(alg, save, kahan, waves);
+                                                                         int waves, int tpl) {
+  if (dtype == 0) return by_alg<double>(alg, save, kahan, waves, tpl);
+  if (dtype == 1) return by_alg<float>(alg, save, kahan, waves, tpl);
   return nullptr;
 }

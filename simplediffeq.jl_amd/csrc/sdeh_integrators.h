@@ -1558,6 +1558,192 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
 }
 
 // ---------------------------------------------------------------------------
+// Several trajectories per lane (TPL slots; one lane per trajectory's state,
+// endpoint output, ATsit5 and AVern7).  Every iteration runs one attempt of every
+// slot: the slots' stage and controller arithmetic sits in one straight-line block,
+// so the compiler interleaves the independent chains (instruction-level parallelism
+// in place of waves), and then each slot that attempted takes its own decision and
+// bookkeeping exactly as adaptive_body does, with the same arithmetic: a
+// trajectory's results do not depend on its slot or neighbour.  A slot without a
+// running trajectory computes on its last values (all finite or not, results
+// dropped) until it is given a new one; only attempting slots can send the wave to
+// the IEEE controller.  One launch: lane g takes trajectories TPL g .. TPL g + TPL-1
+// (of the hand-out order); persistent: each empty slot claims from the queue.
+// ---------------------------------------------------------------------------
+template <class T, class R, int ALG, bool KAHAN, int TPL>
+__device__ __forceinline__ void adaptive_body_multi(const sdeh_kargs& a) {
+  static_assert(R::G == 1 && !IsWave<R>::value, "one lane per trajectory state");
+  static_assert(ALG == ALG_ATSIT5 || ALG == ALG_AVERN7, "ATsit5 and AVern7");
+  constexpr int N = R::N;
+  constexpr int P = R::P > 0 ? R::P : 1;
+  constexpr double DTMIN = 1.0e-14, SNAP = 1.0e-14;  // gpuatsit5.jl:256,294; gpuvern7.jl:383,421
+  // build_adaptive_controller_cache(T), src/SimpleDiffEq.jl:67-77
+  const T beta1 = T(7.0 / 50), beta2 = T(2.0 / 25), qmax = T(10.0), qmin = T(1.0 / 5),
+          gamma = T(9.0 / 10), qoldinit = T(1.0e-4);
+  const T inv_qmax = T(1) / qmax, inv_qmin = T(1) / qmin;
+  const T qoldinit_pow = Ops<T>::pw(qoldinit, beta2);
+  const T abstol = (T)a.abstol, reltol = (T)a.reltol;
+  const CtrlK<T> ck{beta1, inv_qmax, inv_qmin, gamma, Ops<T>::rcp_nr(T(NormLen<R>::value)),
+                    Ops<T>::rcp_nr(gamma)};
+  const T* ps = (const T*)a.ps;
+  const int64_t n = a.n;
+
+  T uprev[TPL][N], comp[TPL][N], fsal[TPL][N], pl[TPL][P];
+  double t[TPL], tf[TPL];
+  T dt[TPL], qold_pow[TPL];
+  int32_t nacc[TPL], nrej[TPL];
+  int64_t attempts[TPL], traj[TPL];
+  bool active[TPL], fresh[TPL];
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) uprev[k][c] = comp[k][c] = fsal[k][c] = T(0);
+#pragma unroll
+    for (int c = 0; c < P; ++c) pl[k][c] = T(0);
+    t[k] = tf[k] = 0.0;
+    dt[k] = T(1);
+    qold_pow[k] = qoldinit_pow;
+    nacc[k] = nrej[k] = 0;
+    attempts[k] = 0;
+    traj[k] = -1;
+    active[k] = fresh[k] = false;
+  }
+  int64_t qbase = 0, qlim = 0;
+  bool drained = !a.persistent;
+  if (!a.persistent) {
+    const int64_t base = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * TPL;
+#pragma unroll
+    for (int k = 0; k < TPL; ++k)
+      if (base + k < n) {
+        traj[k] = a.order ? a.order[base + k] : base + k;
+        active[k] = fresh[k] = true;
+      }
+  }
+
+  for (;;) {
+    if (a.persistent) {
+#pragma unroll
+      for (int k = 0; k < TPL; ++k) {
+        const int64_t slot = wave_claim(a, !active[k], qbase, qlim, drained);
+        if (slot >= 0) {
+          traj[k] = a.order ? a.order[slot] : slot;
+          active[k] = fresh[k] = true;
+        }
+      }
+    }
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) any = any || active[k];
+    if (__ballot(any) == 0ull) break;
+
+#pragma unroll
+    for (int k = 0; k < TPL; ++k)
+      if (fresh[k]) {  // start trajectory traj[k] (gpuatsit5.jl:221-241)
+        fresh[k] = false;
+        note_visit(a, traj[k]);
+        load_state<T, R>(uprev[k], a.u0, n, traj[k]);
+#pragma unroll
+        for (int c = 0; c < N; ++c) comp[k][c] = T(0);
+        load_params<T, R>(pl[k], a, traj[k]);
+        t[k] = a.tspan ? a.tspan[traj[k]] : a.t0;
+        tf[k] = a.tspan ? a.tspan[n + traj[k]] : a.tf;
+        dt[k] = (T)(a.dt0 ? a.dt0[traj[k]] : a.dt);
+        qold_pow[k] = qoldinit_pow;
+        nacc[k] = nrej[k] = 0;
+        attempts[k] = 0;
+        if constexpr (ALG == ALG_ATSIT5) R::f(fsal[k], uprev[k], pl[k], ps, t[k]);  // :241
+      }
+
+    int ret[TPL];
+    bool go[TPL];
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+      ret[k] = -1;
+      go[k] = false;
+      if (active[k]) {
+        if (!(t[k] < tf[k])) ret[k] = RET_DEFAULT;
+        else if ((double)dt[k] < DTMIN) ret[k] = RET_DTMIN;  // :256
+        else if (attempts[k] >= a.maxiters) ret[k] = RET_MAXITERS;
+        else {
+          go[k] = true;
+          ++attempts[k];
+        }
+      }
+    }
+    // ---- one attempt of every slot: stages, trial state and embedded error ----
+    T u[TPL][N], err[TPL][N], comp_new[TPL][N];
+    T k2[TPL][N], k3[TPL][N], k4[TPL][N], k5[TPL][N], k6[TPL][N], k7[TPL][N];
+    V7K<T, N> K7[TPL];
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+      if constexpr (ALG == ALG_ATSIT5) {
+        tsit5_stages<T, R, KAHAN>(uprev[k], fsal[k], k2[k], k3[k], k4[k], k5[k], k6[k], k7[k],
+                                  u[k], comp[k], comp_new[k], pl[k], ps, t[k], dt[k]);
+        lsum(err[k], T(sdeh_tab::tsit5::btilde1), fsal[k], T(sdeh_tab::tsit5::btilde2), k2[k],
+             T(sdeh_tab::tsit5::btilde3), k3[k], T(sdeh_tab::tsit5::btilde4), k4[k],
+             T(sdeh_tab::tsit5::btilde5), k5[k], T(sdeh_tab::tsit5::btilde6), k6[k],
+             T(sdeh_tab::tsit5::btilde7), k7[k]);
+      } else {
+        vern7_stages<T, R, true, KAHAN>(uprev[k], K7[k], u[k], err[k], comp[k], comp_new[k],
+                                        pl[k], ps, t[k], dt[k]);
+      }
+    }
+    CtrlOut<T> co[TPL];
+    T ssq[TPL];
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) err[k][c] = dt[k] * err[k][c];
+      ssq[k] = err_sumsq<T, R>(err[k], uprev[k], u[k], abstol, reltol);
+      if constexpr (Ops<T>::kFast) co[k] = controller<T, R, true>(ssq[k], dt[k], qold_pow[k], ck);
+      else co[k] = controller<T, R, false>(ssq[k], dt[k], qold_pow[k], ck);
+    }
+    if constexpr (Ops<T>::kFast) {
+#pragma unroll
+      for (int k = 0; k < TPL; ++k)  // the IEEE controller again where the check fails
+        if (go[k] && !ctrl_in_range(ssq[k], dt[k]))
+          co[k] = controller<T, R, false>(ssq[k], dt[k], qold_pow[k], ck);
+    }
+    // ---- each attempting slot's decision (gpuatsit5.jl:279-299, gpuvern7.jl:406-426) ----
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+      if (!go[k]) continue;
+      const T EEst = co[k].EEst;
+      if (EEst != EEst) {
+        ret[k] = RET_NAN;
+        continue;
+      }
+      if (EEst > T(1)) {
+        dt[k] = co[k].dtq;
+        ++nrej[k];
+        continue;
+      }
+      const T qold = sdeh_math::fast_max(EEst, qoldinit);
+      qold_pow[k] = (qold == EEst) ? Ops<T>::pw_from(co[k].lg, beta2) : qoldinit_pow;
+      const T dtold = dt[k];
+      const double rem = tf[k] - t[k] - (double)dtold;
+      dt[k] = (T)sdeh_math::fast_min((double)fabs(co[k].dtq), fabs(rem));
+      t[k] = (rem < SNAP) ? tf[k] : t[k] + (double)dtold;
+      ++nacc[k];
+      copyv(uprev[k], u[k]);
+      if constexpr (ALG == ALG_ATSIT5) copyv(fsal[k], k7[k]);
+      if constexpr (KAHAN) copyv(comp[k], comp_new[k]);
+      if (!(t[k] < tf[k])) ret[k] = RET_DEFAULT;
+    }
+#pragma unroll
+    for (int k = 0; k < TPL; ++k)
+      if (ret[k] != -1) {  // the trajectory ends: outputs (gpuatsit5.jl:322-329)
+        store_state<T, R>(a.u_end, 0, n, traj[k], uprev[k]);
+        a.t_end[traj[k]] = t[k];
+        a.retcode[traj[k]] = ret[k];
+        if (a.naccept) a.naccept[traj[k]] = nacc[k];
+        if (a.nreject) a.nreject[traj[k]] = nrej[k];
+        active[k] = false;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Fixed-step solvers: Tsit5, Vern7, Vern9 over t0:dt:tf.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double grid_elem(const sdeh_kargs& a, int64_t i) {
@@ -1737,6 +1923,12 @@ __global__ void __launch_bounds__(256)
                                                  : sdeh::WavesHint<T, R, ALG, SAVE>::value)))
     sdeh_adaptive_kernel(sdeh_kargs a) {
   sdeh::adaptive_body<T, R, ALG, SAVE, KAHAN>(a);
+}
+
+// TPL trajectories per lane (adaptive_body_multi).
+template <class T, class R, int ALG, bool KAHAN, int TPL>
+__global__ void __launch_bounds__(256) sdeh_adaptive_multi_kernel(sdeh_kargs a) {
+  sdeh::adaptive_body_multi<T, R, ALG, KAHAN, TPL>(a);
 }
 
 template <class T, class R, int ALG, int SAVE>

@@ -31,13 +31,13 @@
 #define SDEH_HIDDEN __attribute__((visibility("hidden")))
 
 extern "C" {
-SDEH_HIDDEN const void* sdeh_aot_Lorenz(int, int, int, int, int);
-SDEH_HIDDEN const void* sdeh_aot_GLV16(int, int, int, int, int);
-SDEH_HIDDEN const void* sdeh_aot_GLV16G(int, int, int, int, int);
-SDEH_HIDDEN const void* sdeh_aot_GLV16M(int, int, int, int, int);
-SDEH_HIDDEN const void* sdeh_aot_ExpDecay3(int, int, int, int, int);
-SDEH_HIDDEN const void* sdeh_aot_Affine1(int, int, int, int, int);
-SDEH_HIDDEN const void* sdeh_aot_Linear2(int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_Lorenz(int, int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_GLV16(int, int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_GLV16G(int, int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_GLV16M(int, int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_ExpDecay3(int, int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_Affine1(int, int, int, int, int, int);
+SDEH_HIDDEN const void* sdeh_aot_Linear2(int, int, int, int, int, int);
 SDEH_HIDDEN hipError_t sdeh_util_fill_f64(double*, double, int64_t, hipStream_t);
 SDEH_HIDDEN hipError_t sdeh_util_fill_f32(float*, float, int64_t, hipStream_t);
 SDEH_HIDDEN hipError_t sdeh_util_zero_u64(unsigned long long*, int, hipStream_t);
@@ -94,13 +94,13 @@ struct Builtin {
   const char* name;
   int32_t n, p;
   int64_t s;
-  const void* (*lookup)(int, int, int, int, int);
+  const void* (*lookup)(int, int, int, int, int, int);
   // Kernels of the group layout (`group` lanes per trajectory), if built.
   int32_t group;
-  const void* (*group_lookup)(int, int, int, int, int);
+  const void* (*group_lookup)(int, int, int, int, int, int);
   // Kernels of the matrix-core wave layout (`mgroup` lanes per trajectory), if built.
   int32_t mgroup;
-  const void* (*mgroup_lookup)(int, int, int, int, int);
+  const void* (*mgroup_lookup)(int, int, int, int, int, int);
 };
 
 const Builtin kBuiltins[] = {
@@ -129,11 +129,19 @@ struct KernelRef {
   hipFunction_t fn = nullptr;
   int32_t lanes = 1;  // lanes per trajectory
   int32_t waves = 0;  // register-cap variant (0 = the default kernel)
+  int32_t tpl = 1;    // trajectories per lane
 };
 
 struct Shape {
-  int32_t schedule = SDEH_SCHED_ONE_LAUNCH, block = 256, grid = 0, chunk = 64, waves = 0;
+  int32_t schedule = SDEH_SCHED_ONE_LAUNCH, block = 256, grid = 0, chunk = 64, waves = 0,
+          per_lane = 1;
 };
+
+// Workgroups of a one-launch grid: every trajectory a lane (or TPL per lane).
+int64_t one_launch_groups(int64_t n, const KernelRef& k, int block) {
+  const int64_t lanes = (n + k.tpl - 1) / k.tpl * k.lanes;
+  return (lanes + block - 1) / block;
+}
 
 using TuneKey = std::tuple<int64_t, int, int, int, int, int>;  // rhs, alg, dtype, save, kahan, lanes
 
@@ -235,17 +243,21 @@ const std::vector<std::string>& rtc_options() {
 
 // Compile (or fetch) the integrator kernel for a registered RHS.
 int user_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kahan,
-                KernelRef* out, int waves = 0) {
+                KernelRef* out, int waves = 0, int tpl = 1) {
   const UserRhs& u = c->user[rhs - kUserBase];
   if (waves && (!is_adaptive(alg) || save != SAVE_END))
     return fail(c, SDEH_E_UNSUPPORTED, "register-cap variants are adaptive endpoint kernels");
+  if (tpl > 1 && (save != SAVE_END || waves || (alg != SDEH_ALG_ATSIT5 && alg != SDEH_ALG_AVERN7)))
+    return fail(c, SDEH_E_UNSUPPORTED,
+                "several trajectories per lane: ATsit5 and AVern7, endpoint output");
   char key[128];
-  snprintf(key, sizeof key, "%lld/%d/%d/%d/%d/%d", (long long)rhs, alg, dtype, save, kahan,
-           waves);
+  snprintf(key, sizeof key, "%lld/%d/%d/%d/%d/%d/%d", (long long)rhs, alg, dtype, save, kahan,
+           waves, tpl);
   auto it = c->rtc.find(key);
   if (it != c->rtc.end()) {
     out->fn = it->second.second;
     out->waves = waves;
+    out->tpl = tpl;
     return SDEH_OK;
   }
   const char* T = dtype == SDEH_F64 ? "double" : "float";
@@ -263,7 +275,12 @@ int user_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kaha
            "};\n",
            u.n, u.p, (long long)u.s, u.fn.c_str());
   src += wrap;
-  if (is_adaptive(alg) && waves)
+  if (tpl > 1)
+    snprintf(wrap, sizeof wrap,
+             "extern \"C\" __global__ void __launch_bounds__(256) sdeh_user_kernel(sdeh_kargs a) "
+             "{ sdeh::adaptive_body_multi<%s, sdeh_user_rhs, %d, %s, %d>(a); }\n",
+             T, alg, kahan ? "true" : "false", tpl);
+  else if (is_adaptive(alg) && waves)
     snprintf(wrap, sizeof wrap,
              "extern \"C\" __global__ void __launch_bounds__(256) "
              "__attribute__((amdgpu_waves_per_eu(%d))) sdeh_user_kernel(sdeh_kargs a) "
@@ -343,6 +360,7 @@ int user_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kaha
   c->rtc[key] = {mod, fn};
   out->fn = fn;
   out->waves = waves;
+  out->tpl = tpl;
   return SDEH_OK;
 }
 
@@ -356,19 +374,28 @@ int user_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kaha
 // faster there than in the group layout, which repeats the controller on all 16
 // lanes.)
 int resolve_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int kahan,
-                   int lanes, KernelRef* out, int waves = 0) {
+                   int lanes, KernelRef* out, int waves = 0, int tpl = 1) {
   out->waves = 0;
+  out->tpl = 1;
+  if (waves && tpl > 1)
+    return fail(c, SDEH_E_UNSUPPORTED,
+                "register caps and several trajectories per lane do not combine");
   if (const Builtin* b = find_builtin(rhs)) {
-    if (waves) {  // register-cap variants: one lane per trajectory only
+    if (waves || tpl > 1) {  // the variants run one lane per trajectory state
       if (lanes > 1 || (lanes == 0 && dtype == SDEH_F64 && (b->mgroup > 1 || b->group > 1)))
-        return fail(c, SDEH_E_UNSUPPORTED, "register-cap variants run one lane per trajectory");
+        return fail(c, SDEH_E_UNSUPPORTED, waves ? "register-cap variants run one lane per "
+                                                   "trajectory"
+                                                 : "several trajectories per lane run one lane "
+                                                   "per trajectory state");
       out->lanes = 1;
-      out->aot = b->lookup(dtype, alg, save, kahan, waves);
+      out->aot = b->lookup(dtype, alg, save, kahan, waves, tpl);
       if (!out->aot)
         return fail(c, SDEH_E_UNSUPPORTED,
-                    "no register-cap variant (%d waves) for rhs %s, alg %d, dtype %d, save "
-                    "mode %d, precision %d", waves, b->name, alg, dtype, save, kahan);
+                    "no %s variant for rhs %s, alg %d, dtype %d, save mode %d, precision %d",
+                    waves ? "register-cap" : "several-trajectories-per-lane", b->name, alg,
+                    dtype, save, kahan);
       out->waves = waves;
+      out->tpl = tpl;
       return SDEH_OK;
     }
     if (lanes != 0 && lanes != 1 && lanes != b->group && !(b->mgroup > 1 && lanes == b->mgroup)) {
@@ -383,7 +410,7 @@ int resolve_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int k
                   lanes);
     }
     if (b->mgroup > 1 && (lanes == b->mgroup || (lanes == 0 && dtype == SDEH_F64))) {
-      out->aot = b->mgroup_lookup(dtype, alg, save, kahan, 0);
+      out->aot = b->mgroup_lookup(dtype, alg, save, kahan, 0, 1);
       out->lanes = b->mgroup;
       if (out->aot) return SDEH_OK;
       if (lanes == b->mgroup)
@@ -393,7 +420,7 @@ int resolve_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int k
     }
     if ((lanes == b->group || (lanes == 0 && dtype == SDEH_F64)) && b->group > 1 &&
         b->group_lookup) {
-      out->aot = b->group_lookup(dtype, alg, save, kahan, 0);
+      out->aot = b->group_lookup(dtype, alg, save, kahan, 0, 1);
       out->lanes = b->group;
       if (out->aot || lanes == b->group) {
         if (!out->aot)
@@ -404,7 +431,7 @@ int resolve_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int k
       }
     }
     out->lanes = 1;
-    out->aot = b->lookup(dtype, alg, save, kahan, 0);
+    out->aot = b->lookup(dtype, alg, save, kahan, 0, 1);
     if (!out->aot)
       return fail(c, SDEH_E_UNSUPPORTED,
                   "no built-in kernel for rhs %s, alg %d, dtype %d, save mode %d, precision %d",
@@ -414,7 +441,7 @@ int resolve_kernel(sdeh_ctx* c, int64_t rhs, int alg, int dtype, int save, int k
   if (lanes > 1)
     return fail(c, SDEH_E_INVALID, "registered right-hand sides run one lane per trajectory");
   out->lanes = 1;
-  return user_kernel(c, rhs, alg, dtype, save, kahan, out, waves);
+  return user_kernel(c, rhs, alg, dtype, save, kahan, out, waves, tpl);
 }
 
 hipError_t launch(const KernelRef& k, int grid, int block, sdeh_kargs* a, hipStream_t s) {
@@ -625,24 +652,32 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
   Shape sh;
   if (d->waves != 0 && d->waves != 4 && d->waves != 6 && d->waves != 8)
     return fail(c, SDEH_E_INVALID, "waves must be 0, 4, 6 or 8");
-  if (d->waves && !adaptive)
-    return fail(c, SDEH_E_UNSUPPORTED, "register-cap variants are adaptive endpoint kernels");
+  if (d->per_lane < 0 || d->per_lane > 2)
+    return fail(c, SDEH_E_INVALID, "per_lane must be 0, 1 or 2");
+  if ((d->waves || d->per_lane == 2) && !adaptive)
+    return fail(c, SDEH_E_UNSUPPORTED,
+                "register-cap and several-trajectories-per-lane variants are adaptive endpoint "
+                "kernels");
   if (adaptive) {
     auto it = c->tuned.find(TuneKey(d->rhs, d->alg, d->dtype, pl->save, pl->kahan, pl->k.lanes));
     if (it != c->tuned.end()) sh = it->second;
     else sh.schedule = SDEH_SCHED_PERSISTENT;
-    // the register-cap variant: the caller's, else the tuned one (when it exists
-    // for this solve), else the default kernel
-    const int waves = d->waves ? d->waves : sh.waves;
+    // the kernel variant: the caller's register cap or trajectories per lane, else
+    // the tuned ones (where they exist for this solve), else the default kernel
+    const bool mine = d->waves || d->per_lane;
+    const int waves = mine ? d->waves : sh.waves;
+    const int tpl = mine ? (d->per_lane ? d->per_lane : 1) : sh.per_lane;
     sh.waves = 0;
-    if (waves) {
+    sh.per_lane = 1;
+    if (waves || tpl > 1) {
       KernelRef kv;
       r = resolve_kernel(c, d->rhs, d->alg, d->dtype, pl->save, pl->kahan, pl->k.lanes, &kv,
-                         waves);
-      if (r && d->waves) return r;
+                         waves, tpl);
+      if (r && mine) return r;
       if (!r) {
         pl->k = kv;
         sh.waves = waves;
+        sh.per_lane = tpl;
       }
     }
     if (d->schedule != SDEH_SCHED_AUTO) sh.schedule = d->schedule;
@@ -651,7 +686,7 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
     if (sh.schedule == SDEH_SCHED_PERSISTENT || sh.schedule == SDEH_SCHED_STAGED) {
       if (d->grid) sh.grid = d->grid;
       else if (it == c->tuned.end() || d->block || it->second.schedule != sh.schedule ||
-               it->second.waves != sh.waves)
+               it->second.waves != sh.waves || it->second.per_lane != sh.per_lane)
         sh.grid = c->cus * blocks_per_cu(c, pl->k, sh.block);
       if (sh.grid < 1) sh.grid = 1;
     }
@@ -660,7 +695,7 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
     if (d->block) sh.block = d->block;
   }
   if (sh.schedule == SDEH_SCHED_ONE_LAUNCH) {
-    const int64_t g = (d->n_traj * pl->k.lanes + sh.block - 1) / sh.block;
+    const int64_t g = one_launch_groups(d->n_traj, pl->k, sh.block);
     if (g > 0x7fffffff) return fail(c, SDEH_E_INVALID, "ensemble too large for one launch");
     sh.grid = (int)g;
   }
@@ -668,7 +703,7 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
     // endpoint output, one lane per trajectory, the library's own order; the
     // second launch is the persistent one, so a staged shape falls back to it
     pl->staged = adaptive && pl->save == SAVE_END && !pl->packed && pl->k.lanes == 1 &&
-                 !io->order && d->n_traj > 1 && d->n_traj < (1ll << 32);
+                 pl->k.tpl == 1 && !io->order && d->n_traj > 1 && d->n_traj < (1ll << 32);
     if (!pl->staged) sh.schedule = SDEH_SCHED_PERSISTENT;
     // (a staged solve orders its second launch itself; pl->span stays for the paths
     // that run the shape as persistent: graph capture, the host pipeline)
@@ -880,7 +915,7 @@ int run_host_pipeline(sdeh_ctx* c, Plan& pl, const sdeh_io* io, hipStream_t s) {
     sdeh_kargs& ka = cp.a;
     ka.n = m;
     if (cp.shape.schedule == SDEH_SCHED_ONE_LAUNCH)
-      cp.shape.grid = (int)((m * cp.k.lanes + cp.shape.block - 1) / cp.shape.block);
+      cp.shape.grid = (int)one_launch_groups(m, cp.k, cp.shape.block);
     ka.order = nullptr;
     ka.u0 = offs(base.u0, N, a0, es);
     ka.p = offs(base.p, P, a0, es);
@@ -1132,6 +1167,7 @@ int solve_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, int32_t
   c->stats.schedule = pl.shape.schedule;
   c->stats.lanes = pl.k.lanes;
   c->stats.waves = pl.k.waves;
+  c->stats.per_lane = pl.k.tpl;
   if (do_sync) {
     if (piped) HIPCHK(c, hipStreamSynchronize(c->s_out));
     HIPCHK(c, hipStreamSynchronize(s));
@@ -1247,6 +1283,7 @@ int solve_static_impl(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, 
   c->stats.schedule = pl.shape.schedule;
   c->stats.lanes = pl.k.lanes;
   c->stats.waves = pl.k.waves;
+  c->stats.per_lane = pl.k.tpl;
   if (!sync) return SDEH_OK;
   unsigned long long cnt[4] = {0, 0, 0, 0};
   if (n > 0) {
@@ -1608,6 +1645,7 @@ int sdeh_autotune(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, sdeh
       best_shape.chunk = k.chunk ? k.chunk : 64;
       best_shape.grid = c->stats.grid;
       best_shape.waves = c->stats.waves;
+      best_shape.per_lane = c->stats.per_lane;
       best_stats = c->stats;
     }
   }
@@ -1637,6 +1675,43 @@ int sdeh_autotune(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, sdeh
         best_shape = base;
         best_shape.grid = c->stats.grid;
         best_shape.waves = w;
+        best_stats = c->stats;
+      }
+    }
+  }
+  // And two trajectories per lane, one launch and persistent, where built.
+  if (!d->waves && !d->per_lane) {
+    struct Multi {
+      int sched, block, chunk;
+    };
+    const Multi mc[] = {{SDEH_SCHED_ONE_LAUNCH, 64, 0},   {SDEH_SCHED_ONE_LAUNCH, 128, 0},
+                        {SDEH_SCHED_ONE_LAUNCH, 256, 0},  {SDEH_SCHED_PERSISTENT, 128, 64},
+                        {SDEH_SCHED_PERSISTENT, 256, 64}, {SDEH_SCHED_PERSISTENT, 256, 16}};
+    for (const Multi& k : mc) {
+      sdeh_solve_desc dd = *d;
+      dd.schedule = k.sched;
+      dd.block = k.block;
+      dd.chunk = k.chunk;
+      dd.grid = 0;
+      dd.per_lane = 2;
+      Plan pl;
+      int r = plan_solve(c, &dd, io, &pl);
+      if (r == SDEH_E_UNSUPPORTED) break;  // no such variant for this solve
+      if (r) return r;
+      double ms = 1e300;
+      for (int rep = 0; rep < 2; ++rep) {
+        r = sdeh_solve(c, &dd, io, SDEH_MEM_DEVICE, nullptr, 1);
+        if (r && r != SDEH_E_DTMIN) return r;
+        ms = c->stats.kernel_ms;
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_shape = Shape{};
+        best_shape.schedule = k.sched;
+        best_shape.block = k.block;
+        best_shape.chunk = k.chunk ? k.chunk : 64;
+        best_shape.grid = c->stats.grid;
+        best_shape.per_lane = 2;
         best_stats = c->stats;
       }
     }

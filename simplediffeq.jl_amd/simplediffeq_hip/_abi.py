@@ -78,6 +78,8 @@ class SolveDesc(ctypes.Structure):
         ("save_layout", ctypes.c_int32),
         ("span_order", ctypes.c_int32),
         ("waves", ctypes.c_int32),
+        ("per_lane", ctypes.c_int32),
+        ("reserved2", ctypes.c_int32),
     ]
 
 
@@ -118,6 +120,8 @@ class Stats(ctypes.Structure):
         ("lanes", ctypes.c_int32),
         ("waves", ctypes.c_int32),
         ("steps_needed", ctypes.c_int64),
+        ("per_lane", ctypes.c_int32),
+        ("reserved1", ctypes.c_int32),
     ]
 
 

@@ -1,8 +1,10 @@
-"""The autotuner's register-cap dimension: kernels compiled for 4, 6 and 8 waves per
-SIMD (amdgpu_waves_per_eu, at most 512 / waves VGPRs per lane) run the same
-arithmetic as the default kernel, so every variant gives the same bits, and the
-binary64 ones equal the oracle; the tuner times them after the launch shape and
-later solves use its choice (reported in stats.waves)."""
+"""The autotuner's kernel dimensions: the register cap (kernels compiled for 4, 6
+and 8 waves per SIMD: amdgpu_waves_per_eu, at most 512 / waves VGPRs per lane) and
+trajectories per lane (a variant whose lanes each integrate two trajectories,
+their stages interleaved).  Every variant runs the same per-trajectory arithmetic
+as the default kernel, so it gives the same bits, and the binary64 ones equal the
+oracle; the tuner times them after the launch shape and later solves use its
+choice (reported in stats.waves and stats.per_lane)."""
 import numpy as np
 import pytest
 
@@ -19,11 +21,12 @@ CASES = [(_abi.ALG_ATSIT5, _abi.F64, _abi.PREC_PLAIN), (_abi.ALG_ATSIT5, _abi.F3
 IDS = ["atsit5-f64", "atsit5-f32", "atsit5-f32kahan", "avern7-f64", "avern7-f32"]
 
 
-def _run(ctx, alg, dtype, prec, waves, sched, n=30_001, seed=3):
+def _run(ctx, alg, dtype, prec, waves, sched, n=30_001, seed=3, per_lane=0):
     u0, p, ts = inputs.lorenz_uneven(n, seed=seed)
     ts[1] = np.minimum(ts[1], 5.0)
     d = runtime.make_desc(alg, dtype=dtype, rhs=_abi.RHS_LORENZ, n_traj=n, dt=F32_01,
-                          abstol=1e-7, reltol=1e-6, precision=prec, schedule=sched, waves=waves)
+                          abstol=1e-7, reltol=1e-6, precision=prec, schedule=sched, waves=waves,
+                          per_lane=per_lane)
     g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device="cuda:0")
     ctx.solve(d, g, sync=True)
     return d, g, ctx.stats(), (u0, p, ts)
@@ -55,6 +58,46 @@ def test_register_cap_variants_give_the_same_bits(fresh_ctx, alg, dtype, prec, s
         assert_bit_identical(g, c)
 
 
+MULTI = [(_abi.ALG_ATSIT5, _abi.F64, _abi.PREC_PLAIN), (_abi.ALG_ATSIT5, _abi.F32, _abi.PREC_PLAIN),
+         (_abi.ALG_ATSIT5, _abi.F32, _abi.PREC_KAHAN), (_abi.ALG_AVERN7, _abi.F64, _abi.PREC_PLAIN)]
+
+
+@pytest.mark.parametrize("alg,dtype,prec", MULTI, ids=IDS[:3] + ["avern7-f64"])
+@pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT, _abi.SCHED_STAGED],
+                         ids=["one_launch", "persistent", "staged_shape"])
+@pytest.mark.parametrize("n", [30_001, 2, 1], ids=["n30001", "n2", "n1"])
+def test_two_trajectories_per_lane_give_the_same_bits(fresh_ctx, alg, dtype, prec, sched, n):
+    ctx = fresh_ctx
+    _, ref, _, _ = _run(ctx, alg, dtype, prec, 0, sched, n=n)
+    d, g, st, (u0, p, ts) = _run(ctx, alg, dtype, prec, 0, sched, n=n, per_lane=2)
+    assert st["per_lane"] == 2 and st["schedule"] != _abi.SCHED_STAGED
+    assert_bit_identical(g, ref)
+    if dtype == _abi.F64 and n > 2:
+        c = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts)
+        assert pyoracle.solve(d, c, nthreads=8) == 0
+        assert_bit_identical(g, c)
+
+
+def test_two_per_lane_traced_rhs(fresh_ctx):
+    # a registered (hipRTC) right-hand side gets the variant compiled on demand
+    from simplediffeq_hip import tracer
+    ctx = fresh_ctx
+    src, fn = tracer.trace(lambda u, p, t: [p[0] * (u[1] - u[0]), u[0] * (p[1] - u[2]) - u[1],
+                                            u[0] * u[1] - p[2] * u[2]], 3, 3, "lorenz_pair")
+    h = ctx.compile_rhs(src, fn, 3, 3)
+    n = 5003
+    u0, p = inputs.lorenz_wide(n, seed=1)
+    out = []
+    for pl in (1, 2):
+        d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=h, n_traj=n, tspan=(0.0, 3.0), dt=F32_01,
+                              per_lane=pl, schedule=_abi.SCHED_PERSISTENT)
+        g = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
+        ctx.solve(d, g, sync=True)
+        assert ctx.stats()["per_lane"] == pl
+        out.append(g)
+    assert_bit_identical(out[1], out[0])
+
+
 def test_register_cap_refusals(ctx):
     n = 64
     u0, p = inputs.lorenz_wide(n, seed=1)
@@ -74,6 +117,16 @@ def test_register_cap_refusals(ctx):
                           waves=4)
     with pytest.raises(runtime.SolveError, match="register-cap"):
         ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0g, r, A, backend="torch", device="cuda:0"))
+    # two per lane: adaptive ATsit5/AVern7 endpoint only, not with a register cap
+    d = runtime.make_desc(_abi.ALG_AVERN9, n_traj=n, per_lane=2)
+    with pytest.raises(runtime.SolveError, match="several-trajectories-per-lane"):
+        ctx.solve(d, runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0"))
+    d = runtime.make_desc(_abi.ALG_ATSIT5, n_traj=n, per_lane=2, waves=4)
+    with pytest.raises(runtime.SolveError, match="do not combine"):
+        ctx.solve(d, runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0"))
+    with pytest.raises(runtime.SolveError, match="per_lane must be"):
+        d = runtime.make_desc(_abi.ALG_ATSIT5, n_traj=n, per_lane=3)
+        ctx.solve(d, runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0"))
 
 
 def test_autotuner_reports_and_keeps_its_register_cap(ctx):
@@ -83,10 +136,11 @@ def test_autotuner_reports_and_keeps_its_register_cap(ctx):
                           dt=F32_01, abstol=1e-6, reltol=1e-3)
     g = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
     best = ctx.autotune(d, g)
-    assert best["waves"] in (0, 4, 6, 8)
+    assert best["waves"] in (0, 4, 6, 8) and best["per_lane"] in (1, 2)
     ctx.solve(d, g, sync=True)
     st = ctx.stats()
     assert st["waves"] == best["waves"] and st["schedule"] == best["schedule"]
+    assert st["per_lane"] == best["per_lane"]
     ref = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
     d1 = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=n, tspan=(0.0, 10.0),
                            dt=F32_01, abstol=1e-6, reltol=1e-3, schedule=_abi.SCHED_ONE_LAUNCH)

@@ -48,6 +48,15 @@ def test_each_trajectory_once(ctx, sched, chunk, spans):
         assert (v == 1).all(), (alg, np.unique(v, return_counts=True))
 
 
+@pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT],
+                         ids=["one_launch", "persistent"])
+def test_two_per_lane_once(ctx, sched):
+    d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=N, dt=F32_01,
+                          abstol=1e-6, reltol=1e-4, schedule=sched, per_lane=2)
+    v, st = _counted(ctx, d, _lorenz(d, spans=True))
+    assert st["per_lane"] == 2 and (v == 1).all()
+
+
 def test_caller_order_and_fixed_step(ctx):
     perm = np.random.default_rng(0).permutation(N).astype(np.int64)
     d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=N, tspan=(0.0, 1.0),
