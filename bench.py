@@ -448,7 +448,8 @@ def lorenz_wide(args, ctx, stream, steps, warmup):
                                            MEASURED_TRAFFIC.get((name, n, sched)),
                                            {"attempts_per_launch": st["attempts"]}),
                      "error_vs_avern9_1e-14": ref.error(g),
-                     "launch": {k: st[k] for k in ("block", "grid", "chunk", "schedule")}}
+                     "launch": {k: st[k] for k in ("block", "grid", "chunk", "schedule", "waves",
+                                                   "per_lane")}}
         if name == "f32":
             out[name]["roofline"]["frac_of_78.6_unpacked"] = out[name]["roofline"]["achieved"] / 78.6
         del g
@@ -479,10 +480,10 @@ def headline(args):
     torch.cuda.synchronize()
     ref = _Reference(ctx, u0, p, tspan=ts)
 
-    def bufs(dtype=_abi.F64, sched=0, precision=_abi.PREC_PLAIN, order=None, span=True):
+    def bufs(dtype=_abi.F64, sched=0, precision=_abi.PREC_PLAIN, order=None, span=True, **kw):
         d = runtime.make_desc(_abi.ALG_ATSIT5, dtype=dtype, rhs=_abi.RHS_LORENZ, n_traj=n,
                               dt=f32_01, schedule=sched, precision=precision,
-                              span_order=0 if span else 1, **tol)
+                              span_order=0 if span else 1, **tol, **kw)
         g = runtime.EnsembleBuffers(d, 3, u0d, pd, tspan=tsd, order=order, backend="torch",
                                     device=dev)
         return d, g
@@ -509,8 +510,20 @@ def headline(args):
     att = g64.numpy("naccept").astype(np.int64) + g64.numpy("nreject")
     err64 = ref.error(g64)
     sched64 = st64["schedule"]
-    launch64 = {k: st64[k] for k in ("block", "grid", "chunk", "schedule")}
+    launch64 = {k: st64[k] for k in ("block", "grid", "chunk", "schedule", "waves", "per_lane")}
     del g64
+    # the autotuner's kernel variants on the same solve (the tuner's choice is above)
+    variants = {}
+    for name, kw in (("waves_4", dict(waves=4)), ("waves_6", dict(waves=6)),
+                     ("waves_8", dict(waves=8)),
+                     ("two_per_lane_persistent", dict(per_lane=2, sched=2)),
+                     ("two_per_lane_one_launch", dict(per_lane=2, sched=1))):
+        kw.setdefault("sched", sched64 if sched64 != 3 or "waves" in kw else 2)
+        d, g = bufs(**kw)
+        stv = solve_once(d, g)
+        variants[name] = {"kernel_ms": time_of(d, g, 5), "schedule": stv["schedule"],
+                          "grid": stv["grid"]}
+        del g
 
     # 4. binary32 (plain) in its autotuned shape, for its error and rate
     d32, g32 = bufs(dtype=_abi.F32)
@@ -523,7 +536,8 @@ def headline(args):
            "roofline": _roofline(algorithmic_flops(_abi.ALG_ATSIT5, 3, LORENZ_FLOPS,
                                                    st32["attempts"], st32["accepts"], n),
                                  ms32, PEAK_TFLOPS["f32"], _kernel_name("f32", st32["schedule"])),
-           "launch": {k: st32[k] for k in ("block", "grid", "chunk", "schedule")}}
+           "launch": {k: st32[k] for k in ("block", "grid", "chunk", "schedule", "waves",
+                                           "per_lane")}}
     del g32
 
     # 5. learned: planning on the device, binned binary32 Kahan solves, baselines
@@ -617,6 +631,7 @@ def headline(args):
                                "algorithmic_bytes_gbs": n * (3 * 8 * 3 + 16 + 8 + 12) /
                                (ms64 * 1e-3) / 1e9}),
         "error_vs_avern9_1e-14": err64,
+        "kernel_variants_f64_ms": variants,
         "f32": f32,
         "learned": learned,
     }

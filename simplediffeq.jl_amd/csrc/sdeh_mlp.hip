@@ -45,7 +45,12 @@ constexpr int H2P = NH + 1;
 
 __device__ __forceinline__ float tanh_fast(float x) {
   const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);  // e^{2x}
-  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+  return fmaf(-2.0f, __builtin_amdgcn_rcpf(e + 1.0f), 1.0f);
+}
+
+// tanh(x) from a = 2 x log2(e), the scaling already applied by the caller.
+__device__ __forceinline__ float tanh_prescaled(float a) {
+  return fmaf(-2.0f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(a) + 1.0f), 1.0f);
 }
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -60,32 +65,6 @@ struct XFromBuffer {  // x [n][6] FP32
   __device__ void operator()(int64_t smp, bool valid, float (&xi)[NIN]) const {
 #pragma unroll
     for (int j = 0; j < NIN; ++j) xi[j] = valid ? x[smp * NIN + j] : 0.f;
-  }
-};
-
-// The Lorenz cost model's features from binary64 ensemble inputs (component-major
-// p[3][n], u0[3][n], tspan[2][n]), each computed in binary64 in this order and then
-// rounded to FP32: (p_c - c[2c]) / c[2c+1] for c = 0, 1, 2; (log(tf - t0) - c[6]) /
-// c[7]; sqrt((u0_0^2 + u0_1^2) + u0_2^2) / sqrt(3) * 2 - 1; u0_2.
-struct XLorenz {
-  const double* __restrict__ p;
-  const double* __restrict__ u0;
-  const double* __restrict__ ts;
-  int64_t n;
-  double c[8];
-  __device__ void operator()(int64_t smp, bool valid, float (&xi)[NIN]) const {
-    if (!valid) {
-#pragma unroll
-      for (int j = 0; j < NIN; ++j) xi[j] = 0.f;
-      return;
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) xi[k] = (float)((p[k * n + smp] - c[2 * k]) / c[2 * k + 1]);
-    xi[3] = (float)((log(ts[n + smp] - ts[smp]) - c[6]) / c[7]);
-    const double a = u0[smp], b = u0[n + smp], z = u0[2 * n + smp];
-    const double r2 = (a * a + b * b) + z * z;
-    xi[4] = (float)(sqrt(r2) / 1.7320508075688772 * 2.0 - 1.0);
-    xi[5] = (float)z;
   }
 };
 
@@ -116,6 +95,145 @@ struct YPlan {
     }
   }
 };
+
+// ---- the binning step on the f16 matrix cores -------------------------------
+// sdeh_mlp_plan_lorenz's forward pass: the same network with its weights and
+// activations rounded to f16 and every product accumulated in FP32 on
+// v_mfma_f32_32x32x8_f16 (K = 8 per instruction).  Each layer is computed
+// transposed, D^T = W X^T (M = units, N = 32 samples), because the MFMA output
+// layout then hands the next layer its B operand without moving data: lane l holds
+// sample l % 32 and, in register r of output tile t, unit 32 t + 8 (r / 4) +
+// 4 (l / 32) + r % 4, which is exactly the B operand (rows k = 8 m + 4 (l / 32) + i,
+// column l % 32) of the next layer's K step m = 4 t + r / 4, i = r % 4.  Biases are
+// one more K step against a constant 1 input (layer 1: feature column 6), and layer
+// 3 (two outputs) is rows 0 and 1 of a padded 32-row tile.  The six features of a
+// sample are split between its two lanes: lane l < 32 computes the parameter and
+// horizon features (K rows 0-3), lane l + 32 the initial-state ones (rows 4-5), the
+// bias input 1 (row 6) and a zero (row 7).  Activations use the hardware exp and
+// reciprocal (tanh = 1 - 2 / (e^2x + 1), absolute error about 1e-7 before the f16
+// rounding), with the factor 2 log2(e) of the exponent folded into the layer 1 and
+// layer 2 weights and biases before they are rounded to f16, so that each activation
+// is exp2, add, reciprocal and fma.  The predictions only order and seed the solves;
+// sdeh_mlp_forward keeps the FP32 forward pass.
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef __fp16 fp16x2 __attribute__((ext_vector_type(2)));
+
+// Four floats as f16 (rounded toward zero, two per v_cvt_pkrtz_f16_f32).
+__device__ __forceinline__ half4 pack4(float a, float b, float c, float d) {
+  const fp16x2 lo = __builtin_amdgcn_cvt_pkrtz(a, b), hi = __builtin_amdgcn_cvt_pkrtz(c, d);
+  half4 r;
+  r[0] = (_Float16)lo[0];
+  r[1] = (_Float16)lo[1];
+  r[2] = (_Float16)hi[0];
+  r[3] = (_Float16)hi[1];
+  return r;
+}
+
+// Features of sample smp in FP32 for lane half q (see above): q = 0: (p_c - f[2c]) /
+// f[2c+1] for c = 0, 1, 2 and (log(tf - t0) - f[6]) / f[7]; q = 1: |u0| / sqrt(3) *
+// 2 - 1, u0_2, 1, 0.
+struct XLorenzPlan {
+  const double* __restrict__ p;
+  const double* __restrict__ u0;
+  const double* __restrict__ ts;
+  int64_t n;
+  float off[4], inv[4];
+  __device__ half4 operator()(int64_t smp, bool valid, int q) const {
+    if (!valid) return pack4(0.f, 0.f, q ? 1.f : 0.f, 0.f);
+    if (q == 0) {
+      const float x0 = ((float)p[smp] - off[0]) * inv[0];
+      const float x1 = ((float)p[n + smp] - off[1]) * inv[1];
+      const float x2 = ((float)p[2 * n + smp] - off[2]) * inv[2];
+      const float x3 = (__logf((float)(ts[n + smp] - ts[smp])) - off[3]) * inv[3];
+      return pack4(x0, x1, x2, x3);
+    }
+    const float a = (float)u0[smp], b = (float)u0[n + smp], z = (float)u0[2 * n + smp];
+    const float r = __builtin_sqrtf((a * a + b * b) + z * z);
+    return pack4(r * 1.1547005383792515f - 1.f, z, 1.f, 0.f);
+  }
+};
+
+template <class Sink>
+__global__ void __launch_bounds__(256) mlp_plan_f16(const float* __restrict__ params,
+                                                    XLorenzPlan src, Sink sink, int64_t n) {
+  __shared__ float sw[NPARAM];
+  load_params(sw, params);
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = l & 31, q = l >> 5;
+  // A operands (weights, fixed per kernel): lane l holds row li of the tile, K columns
+  // 4q .. 4q + 3 of each K step; step 8 of layers 2 and 3 carries the bias (column 0).
+  // Layers 1 and 2 are scaled by c = 2 log2(e): their outputs are the exponents of
+  // tanh_prescaled.
+  constexpr float c = 2.8853900817779268f;
+  half4 w1[2], w2[2][9], w3[9];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int u = 32 * t + li;
+    float v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = 4 * q + i;
+      v[i] = c * (k < NIN ? sw[OFF_W1 + u * NIN + k] : (k == NIN ? sw[OFF_B1 + u] : 0.f));
+    }
+    w1[t] = pack4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const float* w = sw + OFF_W2 + u * NH + 8 * m + 4 * q;
+      w2[t][m] = pack4(c * w[0], c * w[1], c * w[2], c * w[3]);
+    }
+    w2[t][8] = pack4(q == 0 ? c * sw[OFF_B2 + u] : 0.f, 0.f, 0.f, 0.f);
+  }
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int k = 8 * m + 4 * q;
+    const float* w = sw + OFF_W3 + (li < NOUT ? li : 0) * NH + k;
+    w3[m] = li < NOUT ? pack4(w[0], w[1], w[2], w[3]) : pack4(0.f, 0.f, 0.f, 0.f);
+  }
+  w3[8] = pack4(li < NOUT && q == 0 ? sw[OFF_B3 + (li < NOUT ? li : 0)] : 0.f, 0.f, 0.f, 0.f);
+  const half4 one = pack4(q == 0 ? 1.f : 0.f, 0.f, 0.f, 0.f);  // the bias step's input row
+  const floatx16 zero = {};
+  const int64_t tiles = (n + 31) / 32;
+  for (int64_t tile = (int64_t)blockIdx.x * 4 + wv; tile < tiles;
+       tile += (int64_t)gridDim.x * 4) {
+    const int64_t smp = tile * 32 + li;
+    const bool valid = smp < n;
+    const half4 x = src(smp, valid, q);
+    // layer 1: two 32-unit tiles
+    floatx16 d1[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) d1[t] = __builtin_amdgcn_mfma_f32_32x32x8f16(w1[t], x, zero, 0, 0, 0);
+    half4 h1[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const floatx16& d = d1[m >> 2];
+      const int r = 4 * (m & 3);
+      h1[m] = pack4(tanh_prescaled(d[r]), tanh_prescaled(d[r + 1]), tanh_prescaled(d[r + 2]),
+                    tanh_prescaled(d[r + 3]));
+    }
+    // layer 2: two output tiles, eight K steps and the bias step each
+    floatx16 d2[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      floatx16 acc = __builtin_amdgcn_mfma_f32_32x32x8f16(w2[t][8], one, zero, 0, 0, 0);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) acc = __builtin_amdgcn_mfma_f32_32x32x8f16(w2[t][m], h1[m], acc, 0, 0, 0);
+      d2[t] = acc;
+    }
+    half4 h2[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const floatx16& d = d2[m >> 2];
+      const int r = 4 * (m & 3);
+      h2[m] = pack4(tanh_prescaled(d[r]), tanh_prescaled(d[r + 1]), tanh_prescaled(d[r + 2]),
+                    tanh_prescaled(d[r + 3]));
+    }
+    // layer 3: output rows 0 and 1 land in registers 0 and 1 of lanes 0..31
+    floatx16 d3 = __builtin_amdgcn_mfma_f32_32x32x8f16(w3[8], one, zero, 0, 0, 0);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) d3 = __builtin_amdgcn_mfma_f32_32x32x8f16(w3[m], h2[m], d3, 0, 0, 0);
+    if (q == 0 && valid) sink(smp, d3[0], d3[1]);
+  }
+}
 
 template <class Src, class Sink>
 __global__ void __launch_bounds__(256) mlp_forward_mfma(const float* __restrict__ params,
@@ -422,25 +540,28 @@ __global__ void mlp_adam(float* p, const float* __restrict__ g, float* m, float*
 
 }  // namespace
 
-// Resident blocks of a forward kernel on the current device (LDS bounds it: 52 KB
-// per block), the grid of a forward launch: the blocks then loop over tiles with no
-// partial last round of blocks.
+// Resident blocks of a kernel on the current device, the grid of a forward launch:
+// the blocks then loop over tiles with no partial last round of blocks.
+static int64_t resident_grid(const void* kernel, int64_t n) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
+      cus <= 0 || per_cu <= 0)
+    return -1;
+  const int64_t resident = (int64_t)cus * per_cu;
+  const int64_t want = ((n + 31) / 32 + 3) / 4;  // four waves of 32-sample tiles per block
+  return want < resident ? want : resident;
+}
+
 template <class Src, class Sink>
 static int64_t forward_grid(int64_t n) {
   // per process (initialised once, thread-safe); every MI355X has the same count
-  static const int resident = [] {
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&mlp_forward_mfma<Src, Sink>), 256, 0) !=
-            hipSuccess)
-      return 0;
-    return cus > 0 && per_cu > 0 ? cus * per_cu : 0;
-  }();
-  if (resident <= 0) return -1;
-  const int64_t want = ((n + 31) / 32 + 3) / 4;  // four waves of 32-sample tiles per block
-  return want < resident ? want : resident;
+  static const int64_t full = resident_grid(
+      reinterpret_cast<const void*>(&mlp_forward_mfma<Src, Sink>), int64_t(1) << 40);
+  if (full <= 0) return -1;
+  const int64_t want = ((n + 31) / 32 + 3) / 4;
+  return want < full ? want : full;
 }
 
 extern "C" SDEH_HIDDEN int sdeh_mlp_forward_impl(const float* params, const float* x, float* y,
@@ -461,11 +582,18 @@ extern "C" SDEH_HIDDEN int sdeh_mlp_plan_lorenz_impl(const float* params, const 
                                                      float m1, float s0, float s1, float* key,
                                                      double* dt0, float* pred, hipStream_t s) {
   if (n == 0) return 0;
-  const int64_t g = forward_grid<XLorenz, YPlan>(n);
-  if (g <= 0) return -2;
-  XLorenz src{p, u0, tspan, n, {c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7]}};
-  hipLaunchKernelGGL((mlp_forward_mfma<XLorenz, YPlan>), dim3((unsigned)g), dim3(256), 0, s,
-                     params, src, YPlan{key, dt0, pred, m0, m1, s0, s1}, n);
+  static const int64_t full =
+      resident_grid(reinterpret_cast<const void*>(&mlp_plan_f16<YPlan>), int64_t(1) << 40);
+  if (full <= 0) return -2;
+  const int64_t want = ((n + 31) / 32 + 3) / 4;
+  const int64_t g = want < full ? want : full;
+  XLorenzPlan src{p, u0, tspan, n, {}, {}};
+  for (int k = 0; k < 4; ++k) {
+    src.off[k] = (float)c[2 * k];
+    src.inv[k] = (float)(1.0 / c[2 * k + 1]);
+  }
+  hipLaunchKernelGGL((mlp_plan_f16<YPlan>), dim3((unsigned)g), dim3(256), 0, s, params, src,
+                     YPlan{key, dt0, pred, m0, m1, s0, s1}, n);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -165,8 +165,9 @@ __global__ void write_i64(int64_t* p, int64_t v) {
 // 16 buckets per octave over 64 octaves (2^-30 .. 2^34; spans beyond clamp to the
 // end buckets; spans <= 0 and NaN in bucket 0, last).  Each of SPAN_BLOCKS blocks
 // takes a contiguous range of trajectories: span_hist counts its buckets in LDS,
-// span_scan turns the counts into each (bucket, block)'s first slot (buckets in
-// decreasing order, blocks in index order), and span_scatter places the block's
+// three scan passes (span_tile_sums, span_scan_tiles, span_tile_offsets) turn the
+// counts into each (bucket, block)'s first slot (buckets in decreasing order, blocks
+// in index order), and span_scatter places the block's
 // trajectories, ranking them with LDS atomics.  No global atomics: the few buckets
 // a narrow span distribution fills would serialise them.  Within a (bucket, block)
 // the order is whatever the LDS atomics produce; no result depends on it.  The same
@@ -211,66 +212,104 @@ __device__ __forceinline__ void span_range(int64_t n, int64_t* lo, int64_t* hi) 
   *hi = *lo + per < n ? *lo + per : n;
 }
 
+// span_hist and span_scatter run SPAN_THREADS threads per block (16 waves per CU, so
+// that key loads overlap the LDS atomics of other waves), each thread loading
+// SPAN_BATCH keys before it ranks them.
+constexpr int SPAN_THREADS = 1024;
+constexpr int SPAN_BATCH = 4;
+
 // bh[block][bucket] = trajectories of the block's range in the bucket.
-__global__ void __launch_bounds__(256) span_hist(BucketSrc src, unsigned* bh) {
+__global__ void __launch_bounds__(SPAN_THREADS) span_hist(BucketSrc src, unsigned* bh) {
   __shared__ unsigned h[SPAN_BUCKETS];
   for (int k = threadIdx.x; k < SPAN_BUCKETS; k += blockDim.x) h[k] = 0u;
   __syncthreads();
   int64_t lo, hi;
   span_range(src.n, &lo, &hi);
-  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(h + src(i), 1u);
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (int64_t)SPAN_BATCH * blockDim.x) {
+    int b[SPAN_BATCH];
+#pragma unroll
+    for (int j = 0; j < SPAN_BATCH; ++j) {
+      const int64_t i = i0 + (int64_t)j * blockDim.x;
+      b[j] = i < hi ? src(i) : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < SPAN_BATCH; ++j)
+      if (b[j] >= 0) atomicAdd(h + b[j], 1u);
+  }
   __syncthreads();
   for (int k = threadIdx.x; k < SPAN_BUCKETS; k += blockDim.x)
     bh[(size_t)blockIdx.x * SPAN_BUCKETS + k] = h[k];
 }
 
-// One block of SPAN_BUCKETS threads, thread t for bucket SPAN_BUCKETS-1-t: bh ->
-// first slot of each (block, bucket).
-__global__ void __launch_bounds__(1024) span_scan(unsigned* bh) {
-  __shared__ unsigned long long part[SPAN_BUCKETS];
-  const int t = threadIdx.x;
-  const int k = SPAN_BUCKETS - 1 - t;
-  // eight loads in flight per step (the counts of one bucket are a column of bh)
-  static_assert(SPAN_BLOCKS % 8 == 0, "blocks in steps of eight");
-  unsigned long long tot = 0;
-  for (int b = 0; b < SPAN_BLOCKS; b += 8) {
-    unsigned v[8];
+// The first slot of each (block, bucket): an exclusive scan of the counts taken in
+// the order (bucket SPAN_BUCKETS-1 .. 0, then block 0 .. SPAN_BLOCKS-1), in three
+// passes over tiles of SPAN_TILE entries of that order: the tile sums, their scan in
+// one block, then each tile scanned and offset in place.  Entry e of the order is
+// bucket SPAN_BUCKETS-1 - e / SPAN_BLOCKS of block e % SPAN_BLOCKS.
+constexpr int SPAN_TILE = 1024;  // 256 threads x 4 entries
+constexpr int SPAN_TILES = SPAN_BUCKETS * SPAN_BLOCKS / SPAN_TILE;
+static_assert(SPAN_BUCKETS * SPAN_BLOCKS % SPAN_TILE == 0, "whole tiles");
+static_assert(SPAN_TILES == 256, "one 256-thread block scans the tile sums");
+
+__device__ __forceinline__ size_t span_entry(int64_t e) {
+  const int64_t k = SPAN_BUCKETS - 1 - e / SPAN_BLOCKS, b = e % SPAN_BLOCKS;
+  return (size_t)b * SPAN_BUCKETS + (size_t)k;
+}
+
+__global__ void __launch_bounds__(256) span_tile_sums(const unsigned* bh, unsigned* tsum) {
+  const int64_t e0 = (int64_t)blockIdx.x * SPAN_TILE + 4 * threadIdx.x;
+  int64_t v = 0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = bh[(size_t)(b + j) * SPAN_BUCKETS + k];
+  for (int j = 0; j < 4; ++j) v += bh[span_entry(e0 + j)];
+  int64_t total = 0;
+  (void)block_exclusive_scan(v, &total);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = (unsigned)total;
+}
+
+__global__ void __launch_bounds__(256) span_scan_tiles(unsigned* tsum) {
+  const int64_t v = tsum[threadIdx.x];
+  int64_t total = 0;
+  const int64_t ex = block_exclusive_scan(v, &total);
+  tsum[threadIdx.x] = (unsigned)ex;
+}
+
+__global__ void __launch_bounds__(256) span_tile_offsets(unsigned* bh, const unsigned* tsum) {
+  const int64_t e0 = (int64_t)blockIdx.x * SPAN_TILE + 4 * threadIdx.x;
+  unsigned c[4];
+  int64_t v = 0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) tot += v[j];
+  for (int j = 0; j < 4; ++j) {
+    c[j] = bh[span_entry(e0 + j)];
+    v += c[j];
   }
-  part[t] = tot;
-  __syncthreads();
-  for (int off = 1; off < SPAN_BUCKETS; off <<= 1) {
-    const unsigned long long x = t >= off ? part[t - off] : 0ull;
-    __syncthreads();
-    part[t] += x;
-    __syncthreads();
-  }
-  unsigned long long run = part[t] - tot;
-  for (int b = 0; b < SPAN_BLOCKS; b += 8) {
-    unsigned v[8];
+  int64_t total = 0;
+  unsigned o = tsum[blockIdx.x] + (unsigned)block_exclusive_scan(v, &total);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = bh[(size_t)(b + j) * SPAN_BUCKETS + k];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      bh[(size_t)(b + j) * SPAN_BUCKETS + k] = (unsigned)run;
-      run += v[j];
-    }
+  for (int j = 0; j < 4; ++j) {
+    bh[span_entry(e0 + j)] = o;
+    o += c[j];
   }
 }
 
-__global__ void __launch_bounds__(256) span_scatter(BucketSrc src, const unsigned* bh,
-                                                    int64_t* order) {
+__global__ void __launch_bounds__(SPAN_THREADS) span_scatter(BucketSrc src, const unsigned* bh,
+                                                             int64_t* order) {
   __shared__ unsigned cur[SPAN_BUCKETS];
   for (int k = threadIdx.x; k < SPAN_BUCKETS; k += blockDim.x)
     cur[k] = bh[(size_t)blockIdx.x * SPAN_BUCKETS + k];
   __syncthreads();
   int64_t lo, hi;
   span_range(src.n, &lo, &hi);
-  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
-    order[atomicAdd(cur + src(i), 1u)] = i;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (int64_t)SPAN_BATCH * blockDim.x) {
+    int b[SPAN_BATCH];
+#pragma unroll
+    for (int j = 0; j < SPAN_BATCH; ++j) {
+      const int64_t i = i0 + (int64_t)j * blockDim.x;
+      b[j] = i < hi ? src(i) : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < SPAN_BATCH; ++j)
+      if (b[j] >= 0) order[atomicAdd(cur + b[j], 1u)] = i0 + (int64_t)j * blockDim.x;
+  }
 }
 
 inline int grid_for(int64_t n) {
@@ -341,13 +380,16 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_pack_offse
   return hipGetLastError();
 }
 
-// order[0..n) = trajectories by decreasing weight (see span_hist).  bh: scratch
-// of SPAN_BLOCKS x SPAN_BUCKETS uint32.  n must fit in 32 bits (the slot counters).
+// order[0..n) = trajectories by decreasing weight (see span_hist).  bh: scratch of
+// sdeh_util_span_scratch_bytes().  n must fit in 32 bits (the slot counters).
 static hipError_t bucket_order(BucketSrc src, unsigned* bh, int64_t* order, hipStream_t s) {
   if (src.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(span_hist, dim3(SPAN_BLOCKS), dim3(256), 0, s, src, bh);
-  hipLaunchKernelGGL(span_scan, dim3(1), dim3(SPAN_BUCKETS), 0, s, bh);
-  hipLaunchKernelGGL(span_scatter, dim3(SPAN_BLOCKS), dim3(256), 0, s, src, bh, order);
+  unsigned* tsum = bh + (size_t)SPAN_BLOCKS * SPAN_BUCKETS;
+  hipLaunchKernelGGL(span_hist, dim3(SPAN_BLOCKS), dim3(SPAN_THREADS), 0, s, src, bh);
+  hipLaunchKernelGGL(span_tile_sums, dim3(SPAN_TILES), dim3(256), 0, s, bh, tsum);
+  hipLaunchKernelGGL(span_scan_tiles, dim3(1), dim3(SPAN_TILES), 0, s, tsum);
+  hipLaunchKernelGGL(span_tile_offsets, dim3(SPAN_TILES), dim3(256), 0, s, bh, tsum);
+  hipLaunchKernelGGL(span_scatter, dim3(SPAN_BLOCKS), dim3(SPAN_THREADS), 0, s, src, bh, order);
   return hipGetLastError();
 }
 
@@ -371,8 +413,9 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t sdeh_util_logkey_ord
   return bucket_order(BucketSrc{nullptr, nullptr, lkey, n}, bh, order, s);
 }
 
+// Scratch of the bucket sorts: the per-block counts and the tile sums.
 extern "C" __attribute__((visibility("hidden"))) size_t sdeh_util_span_scratch_bytes() {
-  return (size_t)SPAN_BLOCKS * SPAN_BUCKETS * sizeof(unsigned);
+  return ((size_t)SPAN_BLOCKS * SPAN_BUCKETS + SPAN_TILES) * sizeof(unsigned);
 }
 
 // The device address of `scratch` on the current device, looked up once per device

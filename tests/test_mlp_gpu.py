@@ -65,36 +65,43 @@ def plan_buckets(key):
     return b
 
 
-def test_fused_lorenz_plan_equals_separate_steps(ctx):
-    """sdeh_mlp_plan_lorenz (features, forward, de-normalisation, sort key, dt0 in
-    one launch) against the same step built from torch features, the plain forward
-    pass and torch operations: predictions and dt0 bit for bit, and the library's
-    hand-out order (sdeh_plan_order) against torch's stable argsort of the keys on
-    the order's 1/16-octave buckets; the features against the numpy definition
-    through the forward pass."""
+def test_fused_lorenz_plan_against_reference(ctx):
+    """sdeh_mlp_plan_lorenz (features, forward pass on the f16 matrix cores with FP32
+    accumulation, de-normalisation, sort key, dt0 in one launch) against the float64
+    reference network on the numpy features: predictions within the f16 rounding of
+    weights and activations; key = -pred_0 and dt0 = exp(pred_1) exactly as the
+    kernel's own FP32 arithmetic gives them; and the library's hand-out order
+    (sdeh_plan_order) against a stable argsort of the keys on its 1/16-octave
+    buckets."""
+    import json
     torch = pytest.importorskip("torch")
     from simplediffeq_hip import inputs, runtime
     n = 100_003  # not a multiple of a block's 128 samples
     u0, p, ts = inputs.lorenz_costmodel(n, seed=9)
-    cm = costmodel.CostModel(ctx, params=costmodel.init_params(5))
+    w = costmodel.init_params(5)
+    cm = costmodel.CostModel(ctx, params=w)
     cm.y_mean = np.array([3.1, -2.2], dtype=np.float32)
     cm.y_std = np.array([0.7, 1.3], dtype=np.float32)
     dev = torch.device("cuda:0")
     pd, u0d, tsd = (torch.as_tensor(a, device=dev) for a in (p, u0, ts))
     stream = torch.cuda.Stream()
-    with torch.cuda.stream(stream):
-        xd = costmodel.features_device(pd, u0d, tsd)
-    pred_u, order_u, dt0_u = cm.plan_device(xd, stream)
     pred_f, order_f, dt0_f = cm.plan_lorenz(pd, u0d, tsd, stream)
     torch.cuda.synchronize()
-    assert torch.equal(pred_f, pred_u)
-    assert torch.equal(dt0_f, dt0_u)
-    b = plan_buckets(-pred_f[:, 0].cpu().numpy())
-    of, ou = order_f.cpu().numpy(), order_u.cpu().numpy()
+    pf = pred_f.cpu().numpy()
+    ref, _ = mlp_ref.forward(w.astype(np.float64), costmodel.features(p, u0, ts).astype(np.float64))
+    ref = ref * cm.y_std + cm.y_mean
+    dev_abs = np.abs(pf - ref)
+    print(json.dumps({"plan_vs_float64": {"max": float(dev_abs.max()),
+                                          "median": float(np.median(dev_abs)),
+                                          "p99": float(np.percentile(dev_abs, 99))}}))
+    assert np.median(dev_abs) < 5e-3 and dev_abs.max() < 5e-2
+    # the kernel's own arithmetic: dt0 = exp(pred_1) in FP32, widened
+    assert np.array_equal(dt0_f.cpu().numpy(), np.exp(pf[:, 1]).astype(np.float64)) or \
+        np.max(np.abs(dt0_f.cpu().numpy() / np.exp(pf[:, 1].astype(np.float64)) - 1)) < 1e-6
+    b = plan_buckets(-pf[:, 0])
+    of = order_f.cpu().numpy()
     assert np.array_equal(np.sort(of), np.arange(n))
-    assert np.array_equal(b[of], b[ou])
-    # the numpy feature definition gives the same predictions (host features)
-    assert np.array_equal(cm.predict(costmodel.features(p, u0, ts)), pred_f.cpu().numpy())
+    assert np.array_equal(b[of], b[np.argsort(-b, kind="stable")])
     # validation: a zero feature scale, and an empty ensemble
     key = torch.empty(4, dtype=torch.float32, device=dev)
     dt0 = torch.empty(4, dtype=torch.float64, device=dev)
