@@ -55,14 +55,24 @@ CPU_SHARE = 16
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per solve of the binary64 headline at 2^20 trajectories, by schedule,
-# from two rocprofv3 --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors
-# from a 256 MiB device copy in the same run).  Persistent: 49,405 KiB read + 82,225
-# KiB written.  Staged (first pass, key sort, resumed pass): 234,824 KiB read +
-# 389,626 KiB written, mostly the parked records and the resumed pass's endpoint
-# stores in hand-out order (partial lines).  Counters cannot be read from inside a
-# timed run, so each figure applies only to the configuration it was measured on.
-MEASURED_TRAFFIC = {("f64", 1 << 20, 2): 134.79e6, ("f64", 1 << 20, 3): 639.44e6}
+# HBM bytes per integrator launch, from two rocprofv3 --pmc passes (FETCH_SIZE x
+# 2.000 + WRITE_SIZE x 1.000, factors from a 256 MiB device copy in the same run).
+# Counters cannot be read from inside a timed run, so each figure applies only to the
+# configuration it was measured on.
+#   The wide ensemble, binary64, 2^20 trajectories: persistent 49,405 KiB read +
+#   82,225 KiB written; staged (first pass, key sort, resumed pass) 234,824 KiB read
+#   + 389,626 KiB written, mostly the parked records and the resumed pass's endpoint
+#   stores in hand-out order (partial lines).
+#   The default line, binary64, 2^22 trajectories with their own t_end, persistent
+#   (256 x 1,280, chunk 64), longest span first: 3,816,389 KiB read + 929,960 KiB
+#   written; each trajectory's eight input loads land on eight separate lines, since
+#   the span order hands neighbouring lanes trajectories about a hundred indices apart.
+MEASURED_TRAFFIC = {("f64", 1 << 20, 2): 134.79e6, ("f64", 1 << 20, 3): 639.44e6,
+                    ("f64", 1 << 22, 2, "costmodel"): 4.860e9}
+# Algorithmic bytes per trajectory of an endpoint solve with its own tspan (binary64
+# Lorenz): reads u0, p (48), tspan (16), the hand-out slot (8); writes u_end (24),
+# t_end (8), retcode, naccept, nreject (12).
+ALG_BYTES_TSPAN = 48 + 16 + 8 + 24 + 8 + 12
 
 # Algorithmic FLOPs (per component per attempt, RHS calls per attempt) by solver.
 STAGE_FLOPS = {0: 48, 1: 67, 2: 83, 3: 117, 4: 167, 5: 211, 6: 13, 7: 2}
@@ -393,6 +403,11 @@ class _Reference:
                 "trajectories": int(self.ok.sum())}
 
 
+def _progress(msg):
+    """One line on stderr as each phase of a run ends (the JSON line is stdout's only)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def _roofline(alg_flops, kernel_ms, peak, kernel, traffic=None, extra=None):
     ach = alg_flops / (kernel_ms * 1e-3) / 1e12
     r = {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
@@ -474,11 +489,14 @@ def headline(args):
     # 1. the cost model: step logs of the plain solver and training, on the GPU
     cm, hist, t_train, t_logs = costmodel.train_lorenz(ctx, n=min(N_TRAIN, n), seed=40,
                                                        steps=args.train_steps, **tol)
+    _progress("cost model trained (%.2f s)" % t_train)
     # 2. the evaluation ensemble, resident in HBM
     u0, p, ts = inputs.lorenz_costmodel(n, seed=4)
     u0d, pd, tsd = (torch.as_tensor(a, device=dev) for a in (u0, p, ts))
     torch.cuda.synchronize()
     ref = _Reference(ctx, u0, p, tspan=ts)
+
+    _progress("binary64 AVern9 1e-14 reference solved")
 
     def bufs(dtype=_abi.F64, sched=0, precision=_abi.PREC_PLAIN, order=None, span=True, **kw):
         d = runtime.make_desc(_abi.ALG_ATSIT5, dtype=dtype, rhs=_abi.RHS_LORENZ, n_traj=n,
@@ -500,8 +518,12 @@ def headline(args):
                        reps)
 
     # 3. value: binary64 in the autotuned shape, the driver's timed region
-    d64, g64 = bufs(sched=args.schedule)
-    if args.tune:
+    shape = {}
+    if args.shape:
+        sh = [int(v) for v in args.shape.split(",")]
+        shape = dict(sched=sh[0], block=sh[1], grid=sh[2], chunk=sh[3])
+    d64, g64 = bufs(**(shape or dict(sched=args.schedule)))
+    if args.tune and not shape:
         ctx.autotune(d64, g64)
     st64 = solve_once(d64, g64)
     ms64, wall64 = _timed_steps(ctx, d64, g64, stream, steps, warmup)
@@ -512,6 +534,7 @@ def headline(args):
     sched64 = st64["schedule"]
     launch64 = {k: st64[k] for k in ("block", "grid", "chunk", "schedule", "waves", "per_lane")}
     del g64
+    _progress("binary64 timed: %.3f ms per solve, launch %s" % (ms64, launch64))
     # the autotuner's kernel variants on the same solve (the tuner's choice is above)
     variants = {}
     for name, kw in (("waves_4", dict(waves=4)), ("waves_6", dict(waves=6)),
@@ -524,6 +547,7 @@ def headline(args):
         variants[name] = {"kernel_ms": time_of(d, g, 5), "schedule": stv["schedule"],
                           "grid": stv["grid"]}
         del g
+    _progress("kernel variants timed")
 
     # 4. binary32 (plain) in its autotuned shape, for its error and rate
     d32, g32 = bufs(dtype=_abi.F32)
@@ -539,6 +563,7 @@ def headline(args):
            "launch": {k: st32[k] for k in ("block", "grid", "chunk", "schedule", "waves",
                                            "per_lane")}}
     del g32
+    _progress("binary32 timed")
 
     # 5. learned: planning on the device, binned binary32 Kahan solves, baselines
     def plan(model):
@@ -553,17 +578,20 @@ def headline(args):
     torch.cuda.synchronize()
     reps = max(5, steps // 2)
     runs = {}
-    for name, dtype, sched, prec, od, span in (
+    # a binned run plans right before its solves, as a caller does: the context takes
+    # the order sdeh_plan_order wrote last without the permutation check
+    for name, dtype, sched, prec, model, span in (
             ("f64_one_launch_index_order", _abi.F64, 1, _abi.PREC_PLAIN, None, False),
-            ("f32k_persistent_binned", _abi.F32, 2, _abi.PREC_KAHAN, order, False),
-            ("f32k_one_launch_binned", _abi.F32, 1, _abi.PREC_KAHAN, order, False),
-            ("f32k_one_launch_binned_untrained", _abi.F32, 1, _abi.PREC_KAHAN, order_untrained,
+            ("f32k_persistent_binned", _abi.F32, 2, _abi.PREC_KAHAN, cm, False),
+            ("f32k_one_launch_binned", _abi.F32, 1, _abi.PREC_KAHAN, cm, False),
+            ("f32k_one_launch_binned_untrained", _abi.F32, 1, _abi.PREC_KAHAN, untrained,
              False),
             ("f32k_one_launch_index_order", _abi.F32, 1, _abi.PREC_KAHAN, None, False),
             ("f32k_one_launch_span_order", _abi.F32, 1, _abi.PREC_KAHAN, None, True),
             ("f32k_persistent_index_order", _abi.F32, 2, _abi.PREC_KAHAN, None, False),
             ("f32k_persistent_span_order", _abi.F32, 2, _abi.PREC_KAHAN, None, True),
             ("f32k_staged", _abi.F32, 3, _abi.PREC_KAHAN, None, True)):
+        od = None if model is None else plan(model)[1]
         d, g = bufs(dtype, sched, prec, od, span)
         st = solve_once(d, g)
         runs[name] = {"kernel_ms": time_of(d, g, reps), "attempts": st["attempts"],
@@ -607,6 +635,7 @@ def headline(args):
                   args.train_steps, "seconds": t_train, "loss": [hist[0][1], hist[-1][1]]},
     }
 
+    traffic64 = MEASURED_TRAFFIC.get(("f64", n, sched64, "costmodel"))
     out = {
         "metric": "adaptive trajectories/s at stated tol vs FP64 oracle; % FP32/FP64 VALU peak",
         "value": n * steps / wall64,
@@ -626,17 +655,21 @@ def headline(args):
                    "trajectories": n, "sigma": "U(8,12)", "rho": "U(0,28)", "beta": "U(2,3)",
                    "u0": "U(-1,1)^3", "t_end": "LogUniform(0.5,10)", "launch": launch64},
         "roofline": _roofline(flops64, ms64, PEAK_TFLOPS["f64"], _kernel_name("f64", sched64),
-                              MEASURED_TRAFFIC.get(("f64", n, sched64, "costmodel")),
+                              traffic64,
                               {"attempts_per_launch": st64["attempts"],
-                               "algorithmic_bytes_gbs": n * (3 * 8 * 3 + 16 + 8 + 12) /
-                               (ms64 * 1e-3) / 1e9}),
+                               "kernel_ms_covers": "the whole solve: span order and integrator",
+                               "algorithmic_bytes_gbs": n * ALG_BYTES_TSPAN / (ms64 * 1e-3) / 1e9,
+                               "traffic_gbs": None if traffic64 is None else
+                               traffic64 / (ms64 * 1e-3) / 1e9}),
         "error_vs_avern9_1e-14": err64,
         "kernel_variants_f64_ms": variants,
         "f32": f32,
         "learned": learned,
     }
+    _progress("learned binning timed")
     if not args.no_wide:
         out["lorenz_wide"] = lorenz_wide(args, ctx, stream, steps, warmup)
+        _progress("wide ensemble timed")
     if not args.no_cpu:
         k = min(n, 1 << 16)
         dc = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=k, dt=f32_01, **tol)
@@ -690,6 +723,10 @@ def main():
                     help="headline (= costmodel) is the driver's line; the others report their "
                          "own JSON")
     ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--shape", default="",
+                    help="SCHEDULE,BLOCK,GRID,CHUNK: the binary64 launch of the default line, "
+                         "instead of the autotuner's (to repeat a measured shape, e.g. under "
+                         "counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "

@@ -212,7 +212,10 @@ typedef struct sdeh_io {
   const double* dt0;       /* optional [n_traj] initial dt per trajectory (adaptive) */
   const int64_t* order;    /* optional [n_traj] permutation: the order in which
                               trajectories are handed to lanes; results stay indexed
-                              by trajectory */
+                              by trajectory.  Checked to be a permutation before the
+                              solve (SDEH_E_INVALID otherwise), except the order that
+                              sdeh_plan_order last wrote on this context; the
+                              kernels never read an entry outside [0, n_traj) */
   void* u_end;             /* [n_state][n_traj] state at the final time */
   double* t_end;           /* [n_traj] final time */
   int32_t* retcode;        /* [n_traj] sdeh_retcode */
@@ -361,7 +364,11 @@ int sdeh_mlp_plan_lorenz(sdeh_ctx* ctx, const float* params, const double* p, co
  * 2^-30 .. 2^34; costs outside go to the end buckets).  Trajectories that share a
  * bucket keep an unspecified relative order; NaN keys come last.  Device buffers;
  * key [n] FP32, order [n] int64, n < 2^32; ordered on `stream` (NULL = the
- * context's), using the context's device scratch like sdeh_solve. */
+ * context's), using the context's device scratch like sdeh_solve.  The context
+ * remembers the last order buffer written here and its n: a device solve given that
+ * buffer as io->order, for the same n_traj, skips the permutation check (one pass
+ * over n and a host synchronisation), so the buffer must still hold what this call
+ * wrote. */
 int sdeh_plan_order(sdeh_ctx* ctx, const float* key, int64_t n, int64_t* order, void* stream);
 /* Loss (mean squared error over n x 2 outputs) and its gradient with respect
  * to the parameters, on device buffers.  grad has SDEH_MLP_NPARAM floats;

@@ -952,6 +952,16 @@ __device__ __forceinline__ int64_t wave_claim(const sdeh_kargs& a, bool need_me,
   return mine;
 }
 
+// The trajectory handed out at `slot`: order[slot], or slot itself without an order.
+// An entry outside [0, n) names no trajectory: it gives -1, which no lane takes up, so
+// that a hand-out order is never dereferenced out of bounds (the host checks caller
+// orders to be permutations; the library's own orders are permutations by construction).
+__device__ __forceinline__ int64_t slot_traj(const sdeh_kargs& a, int64_t slot) {
+  if (!a.order) return slot;
+  const int64_t v = a.order[slot];
+  return (uint64_t)v < (uint64_t)a.n ? v : int64_t(-1);
+}
+
 // The debug visit counter of trajectory `traj` (sdeh_kargs::visits), one per take-up.
 __device__ __forceinline__ void note_visit(const sdeh_kargs& a, int64_t traj) {
   if (a.visits) atomicAdd(a.visits + traj, 1);
@@ -1021,8 +1031,8 @@ __device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
     const int64_t slot =
         (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * SLOTS + (lane & (SLOTS - 1));
     if (slot < n) {
-      traj = a.order ? a.order[slot] : slot;
-      active = true;
+      traj = slot_traj(a, slot);
+      active = traj >= 0;
     }
   }
   bool fresh = active;
@@ -1032,9 +1042,8 @@ __device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
       int64_t slot = wave_claim(a, !active && lead, qbase, qlim, drained);
       slot = (int64_t)__shfl((long long)slot, lane & (SLOTS - 1));  // from the slot's lead lane
       if (slot >= 0) {
-        traj = a.order ? a.order[slot] : slot;
-        active = true;
-        fresh = true;
+        traj = slot_traj(a, slot);
+        active = fresh = traj >= 0;
       }
     }
     if (__ballot(active) == 0ull) break;
@@ -1338,8 +1347,8 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
   if (!a.persistent) {
     const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / R::G;
     if (slot < n) {
-      traj = a.order ? a.order[slot] : slot;
-      active = true;
+      traj = slot_traj(a, slot);
+      active = traj >= 0;
     }
   }
   bool fresh = active;
@@ -1359,11 +1368,10 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
         if constexpr (R::G > 1)  // the group leader claimed; its group follows
           slot = (int64_t)__shfl((long long)slot, (int)(__lane_id() & ~(unsigned)(R::G - 1)));
         if (slot >= 0) {
-          traj = a.order ? a.order[slot] : slot;
-          active = true;
-          fresh = true;
+          traj = slot_traj(a, slot);
+          active = fresh = traj >= 0;
           // a resumed launch skips the trajectories that ended in the first stage
-          if (STAGED && a.resume && Park::flag(a, traj) == 0) active = fresh = false;
+          if (STAGED && a.resume && active && Park::flag(a, traj) == 0) active = fresh = false;
         }
         want = !active && lead;
         if (!STAGED || !a.resume || drained || __ballot(want && slot >= 0) == 0ull) break;
@@ -1615,8 +1623,8 @@ __device__ __forceinline__ void adaptive_body_multi(const sdeh_kargs& a) {
 #pragma unroll
     for (int k = 0; k < TPL; ++k)
       if (base + k < n) {
-        traj[k] = a.order ? a.order[base + k] : base + k;
-        active[k] = fresh[k] = true;
+        traj[k] = slot_traj(a, base + k);
+        active[k] = fresh[k] = traj[k] >= 0;
       }
   }
 
@@ -1626,8 +1634,8 @@ __device__ __forceinline__ void adaptive_body_multi(const sdeh_kargs& a) {
       for (int k = 0; k < TPL; ++k) {
         const int64_t slot = wave_claim(a, !active[k], qbase, qlim, drained);
         if (slot >= 0) {
-          traj[k] = a.order ? a.order[slot] : slot;
-          active[k] = fresh[k] = true;
+          traj[k] = slot_traj(a, slot);
+          active[k] = fresh[k] = traj[k] >= 0;
         }
       }
     }
@@ -1766,7 +1774,8 @@ __device__ __forceinline__ void fixed_body(const sdeh_kargs& a) {
   if (slot >= a.n) return;  // whole groups leave together
   const bool lead = lane_comp<R>() == 0;
   const int64_t n = a.n;
-  const int64_t traj = a.order ? a.order[slot] : slot;
+  const int64_t traj = slot_traj(a, slot);
+  if (traj < 0) return;  // the group's slot names no trajectory
   if (lead) note_visit(a, traj);
   const T* ps = (const T*)a.ps;
   T u[N], uprev[N], pl[P], zero[N], dummy[N], err[N];

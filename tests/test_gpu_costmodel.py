@@ -96,6 +96,8 @@ def test_trained_binning_meets_target(ctx, setup):
     assert np.array_equal(np.sort(o), np.arange(N_EVAL))
     res = {}
     for name, sched in (("persistent", _abi.SCHED_PERSISTENT), ("one_launch", _abi.SCHED_ONE_LAUNCH)):
+        order = _plan(S, cm)[1]  # the last planned order: no permutation check
+        torch.cuda.synchronize()
         d, g, st = _solve(ctx, S, _abi.F32, sched, _abi.PREC_KAHAN, order=order)
         res[name] = (_time(ctx, S, d, g), g)
     # binning changes no trajectory's result
@@ -146,8 +148,14 @@ def test_network_gain_ablation(ctx, setup):
     plan_ms = {"trained": _events(torch, S["stream"], lambda: _plan(S, cm)),
                "untrained": _events(torch, S["stream"], lambda: _plan(S, untrained)), "none": 0.0}
     eff, ms = {}, {}
+    models = {"trained": cm, "untrained": untrained, "none": None}
     for k, od in orders.items():
         eff[k] = costmodel.wave_efficiency(att, None if od is None else od.cpu().numpy())
+        # planned right before the solve, as a caller does (the context takes the
+        # order sdeh_plan_order wrote last without the permutation check)
+        if models[k] is not None:
+            od = _plan(S, models[k])[1]
+            torch.cuda.synchronize()
         d, g, _ = _solve(ctx, S, _abi.F32, _abi.SCHED_ONE_LAUNCH, _abi.PREC_KAHAN, order=od)
         ms[k] = _time(ctx, S, d, g)
     # model-free references: longest span first (one launch) and the persistent queue
