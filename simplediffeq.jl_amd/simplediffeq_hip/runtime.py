@@ -156,6 +156,12 @@ class EnsembleBuffers:
         io.steps_offset = ptr(self.steps_offset)
         io.steps_capacity = self.steps_capacity if self.packed else 0
         self.io = io
+        # Device buffers are filled (copies, dtype conversions) on torch's current
+        # stream; a solve on another stream waits for this event first.
+        self.ready = None
+        if backend == "torch":
+            self.ready = torch.cuda.Event()
+            self.ready.record(torch.cuda.current_stream(self.u0.device))
 
     def steps_of(self, i):
         """(t, u) of trajectory i's save_everystep rows in the packed layout, as
@@ -231,12 +237,24 @@ class Context:
         ts.synchronize()
         return None
 
+    @staticmethod
+    def _after_buffers(bufs, stream):
+        """Order a solve on the explicit `stream` after the work that filled the device
+        buffers on torch's stream (without it the kernels could read inputs that a
+        conversion on the other stream is still writing)."""
+        if bufs.backend == "torch" and stream and bufs.ready is not None:
+            import torch
+            torch.cuda.ExternalStream(stream, device=bufs.u0.device).wait_event(bufs.ready)
+
     def solve(self, desc, bufs, stream=None, sync=True, allow=(_abi.E_DTMIN,)):
         """Run sdeh_solve on an EnsembleBuffers; returns the status.  Device
-        (torch) buffers run on torch's current stream unless `stream` is given."""
+        (torch) buffers run on torch's current stream unless `stream` is given; a
+        given stream first waits for the buffers' construction."""
         mem = _abi.MEM_HOST if bufs.backend == "numpy" else _abi.MEM_DEVICE
         if stream is None and bufs.backend == "torch":
             stream = self._torch_stream(bufs)
+        else:
+            self._after_buffers(bufs, stream)
         st = self.lib.sdeh_solve(self.h, ctypes.byref(desc), ctypes.byref(bufs.io), mem,
                                  ctypes.c_void_p(stream), 1 if sync else 0)
         return self._check(st, allow)
@@ -248,6 +266,8 @@ class Context:
             raise ValueError("the allocation-free solve takes device buffers")
         if stream is None:
             stream = self._torch_stream(bufs)
+        else:
+            self._after_buffers(bufs, stream)
         st = self.lib.sdeh_solve_static(self.h, ctypes.byref(desc), ctypes.byref(bufs.io),
                                         ctypes.c_void_p(stream), 1 if sync else 0)
         return self._check(st, allow)

@@ -182,3 +182,26 @@ def test_binary32_modes_against_binary64(ctx):
     assert e64 < 1e-4 and ek < 1e-4
     assert ek <= e32 * 1.05
     assert (comp.numpy("retcode") == 0).all() and (comp.numpy("t_end") == 10.0).all()
+
+
+def test_solve_on_another_stream_waits_for_the_buffer_conversion(ctx):
+    """EnsembleBuffers converts binary64 device inputs to binary32 on torch's current
+    stream; a solve on another stream must start after that conversion (the buffers
+    carry an event the solve's stream waits on).  2^22 trajectories, so that the
+    conversion takes long enough to overlap an unordered launch; maxiters bounds the
+    run should inputs ever be read half-written."""
+    import torch
+    n = 1 << 22
+    u0, p = inputs.lorenz_wide(n, seed=21)
+    dev = torch.device(DEV)
+    u0d, pd = torch.as_tensor(u0, device=dev), torch.as_tensor(p, device=dev)
+    d = _desc(n=n, dtype=_abi.F32, schedule=_abi.SCHED_ONE_LAUNCH, maxiters=20000)
+    ref = _dev_bufs(d, u0d, pd)
+    ctx.solve(d, ref)
+    side = torch.cuda.Stream(device=dev)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b = _dev_bufs(d, u0d, pd)  # conversion kernels queued on the current stream
+        ctx.solve(d, b, stream=side.cuda_stream, sync=True)
+        for f in ("u_end", "t_end", "naccept", "nreject", "retcode"):
+            assert bits_equal(b.numpy(f), ref.numpy(f)), f
