@@ -256,6 +256,9 @@ def workload_glv(args):
         variants = [(_abi.SAVE_TRAJ_MAJOR, 0, ""), (_abi.SAVE_POINT_MAJOR, 0, "_point_major")]
         if dtype == _abi.F64:
             variants += [(_abi.SAVE_TRAJ_MAJOR, 16, "_lanes16"), (_abi.SAVE_TRAJ_MAJOR, 4, "_lanes4")]
+        else:
+            variants += [(_abi.SAVE_TRAJ_MAJOR, 1, "_lanes1"), (_abi.SAVE_TRAJ_MAJOR, 4, "_lanes4"),
+                         (_abi.SAVE_POINT_MAJOR, 4, "_lanes4_point_major")]
         for layout, lanes, tag in variants:
             d = runtime.make_desc(alg, dtype=dtype, rhs=_abi.RHS_GLV16, n_traj=n,
                                   tspan=(0.0, 50.0), dt=0.1, abstol=tol, reltol=tol,

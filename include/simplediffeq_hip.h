@@ -177,7 +177,8 @@ typedef struct sdeh_solve_desc {
                                endpoint and saveat), else its group layout (GLV16: 16
                                adjacent lanes, one component each), else 1; 1 = one
                                lane holds a whole trajectory; or either layout's size
-                               (4, 16) itself.  Results are identical. */
+                               (4, 16) itself (4 also runs binary32 ATsit5, on
+                               v_mfma_f32_16x16x4f32).  Results are identical. */
   int32_t save_layout;      /* sdeh_save_layout of u_saveat, u_steps and t_steps */
   int32_t span_order;       /* 0: with per-trajectory tspans (io->tspan) and no io->order,
                                hand trajectories to lanes longest span first (fewer

@@ -64,9 +64,11 @@ const void* multi_variant(int tpl) {
 template <class T, int ALG, int SAVE>
 constexpr bool allowed() {
 #if defined(SDEH_WAVE_ONLY)
-  // The matrix-core layout: binary64 adaptive solvers, endpoint and saveat output.
-  return sizeof(T) == 8 && (ALG == sdeh::ALG_AVERN9 || ALG == sdeh::ALG_ATSIT5) &&
-         SAVE != sdeh::SAVE_STEPS;
+  // The matrix-core layout: binary64 AVern9 and ATsit5, binary32 ATsit5 (plain and
+  // compensated), endpoint and saveat output.
+  return (sizeof(T) == 8 && (ALG == sdeh::ALG_AVERN9 || ALG == sdeh::ALG_ATSIT5) &&
+          SAVE != sdeh::SAVE_STEPS) ||
+         (sizeof(T) == 4 && ALG == sdeh::ALG_ATSIT5 && SAVE != sdeh::SAVE_STEPS);
 #elif defined(SDEH_LIMITED)
   return (ALG == sdeh::ALG_AVERN9 && sizeof(T) == 8 && SAVE != sdeh::SAVE_STEPS) ||
          (ALG == sdeh::ALG_ATSIT5 && SAVE != sdeh::SAVE_STEPS) ||

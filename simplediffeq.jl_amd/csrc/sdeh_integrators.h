@@ -204,12 +204,21 @@ __device__ __forceinline__ T* wave_lds() {
 }
 
 // Wave layout: all[c] = component c (c = 0 .. N*G-1) of this lane's trajectory,
-// for a wave in uniform control flow.  LDS [slot][component] in the wave's share.
+// for a wave in uniform control flow.  LDS [slot][component] in the wave's share,
+// slots WAVE_SLOT_STRIDE<T, R> values apart: one more than the components, so that
+// the slots of a wave start in different banks (at a stride of 16 binary64 values,
+// 128 B, every slot's component c fell in the same bank pair: 8-way conflicts on
+// each exchange).
+template <class T, class R>
+constexpr int wave_slot_stride() {
+  return R::N * R::G + 1;
+}
+
 template <class T, class R, int NW>
 __device__ __forceinline__ void wave_gather(const T (&v)[R::N], T (&all)[R::N * R::G]) {
-  constexpr int FULL = R::N * R::G, SLOTS = 64 / R::G;
-  static_assert(FULL * SLOTS <= NW, "one wave's share of the exchange buffer");
-  T* w = wave_lds<T, NW>() + (threadIdx.x & (unsigned)(SLOTS - 1)) * FULL;
+  constexpr int FULL = R::N * R::G, SLOTS = 64 / R::G, XS = wave_slot_stride<T, R>();
+  static_assert(XS * SLOTS <= NW, "one wave's share of the exchange buffer");
+  T* w = wave_lds<T, NW>() + (threadIdx.x & (unsigned)(SLOTS - 1)) * XS;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -295,7 +304,8 @@ __device__ __forceinline__ double stage_t(double t, double c, double dt) {
 // = uprev + (terms), so a NaN in uprev is a NaN in u, and one comparison select
 // that keeps |u| unless |u| < |uprev| gives the same value (a NaN only through u,
 // and a NaN makes the norm NaN whatever its payload).
-template <class T, class R, int NW = 256>
+// NW: the wave layout's exchange buffer (values per wave; 0 = the gather's own size).
+template <class T, class R, int NW = 0>
 __device__ __forceinline__ T err_sumsq(T (&tmp)[R::N], const T (&uprev)[R::N],
                                        const T (&u)[R::N], T abstol, T reltol) {
   constexpr int N = R::N;
@@ -307,8 +317,9 @@ __device__ __forceinline__ T err_sumsq(T (&tmp)[R::N], const T (&uprev)[R::N],
   }
   if constexpr (IsWave<R>::value) {
     constexpr int FULL = N * R::G;
+    constexpr int NWX = NW > 0 ? NW : (64 / R::G) * wave_slot_stride<T, R>();
     T all[FULL];
-    wave_gather<T, R, NW>(tmp, all);
+    wave_gather<T, R, NWX>(tmp, all);
     T s = all[0] * all[0];
 #pragma unroll
     for (int c = 1; c < FULL; ++c) s = s + all[c] * all[c];
@@ -993,8 +1004,12 @@ __device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
   constexpr int SLOTS = 64 / R::G;  // trajectories per wave
   constexpr bool V9 = (ALG == ALG_AVERN9);
   // LDS exchange per wave: the error norm's components, and (AVern9 with saveat) the
-  // dense-output weights of R::G points per trajectory.
-  constexpr int NWL = (V9 && SAVE == SAVE_AT) ? SLOTS * R::G * V9W : 256;
+  // dense-output weights of R::G points per trajectory.  Each trajectory's share is
+  // padded to an odd number of values (DWS) so that the wave's trajectories read
+  // their weights from different banks.
+  constexpr int DWS = R::G * V9W + 1;
+  constexpr int NWG = SLOTS * wave_slot_stride<T, R>();
+  constexpr int NWL = (V9 && SAVE == SAVE_AT && SLOTS * DWS > NWG) ? SLOTS * DWS : NWG;
   constexpr double DTMIN = V9 ? (double)1.0e-7f : 1.0e-14;  // gpuvern9.jl:466
   constexpr double SNAP = DTMIN;                            // gpuvern9.jl:591
   // build_adaptive_controller_cache(T), src/SimpleDiffEq.jl:67-77
@@ -1174,7 +1189,7 @@ __device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
               // share them through LDS; every lane then forms its components of each
               // point in order.  Same values as vern9_dense point by point.
               constexpr int PB = R::G;
-              T* wl = wave_lds<T, NWL>() + (lane & (SLOTS - 1)) * (PB * V9W);
+              T* wl = wave_lds<T, NWL>() + (lane & (SLOTS - 1)) * DWS;
               const int q = lane_comp<R>();
               while (cur < a.n_saveat && ts_next <= t) {
                 const int64_t pq = cur + q;

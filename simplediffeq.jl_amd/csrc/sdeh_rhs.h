@@ -83,7 +83,7 @@ struct GLV16G {
   }
 };
 
-// GLV-16 on the matrix cores (binary64), in the wave layout: 4 lanes per
+// GLV-16 on the matrix cores (binary64; binary32 ATsit5 below), in the wave layout: 4 lanes per
 // trajectory and 16 trajectories per wave.  Trajectory slot j of a wave is on lanes
 // j, j+16, j+32, j+48; lane q (= lane / 16) holds components q, q+4, q+8, q+12.
 // A u for the wave's 16 trajectories is four v_mfma_f64_16x16x4f64 (D = A B + C,
@@ -95,6 +95,11 @@ struct GLV16G {
 // k = 0..3 in order (lane layout and rounding checked on the device, 2^20 outputs
 // bit for bit), so acc_i = r_i, then fma(A_ij, u_j, acc) for j = 0..15:
 // GLV16's arithmetic, with no exchange of the state between lanes.
+// Binary32 runs v_mfma_f32_16x16x4f32, which forms each output the same way (fma
+// chain over k = 0..3, checked on the device: 38,400 random outputs bit for bit) but
+// puts output rows 4(lane / 16) + r in lane `lane`, register r.  Its A operand's row
+// i' therefore carries component i' / 4 + 4 (i' % 4), the component that lane i' / 4
+// holds in register i' % 4, and the layout above is unchanged.
 // pl[0..3] = r of the lane's components (per trajectory), pl[4..7] = the lane's A
 // operands (the same for every trajectory: loaded once for every lane).
 struct GLV16M {
@@ -103,8 +108,9 @@ struct GLV16M {
   template <class T>
   __device__ static inline void init_lane(T* pl, const T* __restrict__ ps) {
     const int l = (int)__lane_id();
+    const int row = sizeof(T) == 8 ? (l & 15) : ((l & 15) >> 2) + 4 * (l & 3);
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) pl[4 + kb] = ps[(l & 15) * 16 + 4 * kb + (l >> 4)];
+    for (int kb = 0; kb < 4; ++kb) pl[4 + kb] = ps[row * 16 + 4 * kb + (l >> 4)];
   }
   template <class T>
   __device__ static inline void load_lane(T* pl, const T* p, const T* __restrict__ ps, int64_t n,
@@ -115,14 +121,23 @@ struct GLV16M {
   template <class T>
   __device__ static inline void f(T* du, const T* u, const T* p, const T* __restrict__ ps,
                                   double t) {
-    static_assert(sizeof(T) == 8, "binary64 matrix-core product");
-    typedef double d4 __attribute__((ext_vector_type(4)));
-    d4 acc = {p[0], p[1], p[2], p[3]};
+    if constexpr (sizeof(T) == 8) {
+      typedef double d4 __attribute__((ext_vector_type(4)));
+      d4 acc = {p[0], p[1], p[2], p[3]};
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb)
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(p[4 + kb], u[kb], acc, 0, 0, 0);
+      for (int kb = 0; kb < 4; ++kb)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(p[4 + kb], u[kb], acc, 0, 0, 0);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) du[r] = u[r] * acc[r];
+      for (int r = 0; r < 4; ++r) du[r] = u[r] * acc[r];
+    } else {
+      typedef float f4 __attribute__((ext_vector_type(4)));
+      f4 acc = {p[0], p[1], p[2], p[3]};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(p[4 + kb], u[kb], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) du[r] = u[r] * acc[r];
+    }
   }
 };
 

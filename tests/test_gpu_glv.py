@@ -3,9 +3,10 @@ against the CPU oracle, in its three lane layouts: one lane per trajectory; the
 group layout (16 adjacent lanes per trajectory, one component each, state
 exchanged through LDS); and, for the binary64 adaptive solvers, the matrix-core
 layout (4 lanes per trajectory 16 apart, 4 components each, A u as four
-v_mfma_f64_16x16x4f64 per wave).  binary64 is bit for bit against the oracle
-(endpoint and saveat, both schedules); binary32 is bit for bit between the one-lane
-and group layouts, which evaluate the same operations in the same order."""
+v_mfma_f64_16x16x4f64 per wave; binary32 ATsit5 on v_mfma_f32_16x16x4f32).  binary64
+is bit for bit against the oracle (endpoint and saveat, both schedules); binary32 is
+bit for bit between the one-lane, group and matrix-core layouts, which evaluate the
+same operations in the same order."""
 import numpy as np
 import pytest
 
@@ -51,15 +52,21 @@ def test_glv16_binary64_bitexact(ctx, alg, saveat, lanes, sched, layout):
         assert (g.numpy("naccept") > 5).all()
 
 
-@pytest.mark.parametrize("kahan,saveat", [(False, False), (False, True), (True, False)],
-                         ids=["plain", "plain-saveat", "kahan"])
-def test_glv16_binary32_layouts_identical(ctx, kahan, saveat):
+@pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT])
+@pytest.mark.parametrize("kahan,saveat,layout",
+                         [(False, False, _abi.SAVE_POINT_MAJOR),
+                          (False, True, _abi.SAVE_POINT_MAJOR),
+                          (False, True, _abi.SAVE_TRAJ_MAJOR),
+                          (True, False, _abi.SAVE_POINT_MAJOR)],
+                         ids=["plain", "plain-saveat", "plain-saveat-traj-major", "kahan"])
+def test_glv16_binary32_layouts_identical(ctx, kahan, saveat, layout, sched):
     n = 301
     u0, r, A = problem(_abi.RHS_GLV16, n, seed=4)
     prec = _abi.PREC_KAHAN if kahan else _abi.PREC_PLAIN
     out = {}
-    for lanes in (1, 16):
-        d = _desc(_abi.ALG_ATSIT5, n, saveat, lanes, dtype=_abi.F32, precision=prec)
+    for lanes in (1, 16, 4):
+        d = _desc(_abi.ALG_ATSIT5, n, saveat, lanes, dtype=_abi.F32, precision=prec,
+                  schedule=sched, chunk=3, save_layout=layout)
         d.abstol, d.reltol = 1e-6, 1e-6
         b = runtime.EnsembleBuffers(d, 16, u0, r, A)
         ctx.solve(d, b)
@@ -67,6 +74,7 @@ def test_glv16_binary32_layouts_identical(ctx, kahan, saveat):
         out[lanes] = b
     for f in FIELDS + (("u_saveat",) if saveat else ()):
         assert bits_equal(out[1].numpy(f), out[16].numpy(f)), f
+        assert bits_equal(out[1].numpy(f), out[4].numpy(f)), f
     # and close to the binary64 oracle at a tight tolerance
     from oracle import pyoracle
     d = _desc(_abi.ALG_AVERN9, n, False, 1)
@@ -91,11 +99,13 @@ def test_glv16_layout_selection(ctx):
     d = _desc(_abi.ALG_ATSIT5, n, False, 0, dtype=_abi.F32)
     ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0, r, A))
     assert ctx.stats()["lanes"] == 1
-    # the matrix-core layout on request; it has binary64 adaptive kernels only
-    d = _desc(_abi.ALG_AVERN9, n, False, 4)
-    ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0, r, A))
-    assert ctx.stats()["lanes"] == 4
-    for kw in ({"alg": _abi.ALG_TSIT5}, {"alg": _abi.ALG_ATSIT5, "dtype": _abi.F32}):
+    # the matrix-core layout on request: binary64 AVern9 and ATsit5, binary32 ATsit5
+    for kw in ({"alg": _abi.ALG_AVERN9}, {"alg": _abi.ALG_ATSIT5, "dtype": _abi.F32}):
+        alg = kw.pop("alg")
+        d = _desc(alg, n, False, 4, **kw)
+        ctx.solve(d, runtime.EnsembleBuffers(d, 16, u0, r, A))
+        assert ctx.stats()["lanes"] == 4
+    for kw in ({"alg": _abi.ALG_TSIT5}, {"alg": _abi.ALG_AVERN9, "dtype": _abi.F32}):
         alg = kw.pop("alg")
         d = _desc(alg, n, False, 4, **kw)
         with pytest.raises(runtime.SolveError, match="no matrix-core kernel"):
