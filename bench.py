@@ -587,6 +587,7 @@ def headline(args):
             ("f64_one_launch_index_order", _abi.F64, 1, _abi.PREC_PLAIN, None, False),
             ("f32k_persistent_binned", _abi.F32, 2, _abi.PREC_KAHAN, cm, False),
             ("f32k_one_launch_binned", _abi.F32, 1, _abi.PREC_KAHAN, cm, False),
+            ("f32k_one_launch_binned_two_per_lane", _abi.F32, 1, _abi.PREC_KAHAN, cm, False),
             ("f32k_one_launch_binned_untrained", _abi.F32, 1, _abi.PREC_KAHAN, untrained,
              False),
             ("f32k_one_launch_index_order", _abi.F32, 1, _abi.PREC_KAHAN, None, False),
@@ -595,7 +596,8 @@ def headline(args):
             ("f32k_persistent_span_order", _abi.F32, 2, _abi.PREC_KAHAN, None, True),
             ("f32k_staged", _abi.F32, 3, _abi.PREC_KAHAN, None, True)):
         od = None if model is None else plan(model)[1]
-        d, g = bufs(dtype, sched, prec, od, span)
+        d, g = bufs(dtype, sched, prec, od, span,
+                    **(dict(per_lane=2) if name.endswith("two_per_lane") else {}))
         st = solve_once(d, g)
         runs[name] = {"kernel_ms": time_of(d, g, reps), "attempts": st["attempts"],
                       "schedule": st["schedule"]}
@@ -611,7 +613,8 @@ def headline(args):
     best_free = min(("f32k_one_launch_index_order", "f32k_one_launch_span_order",
                      "f32k_persistent_index_order", "f32k_persistent_span_order", "f32k_staged"),
                     key=lambda k: runs[k]["kernel_ms"])
-    best_binned = min(("f32k_persistent_binned", "f32k_one_launch_binned"),
+    best_binned = min(("f32k_persistent_binned", "f32k_one_launch_binned",
+                       "f32k_one_launch_binned_two_per_lane"),
                       key=lambda k: runs[k]["kernel_ms"])
     learned = {
         "value": n / (binned_ms * 1e-3), "unit": "trajectories/s",
