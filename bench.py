@@ -339,7 +339,7 @@ def workload_everystep(args):
 
 
 def _timed_steps(ctx, d, bufs, stream, steps, warmup):
-    """The driver's timed region: `warmup` untimed solves, then exactly `steps`
+    """The timed region: `warmup` untimed solves, then exactly `steps`
     solves bracketed by synchronisation.  Returns (device ms per solve from HIP
     events on the launch stream, wall seconds of the K solves)."""
     import torch
@@ -520,7 +520,7 @@ def headline(args):
         return _events(stream, lambda: ctx.solve(d, g, stream=stream.cuda_stream, sync=False),
                        reps)
 
-    # 3. value: binary64 in the autotuned shape, the driver's timed region
+    # 3. value: binary64 in the autotuned shape, the timed region
     shape = {}
     if args.shape:
         sh = [int(v) for v in args.shape.split(",")]
@@ -726,7 +726,7 @@ def main():
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "costmodel", "lorenz-wide", "lorenz-small",
                              "lorenz-uneven", "glv16", "everystep"],
-                    help="headline (= costmodel) is the driver's line; the others report their "
+                    help="headline (= costmodel) is the default line; the others report their "
                          "own JSON")
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--shape", default="",
@@ -735,7 +735,7 @@ def main():
                          "counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported "
+        print("bench.py measures one GPU; --gpus %d is not supported "
               "(trajectories are independent: an N-GPU run gives each GPU a contiguous slice)"
               % args.gpus, file=sys.stderr)
         return 2

@@ -13,7 +13,7 @@
  *   DiffEqBase.solve(prob::ODEProblem, alg::GPUSimpleVern9;  ...) src/verner/gpuvern9.jl:55-353
  *   DiffEqBase.solve(prob::ODEProblem, alg::GPUSimpleAVern9; ...) src/verner/gpuvern9.jl:411-779
  *
- * that an EnsembleProblem/EnsembleThreads driver would make, one trajectory per
+ * that an EnsembleProblem under EnsembleThreads would make, one trajectory per
  * GPU lane.  Keyword defaults mirror those signatures (sdeh_desc_init()).
  *
  * Data layout: every per-trajectory array is structure-of-arrays, component
@@ -52,7 +52,7 @@ typedef enum sdeh_status {
   SDEH_E_INVALID = -1,      /* bad argument (message says which) */
   SDEH_E_HIP = -2,          /* HIP runtime error */
   SDEH_E_COMPILE = -3,      /* hipRTC compilation failed (log in sdeh_last_error) */
-  SDEH_E_UNSUPPORTED = -4,  /* valid request this build does not implement */
+  SDEH_E_UNSUPPORTED = -4,  /* valid request the library does not implement */
   SDEH_E_NOMEM = -5,        /* device allocation failed */
   SDEH_E_DTMIN = -6,        /* solve finished; >= 1 trajectory stopped at dt < dtmin */
   SDEH_E_SAVE_OVERFLOW = -7 /* solve finished; >= 1 trajectory had more steps than max_saves */

@@ -1,7 +1,7 @@
 // ORACLE — TEST INFRASTRUCTURE ONLY (see oracle.hpp).
 //
 // C entry points of the CPU restatement, loaded with ctypes by tests/,
-// __graft_entry__.smoke() and bench.py's cpu_baseline.  The ensemble driver
+// __graft_entry__.smoke() and bench.py's cpu_baseline.  The ensemble loop
 // splits [0, n_traj) into contiguous per-thread chunks, the batching
 // SciMLBase's EnsembleThreads uses, and runs one reference-style solve per
 // trajectory.  Buffers follow the library's layout (simplediffeq_hip.h),

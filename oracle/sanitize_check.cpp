@@ -1,6 +1,6 @@
 // ORACLE — TEST INFRASTRUCTURE ONLY (see oracle.hpp).
 //
-// A driver for an AddressSanitizer + UndefinedBehaviorSanitizer build of the CPU
+// A test program for an AddressSanitizer + UndefinedBehaviorSanitizer build of the CPU
 // restatement (make -C oracle sanitize): every solver, both state types, every
 // built-in right-hand side, endpoint / saveat (both layouts) / every-step output
 // (with and without overflow), one and several threads, and the edge cases the

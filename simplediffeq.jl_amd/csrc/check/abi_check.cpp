@@ -1,4 +1,4 @@
-// Driver of the host-sanitized build of libsimplediffeq_hip (make asan): the
+// Test program for the host-sanitized build of libsimplediffeq_hip (make asan): the
 // library's host code (descriptor validation, planning, arena sizing and reuse,
 // the chunked host pipeline, the staged schedule's scratch, the packed layout's
 // capacity protocol, graphs, hipRTC registration, the cost-model entry points and

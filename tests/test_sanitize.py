@@ -1,4 +1,4 @@
-"""The CPU oracle under AddressSanitizer and UndefinedBehaviorSanitizer: a driver
+"""The CPU oracle under AddressSanitizer and UndefinedBehaviorSanitizer: a test program
 (oracle/sanitize_check.cpp) solves every solver x state type x built-in right-hand
 side x output mode (endpoint, saveat in both layouts, every step with room and
 with overflow, the edge cases: NaN, dt below dtmin, maxiters, empty and reversed
