@@ -663,6 +663,11 @@ def headline(args):
         "roofline": _roofline(flops64, ms64, PEAK_TFLOPS["f64"], _kernel_name("f64", sched64),
                               traffic64,
                               {"attempts_per_launch": st64["attempts"],
+                               # rejected attempts run the same branch-free attempt
+                               # body as accepted ones; lane use needs the hardware
+                               # counters (rocprofv3), which a timed run cannot read
+                               "rejected_attempt_fraction":
+                                   1.0 - st64["accepts"] / max(1, st64["attempts"]),
                                "kernel_ms_covers": "the whole solve: span order and integrator",
                                "algorithmic_bytes_gbs": n * ALG_BYTES_TSPAN / (ms64 * 1e-3) / 1e9,
                                "traffic_gbs": None if traffic64 is None else
