@@ -168,6 +168,7 @@ def workload_small(args):
     ctx = runtime.Context(0)
     stream = torch.cuda.Stream()
     out = {"workload": "lorenz-small"}
+    ref = _Reference(ctx, u0, p, t=(0.0, 1.0))
     for name, alg, dt in (("tsit5", _abi.ALG_TSIT5, 0.01), ("atsit5", _abi.ALG_ATSIT5, 0.1)):
         d = runtime.make_desc(alg, rhs=_abi.RHS_LORENZ, n_traj=1024, tspan=(0.0, 1.0), dt=dt,
                               abstol=1e-6, reltol=1e-3)
@@ -181,7 +182,8 @@ def workload_small(args):
         cpu = (time.perf_counter() - t0) / 5
         same = all(np.array_equal(g.numpy(f), c.numpy(f)) for f in ("u_end", "naccept", "nreject"))
         out[name] = {"gpu_kernel_ms": ms, "gpu_call_ms": wall * 1e3, "cpu_ms": cpu * 1e3,
-                     "cpu_threads": threads, "bit_identical": bool(same)}
+                     "cpu_threads": threads, "bit_identical": bool(same),
+                     "error_vs_avern9_1e-14": ref.error(g)}
     return out
 
 
@@ -218,7 +220,8 @@ def workload_uneven(args):
     best = ctx.autotune(d, g)
     ms, _ = _time_solves(ctx, d, g, stream, args.steps, args.warmup)
     out["autotuned"] = {"kernel_ms": ms, "traj_per_s": n / (ms * 1e-3),
-                        "launch": {k: best[k] for k in ("block", "grid", "chunk", "schedule")}}
+                        "launch": {k: best[k] for k in ("block", "grid", "chunk", "schedule")},
+                        "error_vs_avern9_1e-14": _Reference(ctx, u0, p, tspan=ts).error(g)}
     if not args.no_cpu:
         k = 1 << 16
         dc = runtime.make_desc(_abi.ALG_AVERN7, **dict(base, n_traj=k))
@@ -250,6 +253,8 @@ def workload_glv(args):
     ctx = runtime.Context(0)
     stream = torch.cuda.Stream()
     out = {"workload": "glv16-saveat", "trajectories": n}
+    # endpoint error against binary64 AVern9 at 1e-14 on the first 16,384 trajectories
+    ref = _Reference(ctx, u0, r, t=(0.0, 50.0), m=1 << 14, rhs=_abi.RHS_GLV16, ps=A)
     for name, alg, dtype, tol in (("avern9_f64", _abi.ALG_AVERN9, _abi.F64, 1e-10),
                                   ("atsit5_f32", _abi.ALG_ATSIT5, _abi.F32, 1e-6)):
         # default lane layout in both save layouts; binary64 also in each layout
@@ -275,6 +280,8 @@ def workload_glv(args):
                                "attempts": st["attempts"], "lanes": st["lanes"],
                                "saveat_GBs": out_bytes / (ms * 1e-3) / 1e9,
                                "tflops_excl_saveat": flops / (ms * 1e-3) / 1e12}
+            if not tag:
+                out[name + tag]["error_vs_avern9_1e-14"] = ref.error(g)
             del g
         if not args.no_cpu:
             k = min(n, 1 << 14 if dtype == _abi.F64 else 1 << 16)
@@ -380,14 +387,15 @@ class _Reference:
     """Binary64 AVern9 at abstol = reltol = 1e-14 on the first `m` trajectories of
     an ensemble (the oracle runs of the error figures), solved on the GPU."""
 
-    def __init__(self, ctx, u0, p, tspan=None, t=None, m=N_ERR):
+    def __init__(self, ctx, u0, p, tspan=None, t=None, m=N_ERR, rhs=None, ps=None):
         import numpy as np
         from simplediffeq_hip import _abi, runtime
         m = min(m, u0.shape[1])
         kw = dict(tspan=t) if t is not None else {}
-        d = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_LORENZ, n_traj=m,
-                              dt=float(np.float32(0.1)), abstol=1e-14, reltol=1e-14, **kw)
-        g = runtime.EnsembleBuffers(d, 3, u0[:, :m], p[:, :m],
+        d = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_LORENZ if rhs is None else rhs,
+                              n_traj=m, dt=float(np.float32(0.1)), abstol=1e-14, reltol=1e-14,
+                              **kw)
+        g = runtime.EnsembleBuffers(d, u0.shape[0], u0[:, :m], p[:, :m], ps,
                                     tspan=None if tspan is None else tspan[:, :m],
                                     backend="torch", device="cuda:0")
         ctx.solve(d, g, sync=True)
