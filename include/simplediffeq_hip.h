@@ -331,7 +331,7 @@ int64_t sdeh_fixed_grid(double t0, double dt, double tf, double* elems, int64_t 
 
 /* ---- Learned cost model: 6 -> 64 -> 64 -> 2 MLP (FP32), trained on device. ----
  * Layout of the parameter vector (FP32, n = 4738): W1[64][6], b1[64],
- * W2[64][64], b2[64], W3[2][64], b3[2].  Activations are tanh. */
+ * W2[64][64], b2[64], W3[2][64], b3[2].  Hidden activations are max(x, 0) (ReLU). */
 #define SDEH_MLP_NIN 6
 #define SDEH_MLP_NH 64
 #define SDEH_MLP_NOUT 2
@@ -347,10 +347,9 @@ int sdeh_mlp_forward(sdeh_ctx* ctx, const float* params, const float* x, float* 
  * (p_c - feat[2c]) * (1 / feat[2c+1]) for c = 0, 1, 2; (log(tf - t0) - feat[6]) *
  * (1 / feat[7]) (the difference in binary64); sqrt((u0_0^2 + u0_1^2) + u0_2^2) /
  * sqrt(3) * 2 - 1; u0_2.  The forward pass runs on the f16 matrix cores: weights,
- * biases, features and activations rounded to f16, products accumulated in FP32,
- * tanh from the hardware exp2 and reciprocal; its y agrees with sdeh_mlp_forward's
- * FP32 pass to a few 1e-3 (on the trained Lorenz model: largest difference from a
- * binary64 forward pass about 2e-3 over 65,536 samples, median 3e-4).  Then pred =
+ * layer 1's biases, features and activations rounded to f16, products accumulated
+ * in FP32, layer 2's and 3's biases added in FP32; its y agrees with
+ * sdeh_mlp_forward's FP32 pass to a few 1e-3.  Then pred =
  * y * y_norm[2..3] + y_norm[0..1] (FP32: std, then mean), key[i] = -pred_0 (an
  * ascending sort of key is the hand-out order, largest predicted cost first) and
  * dt0[i] = exp(pred_1) in FP32, widened.  pred [n][2] may be NULL. */

@@ -1,6 +1,6 @@
 """ORACLE (cost-model MLP) — TEST INFRASTRUCTURE ONLY.
 
-Float64 numpy restatement of the 6 -> 64 -> 64 -> 2 tanh MLP and its mean
+Float64 numpy restatement of the 6 -> 64 -> 64 -> 2 ReLU MLP and its mean
 squared error gradient, in the parameter layout of include/simplediffeq_hip.h
 (W1[64][6], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]).  Used to check the
 device forward/backward kernels and one Adam step.
@@ -24,8 +24,8 @@ def unpack(w):
 
 def forward(w, x):
     W1, b1, W2, b2, W3, b3 = unpack(w)
-    h1 = np.tanh(x @ W1.T + b1)
-    h2 = np.tanh(h1 @ W2.T + b2)
+    h1 = np.maximum(x @ W1.T + b1, 0.0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0.0)
     return h2 @ W3.T + b3, (h1, h2)
 
 
@@ -40,10 +40,10 @@ def loss_grad(w, x, y):
     g = 2.0 * e / (n * NOUT)
     gW3 = g.T @ h2
     gb3 = g.sum(axis=0)
-    d2 = (g @ W3) * (1 - h2 ** 2)
+    d2 = (g @ W3) * (h2 > 0)  # ReLU derivative, 0 at 0
     gW2 = d2.T @ h1
     gb2 = d2.sum(axis=0)
-    d1 = (d2 @ W2) * (1 - h1 ** 2)
+    d1 = (d2 @ W2) * (h1 > 0)
     gW1 = d1.T @ x
     gb1 = d1.sum(axis=0)
     grad = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])

@@ -1,4 +1,4 @@
-// Learned cost model kernels: a 6 -> 64 -> 64 -> 2 MLP (tanh hidden layers,
+// Learned cost model kernels: a 6 -> 64 -> 64 -> 2 MLP (ReLU hidden layers,
 // linear output, FP32) trained on device.  It predicts, from a trajectory's
 // parameters, initial state and horizon, the log of its accepted-step count
 // and the log of a good initial dt; the runtime's caller uses the predictions
@@ -36,22 +36,13 @@ __device__ inline void load_params(float* sw, const float* __restrict__ params) 
 // k = lane / 32 for K-step m).  Layer 2 (64 x 64) is 2 x 32 MFMAs per tile: D tile t
 // (units 32t .. 32t+31) = sum over m of A_m x B_m with B_m[k][j] = W2[32t + j][2m + k],
 // held in 64 registers per lane.  D register r of lane l holds sample row
-// 8(r/4) + 4(l/32) + r%4 and unit l % 32 (+32 t); after tanh the tile goes through
-// LDS (rows padded to 65 floats) so that layer 3 (64 -> 2) reads each sample's row.
-// Inference uses a fast tanh (1 - 2/(e^{2x} + 1) with the hardware exp2 and
-// reciprocal; absolute error about 1e-7); training keeps tanhf.
+// 8(r/4) + 4(l/32) + r%4 and unit l % 32 (+32 t); after the ReLU the tile goes
+// through LDS (rows padded to 65 floats) so that layer 3 (64 -> 2) reads each
+// sample's row.
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 constexpr int H2P = NH + 1;
 
-__device__ __forceinline__ float tanh_fast(float x) {
-  const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);  // e^{2x}
-  return fmaf(-2.0f, __builtin_amdgcn_rcpf(e + 1.0f), 1.0f);
-}
-
-// tanh(x) from a = 2 x log2(e), the scaling already applied by the caller.
-__device__ __forceinline__ float tanh_prescaled(float a) {
-  return fmaf(-2.0f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(a) + 1.0f), 1.0f);
-}
+__device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.0f); }
 
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -98,140 +89,195 @@ struct YPlan {
 
 // ---- the binning step on the f16 matrix cores -------------------------------
 // sdeh_mlp_plan_lorenz's forward pass: the same network with its weights and
-// activations rounded to f16 and every product accumulated in FP32 on
-// v_mfma_f32_32x32x8_f16 (K = 8 per instruction).  Each layer is computed
+// hidden activations rounded to f16 and every product accumulated in FP32 on
+// v_mfma_f32_32x32x16_f16 (K = 16 per instruction).  Each layer is computed
 // transposed, D^T = W X^T (M = units, N = 32 samples), because the MFMA output
-// layout then hands the next layer its B operand without moving data: lane l holds
-// sample l % 32 and, in register r of output tile t, unit 32 t + 8 (r / 4) +
-// 4 (l / 32) + r % 4, which is exactly the B operand (rows k = 8 m + 4 (l / 32) + i,
-// column l % 32) of the next layer's K step m = 4 t + r / 4, i = r % 4.  Biases are
-// one more K step against a constant 1 input (layer 1: feature column 6), and layer
-// 3 (two outputs) is rows 0 and 1 of a padded 32-row tile.  The six features of a
-// sample are split between its two lanes: lane l < 32 computes the parameter and
-// horizon features (K rows 0-3), lane l + 32 the initial-state ones (rows 4-5), the
-// bias input 1 (row 6) and a zero (row 7).  Activations use the hardware exp and
-// reciprocal (tanh = 1 - 2 / (e^2x + 1), absolute error about 1e-7 before the f16
-// rounding), with the factor 2 log2(e) of the exponent folded into the layer 1 and
-// layer 2 weights and biases before they are rounded to f16, so that each activation
-// is exp2, add, reciprocal and fma.  The predictions only order and seed the solves;
+// layout then hands the next layer its B operand without moving data: in output
+// tile t, register r of lane l holds sample l % 32 and unit
+// 32 t + 8 (r / 4) + 4 (l / 32) + r % 4.  The next layer's K step m takes registers
+// 8 (m % 2) .. 8 (m % 2) + 7 of tile m / 2, so element e of lane half q stands for
+// unit 16 m + 8 (e / 4) + 4 q + e % 4 (unit_of_k).  The weights, the A operand, are
+// loaded under the same labelling of K; the product only needs both operands to
+// agree, since the hardware pairs element e of lane half q of A with the same
+// element of B.  Layer 1 has one K step: lane half 0 holds the parameter and horizon
+// features (elements 0-3), lane half 1 the initial-state ones (elements 0, 1) and a
+// constant 1 (element 2) against the biases b1.  Layer 2's biases are the
+// accumulator's initial value, in the output layout.  Layer 3 (two outputs) is rows
+// 0 and 1 of a 32-row tile, its biases added in FP32.  The activations are max(x, 0)
+// of the f16-rounded sums.  The predictions only order and seed the solves;
 // sdeh_mlp_forward keeps the FP32 forward pass.
-typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __fp16 fp16x2 __attribute__((ext_vector_type(2)));
 
-// Four floats as f16 (rounded toward zero, two per v_cvt_pkrtz_f16_f32).
-__device__ __forceinline__ half4 pack4(float a, float b, float c, float d) {
-  const fp16x2 lo = __builtin_amdgcn_cvt_pkrtz(a, b), hi = __builtin_amdgcn_cvt_pkrtz(c, d);
-  half4 r;
-  r[0] = (_Float16)lo[0];
-  r[1] = (_Float16)lo[1];
-  r[2] = (_Float16)hi[0];
-  r[3] = (_Float16)hi[1];
+// Eight floats as f16 (rounded toward zero, two per v_cvt_pkrtz_f16_f32).
+__device__ __forceinline__ half8 pack8(const float (&v)[8]) {
+  half8 r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const fp16x2 h = __builtin_amdgcn_cvt_pkrtz(v[2 * i], v[2 * i + 1]);
+    r[2 * i] = (_Float16)h[0];
+    r[2 * i + 1] = (_Float16)h[1];
+  }
   return r;
 }
 
-// Features of sample smp in FP32 for lane half q (see above): q = 0: (p_c - f[2c]) /
-// f[2c+1] for c = 0, 1, 2 and (log(tf - t0) - f[6]) / f[7]; q = 1: |u0| / sqrt(3) *
-// 2 - 1, u0_2, 1, 0.
+// max(x, 0) of registers b .. b + 7 of an output tile, as a B operand: the max is
+// taken after the rounding, two values per v_pk_max_f16 (the same result as
+// rounding max(x, 0): rounding toward zero keeps the sign and 0 stays 0).
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ half8 relu_pack8(const floatx16& d, int b) {
+  const half2v z = {(_Float16)0.f, (_Float16)0.f};
+  half8 r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const fp16x2 h = __builtin_amdgcn_cvt_pkrtz(d[b + 2 * i], d[b + 2 * i + 1]);
+    const half2v m = __builtin_elementwise_max(half2v{(_Float16)h[0], (_Float16)h[1]}, z);
+    r[2 * i] = m[0];
+    r[2 * i + 1] = m[1];
+  }
+  return r;
+}
+
+// The hidden unit that element e of lane half q stands for in K step m.
+__device__ __forceinline__ constexpr int unit_of_k(int m, int q, int e) {
+  return 16 * m + 8 * (e >> 2) + 4 * q + (e & 3);
+}
+
+// Features of a sample for lane half q.  load() reads the binary64 inputs (q = 0:
+// p_0, p_1, p_2, t0, tf; q = 1: u0_0, u0_1, u0_2) of sample min(smp, n - 1), so a
+// tile's tail and the prefetches past the last tile read in bounds; operator() makes
+// the FP32 features and the f16 K step: q = 0: (p_c - off[c]) * inv[c] for c = 0, 1, 2
+// and (log(tf - t0) - off[3]) * inv[3]; q = 1: |u0| / sqrt(3) * 2 - 1, u0_2, 1.
 struct XLorenzPlan {
   const double* __restrict__ p;
   const double* __restrict__ u0;
   const double* __restrict__ ts;
   int64_t n;
   float off[4], inv[4];
-  __device__ half4 operator()(int64_t smp, bool valid, int q) const {
-    if (!valid) return pack4(0.f, 0.f, q ? 1.f : 0.f, 0.f);
+  struct Raw {
+    double v[5];
+  };
+  __device__ Raw load(int64_t smp, int q) const {
+    const int64_t s = smp < n ? smp : n - 1;
+    const double* a = q ? u0 : p;
+    Raw r;
+    r.v[0] = a[s];
+    r.v[1] = a[n + s];
+    r.v[2] = a[2 * n + s];
+    r.v[3] = ts[s];  // used by lane half 0 only; loaded by both, same lines
+    r.v[4] = ts[n + s];
+    return r;
+  }
+  __device__ half8 operator()(const Raw& r, int q) const {
+    float v[8] = {};
     if (q == 0) {
-      const float x0 = ((float)p[smp] - off[0]) * inv[0];
-      const float x1 = ((float)p[n + smp] - off[1]) * inv[1];
-      const float x2 = ((float)p[2 * n + smp] - off[2]) * inv[2];
-      const float x3 = (__logf((float)(ts[n + smp] - ts[smp])) - off[3]) * inv[3];
-      return pack4(x0, x1, x2, x3);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = ((float)r.v[c] - off[c]) * inv[c];
+      v[3] = (__logf((float)(r.v[4] - r.v[3])) - off[3]) * inv[3];
+    } else {
+      const float a = (float)r.v[0], b = (float)r.v[1], z = (float)r.v[2];
+      const float rr = __builtin_sqrtf((a * a + b * b) + z * z);
+      v[0] = rr * 1.1547005383792515f - 1.f;
+      v[1] = z;
+      v[2] = 1.f;
     }
-    const float a = (float)u0[smp], b = (float)u0[n + smp], z = (float)u0[2 * n + smp];
-    const float r = __builtin_sqrtf((a * a + b * b) + z * z);
-    return pack4(r * 1.1547005383792515f - 1.f, z, 1.f, 0.f);
+    return pack8(v);
   }
 };
 
 template <class Sink>
-__global__ void __launch_bounds__(256) mlp_plan_f16(const float* __restrict__ params,
-                                                    XLorenzPlan src, Sink sink, int64_t n) {
+__global__ void __launch_bounds__(256, 2) mlp_plan_f16(const float* __restrict__ params,
+                                                       XLorenzPlan src, Sink sink, int64_t n) {
   __shared__ float sw[NPARAM];
+  // layer 3's A operand, f16, per K step m and lane half q: rows 0 and 1 (W3) and a
+  // zero row 2 that lanes 2..31 read
+  __shared__ half8 sw3[4][2][3];
   load_params(sw, params);
+  if (threadIdx.x < 24) {
+    const int m = threadIdx.x / 6, hq = (threadIdx.x / 3) & 1, row = threadIdx.x % 3;
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = row < NOUT ? sw[OFF_W3 + row * NH + unit_of_k(m, hq, e)] : 0.f;
+    sw3[m][hq][row] = pack8(v);
+  }
+  __syncthreads();
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int li = l & 31, q = l >> 5;
-  // A operands (weights, fixed per kernel): lane l holds row li of the tile, K columns
-  // 4q .. 4q + 3 of each K step; step 8 of layers 2 and 3 carries the bias (column 0).
-  // Layers 1 and 2 are scaled by c = 2 log2(e): their outputs are the exponents of
-  // tanh_prescaled.
-  constexpr float c = 2.8853900817779268f;
-  half4 w1[2], w2[2][9], w3[9];
+  // A operands of layers 1 and 2 (weights, fixed per kernel): lane l holds row li of
+  // the tile and the eight K elements of lane half q; layer 2's biases (FP32) in the
+  // output layout, the accumulator's initial value.
+  half8 w1[2], w2[2][4];
+  floatx16 b2[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int u = 32 * t + li;
-    float v[4];
+    float v[8];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int k = 4 * q + i;
-      v[i] = c * (k < NIN ? sw[OFF_W1 + u * NIN + k] : (k == NIN ? sw[OFF_B1 + u] : 0.f));
+    for (int e = 0; e < 8; ++e) {
+      const int f = q == 0 ? (e < 4 ? e : -1) : (e < 2 ? 4 + e : (e == 2 ? NIN : -1));
+      v[e] = f < 0 ? 0.f : (f < NIN ? sw[OFF_W1 + u * NIN + f] : sw[OFF_B1 + u]);
     }
-    w1[t] = pack4(v[0], v[1], v[2], v[3]);
+    w1[t] = pack8(v);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const float* w = sw + OFF_W2 + u * NH + 8 * m + 4 * q;
-      w2[t][m] = pack4(c * w[0], c * w[1], c * w[2], c * w[3]);
+    for (int m = 0; m < 4; ++m) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = sw[OFF_W2 + u * NH + unit_of_k(m, q, e)];
+      w2[t][m] = pack8(v);
     }
-    w2[t][8] = pack4(q == 0 ? c * sw[OFF_B2 + u] : 0.f, 0.f, 0.f, 0.f);
-  }
 #pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    const int k = 8 * m + 4 * q;
-    const float* w = sw + OFF_W3 + (li < NOUT ? li : 0) * NH + k;
-    w3[m] = li < NOUT ? pack4(w[0], w[1], w[2], w[3]) : pack4(0.f, 0.f, 0.f, 0.f);
+    for (int r = 0; r < 16; ++r) b2[t][r] = sw[OFF_B2 + 32 * t + 8 * (r >> 2) + 4 * q + (r & 3)];
   }
-  w3[8] = pack4(li < NOUT && q == 0 ? sw[OFF_B3 + (li < NOUT ? li : 0)] : 0.f, 0.f, 0.f, 0.f);
-  const half4 one = pack4(q == 0 ? 1.f : 0.f, 0.f, 0.f, 0.f);  // the bias step's input row
+  const int row3 = li < NOUT ? li : NOUT;
+  const float b3a = sw[OFF_B3], b3b = sw[OFF_B3 + 1];
   const floatx16 zero = {};
-  const int64_t tiles = (n + 31) / 32;
-  for (int64_t tile = (int64_t)blockIdx.x * 4 + wv; tile < tiles;
-       tile += (int64_t)gridDim.x * 4) {
+  const int64_t tiles = (n + 31) / 32, stride = (int64_t)gridDim.x * 4;
+  auto tile_pass = [&](const XLorenzPlan::Raw& in, int64_t tile) {
     const int64_t smp = tile * 32 + li;
-    const bool valid = smp < n;
-    const half4 x = src(smp, valid, q);
-    // layer 1: two 32-unit tiles
+    const half8 x = src(in, q);
     floatx16 d1[2];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) d1[t] = __builtin_amdgcn_mfma_f32_32x32x8f16(w1[t], x, zero, 0, 0, 0);
-    half4 h1[8];
+    for (int t = 0; t < 2; ++t) d1[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(w1[t], x, zero, 0, 0, 0);
+    half8 h1[4];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const floatx16& d = d1[m >> 2];
-      const int r = 4 * (m & 3);
-      h1[m] = pack4(tanh_prescaled(d[r]), tanh_prescaled(d[r + 1]), tanh_prescaled(d[r + 2]),
-                    tanh_prescaled(d[r + 3]));
-    }
-    // layer 2: two output tiles, eight K steps and the bias step each
+    for (int m = 0; m < 4; ++m) h1[m] = relu_pack8(d1[m >> 1], 8 * (m & 1));
     floatx16 d2[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      floatx16 acc = __builtin_amdgcn_mfma_f32_32x32x8f16(w2[t][8], one, zero, 0, 0, 0);
+      floatx16 acc = b2[t];
 #pragma unroll
-      for (int m = 0; m < 8; ++m) acc = __builtin_amdgcn_mfma_f32_32x32x8f16(w2[t][m], h1[m], acc, 0, 0, 0);
+      for (int m = 0; m < 4; ++m) acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(w2[t][m], h1[m], acc, 0, 0, 0);
       d2[t] = acc;
     }
-    half4 h2[8];
+    half8 h2[4];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const floatx16& d = d2[m >> 2];
-      const int r = 4 * (m & 3);
-      h2[m] = pack4(tanh_prescaled(d[r]), tanh_prescaled(d[r + 1]), tanh_prescaled(d[r + 2]),
-                    tanh_prescaled(d[r + 3]));
-    }
+    for (int m = 0; m < 4; ++m) h2[m] = relu_pack8(d2[m >> 1], 8 * (m & 1));
     // layer 3: output rows 0 and 1 land in registers 0 and 1 of lanes 0..31
-    floatx16 d3 = __builtin_amdgcn_mfma_f32_32x32x8f16(w3[8], one, zero, 0, 0, 0);
+    floatx16 d3 = zero;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) d3 = __builtin_amdgcn_mfma_f32_32x32x8f16(w3[m], h2[m], d3, 0, 0, 0);
-    if (q == 0 && valid) sink(smp, d3[0], d3[1]);
+    for (int m = 0; m < 4; ++m)
+      d3 = __builtin_amdgcn_mfma_f32_32x32x16_f16(sw3[m][q][row3], h2[m], d3, 0, 0, 0);
+    if (q == 0 && smp < n) sink(smp, d3[0] + b3a, d3[1] + b3b);
+  };
+  // Three input buffers: while the wave computes a tile, the inputs of its next two
+  // tiles are in flight.  The loop is unrolled by three so that each load writes the
+  // buffer it refills (a register copy would wait for the load).
+  int64_t tile = (int64_t)blockIdx.x * 4 + wv;
+  XLorenzPlan::Raw r0 = src.load(tile * 32 + li, q);
+  XLorenzPlan::Raw r1 = src.load((tile + stride) * 32 + li, q);
+  XLorenzPlan::Raw r2 = src.load((tile + 2 * stride) * 32 + li, q);
+  for (;;) {
+    if (tile >= tiles) break;
+    tile_pass(r0, tile);
+    r0 = src.load((tile + 3 * stride) * 32 + li, q);
+    tile += stride;
+    if (tile >= tiles) break;
+    tile_pass(r1, tile);
+    r1 = src.load((tile + 3 * stride) * 32 + li, q);
+    tile += stride;
+    if (tile >= tiles) break;
+    tile_pass(r2, tile);
+    r2 = src.load((tile + 3 * stride) * 32 + li, q);
+    tile += stride;
   }
 }
 
@@ -264,7 +310,7 @@ __global__ void __launch_bounds__(256) mlp_forward_mfma(const float* __restrict_
       float z = sw[OFF_B1 + k];
 #pragma unroll
       for (int j = 0; j < NIN; ++j) z = fmaf(sw[OFF_W1 + k * NIN + j], xi[j], z);
-      h1[m] = tanh_fast(z);
+      h1[m] = relu(z);
     }
     floatx16 acc0 = {}, acc1 = {};
 #pragma unroll
@@ -276,8 +322,8 @@ __global__ void __launch_bounds__(256) mlp_forward_mfma(const float* __restrict_
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = 8 * (r >> 2) + 4 * q + (r & 3);
-      hs[row * H2P + li] = tanh_fast(acc0[r] + b2a);
-      hs[row * H2P + 32 + li] = tanh_fast(acc1[r] + b2b);
+      hs[row * H2P + li] = relu(acc0[r] + b2a);
+      hs[row * H2P + 32 + li] = relu(acc1[r] + b2b);
     }
     wave_sync_lds();
     float o0 = 0.f, o1 = 0.f;
@@ -303,8 +349,8 @@ __global__ void __launch_bounds__(256) mlp_forward_mfma(const float* __restrict_
 //   backward D1[s][k] = sum_j D2[s][j] W2[j][k]      (M = samples, K = units)
 //   gradient gW2[j][k] += sum_s D2[s][j] H1[s][k]    (M = units, K = samples)
 // H1, H2, D2 and the output gradients of a tile pass through the wave's LDS tile
-// to change layout.  Layers 1 and 3, the biases, the loss and the tanh derivative
-// are vector work.  The wave's gradient (and squared-error sum) goes to
+// to change layout.  Layers 1 and 3, the biases, the loss and the ReLU derivative
+// (1 where the unit's output is positive, else 0) are vector work.  The wave's gradient (and squared-error sum) goes to
 // partial[blockIdx.x]; mlp_grad_reduce adds the waves in a fixed order.
 struct GradTile {
   float x[32 * NIN];
@@ -364,7 +410,7 @@ __global__ void __launch_bounds__(64) mlp_grad_mfma(const float* __restrict__ pa
       float z = sw[OFF_B1 + k];
 #pragma unroll
       for (int j = 0; j < NIN; ++j) z = fmaf(sw[OFF_W1 + k * NIN + j], xi[j], z);
-      h1a[m] = tanhf(z);
+      h1a[m] = relu(z);
       L.h1[li * H2P + k] = h1a[m];
     }
     floatx16 z0 = {}, z1 = {};
@@ -376,8 +422,8 @@ __global__ void __launch_bounds__(64) mlp_grad_mfma(const float* __restrict__ pa
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = 8 * (r >> 2) + 4 * q + (r & 3);
-      L.h2[row * H2P + li] = tanhf(z0[r] + b2v[0]);
-      L.h2[row * H2P + 32 + li] = tanhf(z1[r] + b2v[1]);
+      L.h2[row * H2P + li] = relu(z0[r] + b2v[0]);
+      L.h2[row * H2P + 32 + li] = relu(z1[r] + b2v[1]);
     }
     wave_sync_lds();
     // layer 3, error and output gradient of sample li (the two halves add up)
@@ -403,7 +449,7 @@ __global__ void __launch_bounds__(64) mlp_grad_mfma(const float* __restrict__ pa
       gl = fmaf(e1, e1, gl);
     }
     wave_sync_lds();
-    // D2 = (W3^T g) (1 - H2^2) in the D layout (rows: samples, column: unit)
+    // D2 = (W3^T g) [H2 > 0] in the D layout (rows: samples, column: unit)
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -411,14 +457,14 @@ __global__ void __launch_bounds__(64) mlp_grad_mfma(const float* __restrict__ pa
         const int row = 8 * (r >> 2) + 4 * q + (r & 3);
         const float g0 = L.g[row * NOUT], g1 = L.g[row * NOUT + 1];
         const float h = L.h2[row * H2P + 32 * t + li];
-        const float d = fmaf(w3c[0][t], g0, w3c[1][t] * g1) * (1.f - h * h);
+        const float d = h > 0.f ? fmaf(w3c[0][t], g0, w3c[1][t] * g1) : 0.f;
         L.d2[row * H2P + 32 * t + li] = d;
         gb2[t] += d;
         gw3[0][t] = fmaf(g0, h, gw3[0][t]);
         gw3[1][t] = fmaf(g1, h, gw3[1][t]);
       }
     wave_sync_lds();
-    // D1 = D2 W2 on the matrix cores, then times (1 - H1^2)
+    // D1 = D2 W2 on the matrix cores, then times [H1 > 0]
     floatx16 d0 = {}, d1 = {};
 #pragma unroll
     for (int m = 0; m < 32; ++m) {
@@ -434,7 +480,7 @@ __global__ void __launch_bounds__(64) mlp_grad_mfma(const float* __restrict__ pa
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         const float h = L.h1[row * H2P + 32 * t + li];
-        const float dd = (t == 0 ? d0[r] : d1[r]) * (1.f - h * h);
+        const float dd = h > 0.f ? (t == 0 ? d0[r] : d1[r]) : 0.f;
         gb1[t] += dd;
 #pragma unroll
         for (int j = 0; j < NIN; ++j) gw1[t][j] = fmaf(dd, L.x[row * NIN + j], gw1[t][j]);

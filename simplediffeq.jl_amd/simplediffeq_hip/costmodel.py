@@ -1,6 +1,6 @@
 """Learned cost model for adaptive Lorenz ensembles.
 
-A 6 -> 64 -> 64 -> 2 MLP (FP32, tanh hidden layers) predicts, per trajectory,
+A 6 -> 64 -> 64 -> 2 MLP (FP32, ReLU hidden layers) predicts, per trajectory,
   y0 = log(accepted steps)    and    y1 = log(mean accepted step size)
 from six features of (sigma, rho, beta, u0, t_end).  It is trained on step
 logs (the naccept counters) of the plain adaptive solver, entirely on the GPU
@@ -74,12 +74,15 @@ def targets(naccept, tspan):
 
 
 def init_params(seed=0):
-    """Glorot-uniform weights, zero biases (layout of include/simplediffeq_hip.h)."""
+    """Uniform weights, zero biases (layout of include/simplediffeq_hip.h): He's
+    limit sqrt(6 / fan_in) for the two ReLU layers, Glorot's sqrt(6 / (fan_in +
+    fan_out)) for the linear output layer."""
     rng = np.random.default_rng(seed)
     w = np.zeros(NPARAM, dtype=np.float32)
     o = 0
     for fan_in, fan_out in ((NIN, NH), (NH, NH), (NH, NOUT)):
-        lim = math.sqrt(6.0 / (fan_in + fan_out))
+        relu_layer = fan_out == NH
+        lim = math.sqrt(6.0 / (fan_in if relu_layer else fan_in + fan_out))
         w[o:o + fan_in * fan_out] = rng.uniform(-lim, lim, fan_in * fan_out)
         o += fan_in * fan_out + fan_out
     return w

@@ -21,6 +21,7 @@ import json
 import numpy as np
 import pytest
 
+from oracle import mlp_ref
 from simplediffeq_hip import _abi, costmodel, inputs, runtime
 
 pytestmark = pytest.mark.gpu
@@ -79,6 +80,25 @@ def _time(ctx, S, d, g):
 
 def _plan(S, cm):
     return cm.plan_lorenz(S["pd"], S["u0d"], S["tsd"], S["stream"], pred=False)
+
+
+def test_trained_plan_against_float64_network(ctx, setup):
+    """The f16 planning pass of the trained model (sdeh_mlp_plan_lorenz) against the
+    float64 reference network (oracle/mlp_ref.py) on the same features, first 65,536
+    evaluation trajectories: de-normalised predictions within a few 1e-3."""
+    S = setup
+    torch, cm = S["torch"], S["cm"]
+    pred, _, _ = cm.plan_lorenz(S["pd"], S["u0d"], S["tsd"], S["stream"], pred=True)
+    torch.cuda.synchronize()
+    pf = pred.cpu().numpy()[:N_ERR].astype(np.float64)
+    x = costmodel.features(S["p"][:, :N_ERR], S["u0"][:, :N_ERR], S["ts"][:, :N_ERR])
+    ref, _ = mlp_ref.forward(cm.params.cpu().numpy().astype(np.float64), x.astype(np.float64))
+    ref = ref * cm.y_std.astype(np.float64) + cm.y_mean.astype(np.float64)
+    dev = np.abs(pf - ref)
+    print(json.dumps({"trained_plan_vs_float64": {
+        "max": float(dev.max()), "median": float(np.median(dev)),
+        "p99": float(np.percentile(dev, 99))}}), flush=True)
+    assert np.median(dev) < 2e-3 and dev.max() < 2e-2
 
 
 def test_trained_binning_meets_target(ctx, setup):
