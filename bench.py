@@ -475,7 +475,7 @@ def lorenz_wide(args, ctx, stream, steps, warmup):
                                            {"attempts_per_launch": st["attempts"]}),
                      "error_vs_avern9_1e-14": ref.error(g),
                      "launch": {k: st[k] for k in ("block", "grid", "chunk", "schedule", "waves",
-                                                   "per_lane")}}
+                                                   "per_lane", "refill")}}
         if name == "f32":
             out[name]["roofline"]["frac_of_78.6_unpacked"] = out[name]["roofline"]["achieved"] / 78.6
         del g
@@ -532,7 +532,8 @@ def headline(args):
     shape = {}
     if args.shape:
         sh = [int(v) for v in args.shape.split(",")]
-        shape = dict(sched=sh[0], block=sh[1], grid=sh[2], chunk=sh[3])
+        shape = dict(sched=sh[0], block=sh[1], grid=sh[2], chunk=sh[3],
+                     refill=sh[4] if len(sh) > 4 else 0)
     d64, g64 = bufs(**(shape or dict(sched=args.schedule)))
     if args.tune and not shape:
         ctx.autotune(d64, g64)
@@ -543,7 +544,8 @@ def headline(args):
     att = g64.numpy("naccept").astype(np.int64) + g64.numpy("nreject")
     err64 = ref.error(g64)
     sched64 = st64["schedule"]
-    launch64 = {k: st64[k] for k in ("block", "grid", "chunk", "schedule", "waves", "per_lane")}
+    launch64 = {k: st64[k] for k in ("block", "grid", "chunk", "schedule", "waves", "per_lane",
+                                     "refill")}
     del g64
     _progress("binary64 timed: %.3f ms per solve, launch %s" % (ms64, launch64))
     # the autotuner's kernel variants on the same solve (the tuner's choice is above)
@@ -572,7 +574,7 @@ def headline(args):
                                                    st32["attempts"], st32["accepts"], n),
                                  ms32, PEAK_TFLOPS["f32"], _kernel_name("f32", st32["schedule"])),
            "launch": {k: st32[k] for k in ("block", "grid", "chunk", "schedule", "waves",
-                                           "per_lane")}}
+                                           "per_lane", "refill")}}
     del g32
     _progress("binary32 timed")
 
@@ -617,7 +619,8 @@ def headline(args):
             st = solve_once(d, g)
             ms = time_of(d, g, reps)
             if best is None or ms < best[0]:
-                best = (ms, st, {k: st[k] for k in ("block", "grid", "waves", "per_lane")})
+                best = (ms, st, {k: st[k] for k in ("block", "grid", "waves", "per_lane",
+                                                    "refill")})
             if shape != shapes[-1]:
                 del g
         ms, st, launch = best
@@ -758,9 +761,9 @@ def main():
                          "own JSON")
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--shape", default="",
-                    help="SCHEDULE,BLOCK,GRID,CHUNK: the binary64 launch of the default line, "
-                         "instead of the autotuner's (to repeat a measured shape, e.g. under "
-                         "counters)")
+                    help="SCHEDULE,BLOCK,GRID,CHUNK[,REFILL]: the binary64 launch of the default "
+                         "line, instead of the autotuner's (to repeat a measured shape, e.g. "
+                         "under counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; --gpus %d is not supported "

@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define SDEH_ABI_VERSION 4
+#define SDEH_ABI_VERSION 5
 
 typedef enum sdeh_status {
   SDEH_OK = 0,
@@ -198,7 +198,15 @@ typedef struct sdeh_solve_desc {
                                AVern7 (binary64) endpoint kernels and for registered
                                right-hand sides; one-launch and persistent schedules (a
                                staged shape runs persistent).  Same results.  (ABI 4) */
-  int32_t reserved2;
+  int32_t refill;           /* persistent and staged schedules: a wave takes new
+                               trajectories only once at least `refill` of its
+                               trajectory slots are free, so that the loads that start
+                               them are waited for once per batch.  0 = tuned (else 4
+                               with one lane per trajectory, 1 with several); values
+                               above the wave's slot count (64 / lanes) act as that
+                               count; the wave-layout and two-per-lane kernels refill
+                               as soon as a slot is free.  Same results.  (ABI 5; was
+                               reserved, 0.) */
 } sdeh_solve_desc;
 
 /* Solve buffers.  With memory == SDEH_MEM_HOST these are host pointers and the
@@ -250,7 +258,7 @@ typedef struct sdeh_stats {
   int64_t steps_needed;    /* SDEH_SAVE_PACKED: total points offered by all
                               trajectories (the steps_capacity that holds them all) */
   int32_t per_lane;        /* trajectories per lane used */
-  int32_t reserved1;
+  int32_t refill;          /* refill batch used (0: one launch) (ABI 5) */
 } sdeh_stats;
 
 typedef struct sdeh_ctx sdeh_ctx;

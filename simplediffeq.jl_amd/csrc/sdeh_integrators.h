@@ -1378,6 +1378,12 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
   for (;;) {
     if (a.persistent) {
       bool want = !active && lead;
+      // a batch of at least refill_min free slots (at most the wave's slot count, so
+      // that a wave whose lanes are all free always claims)
+      if (a.refill_min > 1) {
+        const int kmin = a.refill_min < 64 / R::G ? a.refill_min : 64 / R::G;
+        if (__popcll(__ballot(want)) < kmin) want = false;
+      }
       for (;;) {
         int64_t slot = wave_claim(a, want, qbase, qlim, drained);
         if constexpr (R::G > 1)  // the group leader claimed; its group follows

@@ -64,7 +64,10 @@ struct sdeh_kargs {
   double* st_key;
   int64_t st_stride;
   int32_t resume;
-  int32_t st_pad_;
+  // Persistent schedule: a wave claims new trajectories only once at least
+  // refill_min of its trajectory slots are free (0 or 1: as soon as one is), so that
+  // the loads that start them are waited for once per batch.
+  int32_t refill_min;
   // Optional [n] debug counters: a lane that takes trajectory i up in a launch
   // (a fresh start, or a staged solve resuming it) adds 1 to visits[i].
   int32_t* visits;

@@ -6,7 +6,7 @@ Only plain ctypes: the structs below must match the C header field for field.
 import ctypes
 import os
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # sdeh_status
 OK = 0
@@ -79,7 +79,7 @@ class SolveDesc(ctypes.Structure):
         ("span_order", ctypes.c_int32),
         ("waves", ctypes.c_int32),
         ("per_lane", ctypes.c_int32),
-        ("reserved2", ctypes.c_int32),
+        ("refill", ctypes.c_int32),
     ]
 
 
@@ -121,7 +121,7 @@ class Stats(ctypes.Structure):
         ("waves", ctypes.c_int32),
         ("steps_needed", ctypes.c_int64),
         ("per_lane", ctypes.c_int32),
-        ("reserved1", ctypes.c_int32),
+        ("refill", ctypes.c_int32),
     ]
 
 

@@ -25,7 +25,7 @@ def make_desc(alg, dtype=_abi.F64, rhs=_abi.RHS_LORENZ, n_traj=0, tspan=(0.0, 1.
               abstol=None, reltol=None, saveat=None, save_everystep=False, max_saves=0,
               maxiters=0, quirks=True, schedule=_abi.SCHED_AUTO, precision=_abi.PREC_PLAIN,
               block=0, grid=0, chunk=0, lanes=0, save_layout=_abi.SAVE_POINT_MAJOR,
-              span_order=0, waves=0, per_lane=0):
+              span_order=0, waves=0, per_lane=0, refill=0):
     """Build an sdeh_solve_desc.  Unset dt/abstol/reltol take the reference
     defaults (0.1f0, 1f-6, 1f-3).  lanes: lanes per trajectory (0 = for
     binary64 the right-hand side's matrix-core layout, else its group layout,
@@ -33,8 +33,8 @@ def make_desc(alg, dtype=_abi.F64, rhs=_abi.RHS_LORENZ, n_traj=0, tspan=(0.0, 1.
     u_saveat/u_steps/t_steps (point- or trajectory-major, or packed: save_everystep
     rows of all trajectories back to back, no per-trajectory cap).  waves: the
     register-cap variant (0 = tuned or default; 4, 6, 8 waves per SIMD).  per_lane:
-    trajectories per lane (0 = tuned or 1; 2).  Keeps the saveat array alive on the
-    struct."""
+    trajectories per lane (0 = tuned or 1; 2).  refill: the persistent schedules'
+    refill batch (0 = tuned or default).  Keeps the saveat array alive on the struct."""
     d = _abi.default_desc()
     d.alg = int(alg)
     d.dtype = int(dtype)
@@ -59,6 +59,7 @@ def make_desc(alg, dtype=_abi.F64, rhs=_abi.RHS_LORENZ, n_traj=0, tspan=(0.0, 1.
     d.span_order = int(span_order)
     d.waves = int(waves)
     d.per_lane = int(per_lane)
+    d.refill = int(refill)
     if saveat is not None and len(saveat) > 0:
         arr = np.ascontiguousarray(np.asarray(saveat, dtype=np.float64))
         d._saveat_keep = arr

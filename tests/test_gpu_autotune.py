@@ -4,7 +4,8 @@ trajectories per lane (a variant whose lanes each integrate two trajectories,
 their stages interleaved).  Every variant runs the same per-trajectory arithmetic
 as the default kernel, so it gives the same bits, and the binary64 ones equal the
 oracle; the tuner times them after the launch shape and later solves use its
-choice (reported in stats.waves and stats.per_lane)."""
+choice (reported in stats.waves and stats.per_lane).  So does the persistent
+schedules' refill batch (stats.refill)."""
 import numpy as np
 import pytest
 
@@ -21,12 +22,12 @@ CASES = [(_abi.ALG_ATSIT5, _abi.F64, _abi.PREC_PLAIN), (_abi.ALG_ATSIT5, _abi.F3
 IDS = ["atsit5-f64", "atsit5-f32", "atsit5-f32kahan", "avern7-f64", "avern7-f32"]
 
 
-def _run(ctx, alg, dtype, prec, waves, sched, n=30_001, seed=3, per_lane=0):
+def _run(ctx, alg, dtype, prec, waves, sched, n=30_001, seed=3, per_lane=0, refill=0):
     u0, p, ts = inputs.lorenz_uneven(n, seed=seed)
     ts[1] = np.minimum(ts[1], 5.0)
     d = runtime.make_desc(alg, dtype=dtype, rhs=_abi.RHS_LORENZ, n_traj=n, dt=F32_01,
                           abstol=1e-7, reltol=1e-6, precision=prec, schedule=sched, waves=waves,
-                          per_lane=per_lane)
+                          per_lane=per_lane, refill=refill)
     g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, backend="torch", device="cuda:0")
     ctx.solve(d, g, sync=True)
     return d, g, ctx.stats(), (u0, p, ts)
@@ -56,6 +57,26 @@ def test_register_cap_variants_give_the_same_bits(fresh_ctx, alg, dtype, prec, s
         assert pyoracle.solve(d, c, nthreads=8) == 0
         _, g, _, _ = _run(ctx, alg, dtype, prec, 6, sched, n=4096, seed=4)
         assert_bit_identical(g, c)
+
+
+@pytest.mark.parametrize("alg,dtype,prec", CASES, ids=IDS)
+@pytest.mark.parametrize("sched", [_abi.SCHED_PERSISTENT, _abi.SCHED_STAGED],
+                         ids=["persistent", "staged"])
+def test_refill_batches_give_the_same_bits(fresh_ctx, alg, dtype, prec, sched):
+    """The persistent schedules' refill batch (a wave claims new trajectories once at
+    least `refill` of its slots are free) changes which lane runs a trajectory and
+    when, never its arithmetic: every batch gives the default's bits, and stats report
+    the batch used (at most the wave's 64 slots; 4 by default)."""
+    ctx = fresh_ctx
+    _, ref, st0, _ = _run(ctx, alg, dtype, prec, 0, sched)
+    assert st0["refill"] == 4
+    for rf, used in ((1, 1), (2, 2), (8, 8), (64, 64), (1000, 64)):
+        _, g, st, _ = _run(ctx, alg, dtype, prec, 0, sched, refill=rf)
+        assert st["refill"] == used
+        assert_bit_identical(g, ref)
+    _, g, st, _ = _run(ctx, alg, dtype, prec, 0, _abi.SCHED_ONE_LAUNCH, refill=8)
+    assert st["refill"] == 0
+    assert_bit_identical(g, ref)
 
 
 MULTI = [(_abi.ALG_ATSIT5, _abi.F64, _abi.PREC_PLAIN), (_abi.ALG_ATSIT5, _abi.F32, _abi.PREC_PLAIN),
@@ -127,6 +148,9 @@ def test_register_cap_refusals(ctx):
     with pytest.raises(runtime.SolveError, match="per_lane must be"):
         d = runtime.make_desc(_abi.ALG_ATSIT5, n_traj=n, per_lane=3)
         ctx.solve(d, runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0"))
+    with pytest.raises(runtime.SolveError, match="refill must be"):
+        d = runtime.make_desc(_abi.ALG_ATSIT5, n_traj=n, refill=-1)
+        ctx.solve(d, runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0"))
 
 
 def test_autotuner_reports_and_keeps_its_register_cap(ctx):
@@ -137,10 +161,11 @@ def test_autotuner_reports_and_keeps_its_register_cap(ctx):
     g = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
     best = ctx.autotune(d, g)
     assert best["waves"] in (0, 4, 6, 8) and best["per_lane"] in (1, 2)
+    assert best["refill"] in ((0,) if best["schedule"] == _abi.SCHED_ONE_LAUNCH else (1, 4, 8))
     ctx.solve(d, g, sync=True)
     st = ctx.stats()
     assert st["waves"] == best["waves"] and st["schedule"] == best["schedule"]
-    assert st["per_lane"] == best["per_lane"]
+    assert st["per_lane"] == best["per_lane"] and st["refill"] == best["refill"]
     ref = runtime.EnsembleBuffers(d, 3, u0, p, backend="torch", device="cuda:0")
     d1 = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=n, tspan=(0.0, 10.0),
                            dt=F32_01, abstol=1e-6, reltol=1e-3, schedule=_abi.SCHED_ONE_LAUNCH)

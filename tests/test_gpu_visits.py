@@ -1,7 +1,8 @@
 """Every trajectory is handed to a lane exactly once per launch: the kernels' debug
 visit counters (sdeh_io.visits) end at 1 everywhere for the one-launch and
-persistent schedules (every queue chunk size, index, longest-span-first and caller
-orders, lane layouts, the allocation-free solve, graph replays) and, for the staged
+persistent schedules (every queue chunk size and refill batch, index,
+longest-span-first and caller orders, lane layouts, the allocation-free solve, graph
+replays) and, for the staged
 schedule, at 1 for trajectories that ended in its first pass and 2 for those it
 parked and resumed (more than 32 attempts)."""
 import numpy as np
@@ -48,6 +49,21 @@ def test_each_trajectory_once(ctx, sched, chunk, spans):
         assert (v == 1).all(), (alg, np.unique(v, return_counts=True))
 
 
+@pytest.mark.parametrize("refill", [1, 4, 64])
+def test_refill_batches_once(ctx, refill):
+    for sched in (_abi.SCHED_PERSISTENT, _abi.SCHED_STAGED):
+        d = runtime.make_desc(_abi.ALG_ATSIT5, rhs=_abi.RHS_LORENZ, n_traj=N, dt=F32_01,
+                              abstol=1e-6, reltol=1e-4, schedule=sched, refill=refill)
+        g = _lorenz(d, spans=True)
+        v, st = _counted(ctx, d, g)
+        assert st["refill"] == refill
+        if sched == _abi.SCHED_PERSISTENT:
+            assert (v == 1).all()
+        else:
+            att = g.numpy("naccept").astype(np.int64) + g.numpy("nreject")
+            assert np.array_equal(v, np.where(att > 32, 2, 1))
+
+
 @pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT],
                          ids=["one_launch", "persistent"])
 def test_two_per_lane_once(ctx, sched):
@@ -90,6 +106,13 @@ def test_group_and_matrix_core_layouts(ctx):
         g = runtime.EnsembleBuffers(d, 16, u0, r, A, backend="torch", device="cuda:0")
         v, st = _counted(ctx, d, g)
         assert st["lanes"] == lanes and (v == 1).all()
+    # the group layout (16 lanes, 4 slots per wave) with the largest batch
+    d = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_GLV16, n_traj=n, tspan=(0.0, 2.0),
+                          dt=0.1, abstol=1e-8, reltol=1e-8, lanes=16,
+                          schedule=_abi.SCHED_PERSISTENT, refill=100)
+    g = runtime.EnsembleBuffers(d, 16, u0, r, A, backend="torch", device="cuda:0")
+    v, st = _counted(ctx, d, g)
+    assert st["lanes"] == 16 and st["refill"] == 4 and (v == 1).all()
 
 
 def test_static_solve_and_graph_replays(ctx):
