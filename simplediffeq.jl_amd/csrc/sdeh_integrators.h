@@ -924,6 +924,23 @@ __device__ __forceinline__ T nan_of() {
 // ---------------------------------------------------------------------------
 // Wave-level work queue for the persistent schedule.
 // ---------------------------------------------------------------------------
+// A loop-invariant value held in vector registers.  The one-lane kernels run out
+// of scalar registers first; a scalar the compiler spills is reloaded with a
+// v_readlane (a vector instruction) on every attempt that uses it.
+template <class X>
+__device__ __forceinline__ X in_vgpr(X x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+// in_vgpr where it costs no occupancy: in the binary64 endpoint kernels (the
+// binary32 ones have scalar registers to spare and would cross 64 vector registers,
+// and the save modes' kernels would lose a wave per SIMD).
+template <bool ON, class X>
+__device__ __forceinline__ X in_vgpr_if(X x) {
+  if constexpr (ON) return in_vgpr(x);
+  return x;
+}
+
 __device__ __forceinline__ unsigned lane_rank(unsigned long long mask) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
@@ -1340,7 +1357,8 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
           gamma = T(9.0 / 10), qoldinit = T(1.0e-4);
   const T inv_qmax = T(1) / qmax, inv_qmin = T(1) / qmin;
   const T qoldinit_pow = Ops<T>::pw(qoldinit, beta2);
-  const T abstol = (T)a.abstol, reltol = (T)a.reltol;
+  constexpr bool PIN = sizeof(T) == 8 && SAVE == SAVE_END;
+  const T abstol = in_vgpr_if<PIN>((T)a.abstol), reltol = in_vgpr_if<PIN>((T)a.reltol);
   const CtrlK<T> ck{beta1, inv_qmax, inv_qmin, gamma, Ops<T>::rcp_nr(T(NormLen<R>::value)),
                     Ops<T>::rcp_nr(gamma)};
   const T* ps = (const T*)a.ps;
@@ -1371,9 +1389,9 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
   // output.
   constexpr bool STAGED = SAVE == SAVE_END && R::G == 1;
   using Park = ParkRec<T, R, ALG == ALG_ATSIT5, KAHAN>;
-  const int64_t att_lim = (STAGED && a.stage_cap > 0 && a.stage_cap < a.maxiters)
-                              ? (int64_t)a.stage_cap
-                              : a.maxiters;
+  const int64_t att_lim = in_vgpr_if<PIN>((STAGED && a.stage_cap > 0 && a.stage_cap < a.maxiters)
+                                              ? (int64_t)a.stage_cap
+                                              : a.maxiters);
 
   for (;;) {
     if (a.persistent) {
