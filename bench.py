@@ -9,12 +9,15 @@ Inputs are resident in HBM before the timed region.
 
 value = trajectories/s of the binary64 solve (the reference's precision) over the
 K timed steps, in the launch shape the library's autotuner picks (schedule,
-block, grid, queue chunk; config.launch).  Beside it:
+block, grid, queue chunk, register cap, trajectories per lane, refill batch;
+config.launch).  Beside it:
   * learned: the cost model (6-64-64-2 MLP) trained on the GPU from 2^20 plain
     binary64 step logs (seed 40), then the binned binary32 Kahan-compensated
     solve of the same ensemble, planning (features, forward pass, hand-out order)
     included, against the plain binary64 one-launch kernel in index order, with an
-    ablation (untrained weights, no order, the model-free orders);
+    ablation (untrained weights, no order, the model-free orders) and the same
+    binned solves with the model's predicted step size as each trajectory's dt0
+    (dt0_seeding: time, attempts, rejections, error);
   * error of every reported dtype against binary64 AVern9 at 1e-14 on the first
     65,536 trajectories (median and 90th percentile of the normwise relative
     endpoint error);
@@ -510,12 +513,13 @@ def headline(args):
 
     _progress("binary64 AVern9 1e-14 reference solved")
 
-    def bufs(dtype=_abi.F64, sched=0, precision=_abi.PREC_PLAIN, order=None, span=True, **kw):
+    def bufs(dtype=_abi.F64, sched=0, precision=_abi.PREC_PLAIN, order=None, span=True,
+             dt0=None, **kw):
         d = runtime.make_desc(_abi.ALG_ATSIT5, dtype=dtype, rhs=_abi.RHS_LORENZ, n_traj=n,
                               dt=f32_01, schedule=sched, precision=precision,
                               span_order=0 if span else 1, **tol, **kw)
-        g = runtime.EnsembleBuffers(d, 3, u0d, pd, tspan=tsd, order=order, backend="torch",
-                                    device=dev)
+        g = runtime.EnsembleBuffers(d, 3, u0d, pd, tspan=tsd, dt0=dt0, order=order,
+                                    backend="torch", device=dev)
         return d, g
 
     def solve_once(d, g):
@@ -598,6 +602,8 @@ def headline(args):
             ("f64_one_launch_index_order", _abi.F64, 1, _abi.PREC_PLAIN, None, False),
             ("f32k_persistent_binned", _abi.F32, 2, _abi.PREC_KAHAN, cm, False),
             ("f32k_one_launch_binned", _abi.F32, 1, _abi.PREC_KAHAN, cm, False),
+            ("f32k_persistent_binned_seeded", _abi.F32, 2, _abi.PREC_KAHAN, cm, False),
+            ("f32k_one_launch_binned_seeded", _abi.F32, 1, _abi.PREC_KAHAN, cm, False),
             ("f32k_one_launch_binned_two_per_lane", _abi.F32, 1, _abi.PREC_KAHAN, cm, False),
             ("f32k_one_launch_binned_untrained", _abi.F32, 1, _abi.PREC_KAHAN, untrained,
              False),
@@ -613,10 +619,12 @@ def headline(args):
         shapes = [{}, dict(block=64), dict(block=128)]
         if dtype == _abi.F32 and not base_kw and sched != 3:
             shapes += [dict(waves=4), dict(waves=6)]
+        seeded = name.endswith("_seeded")  # the model's dt0 as each trajectory's first dt
         best = None
         for shape in shapes:
-            od = None if model is None else plan(model)[1]
-            d, g = bufs(dtype, sched, prec, od, span, **base_kw, **shape)
+            od, dt0 = (None, None) if model is None else plan(model)[1:]
+            d, g = bufs(dtype, sched, prec, od, span, dt0 if seeded else None, **base_kw,
+                        **shape)
             st = solve_once(d, g)
             ms = time_of(d, g, reps)
             if best is None or ms < best[0]:
@@ -625,8 +633,10 @@ def headline(args):
             if shape != shapes[-1]:
                 del g
         ms, st, launch = best
-        runs[name] = {"kernel_ms": ms, "attempts": st["attempts"], "schedule": st["schedule"],
-                      "launch": launch}
+        runs[name] = {"kernel_ms": ms, "attempts": st["attempts"], "accepts": st["accepts"],
+                      "schedule": st["schedule"], "launch": launch}
+        if name == "f32k_persistent_binned_seeded":
+            err_seeded = ref.error(g)
         if name == "f32k_persistent_binned":
             err32k = ref.error(g)
             ach = algorithmic_flops(_abi.ALG_ATSIT5, 3, LORENZ_FLOPS, st["attempts"],
@@ -666,6 +676,19 @@ def headline(args):
                 runs[best_free]["kernel_ms"] - (runs[best_binned]["kernel_ms"] + plan_ms)},
         "train": {"step_logs": min(N_TRAIN, n), "step_log_solve_s": t_logs, "adam_steps":
                   args.train_steps, "seconds": t_train, "loss": [hist[0][1], hist[-1][1]]},
+        # the model's predicted step size as each trajectory's first dt instead of 0.1f0:
+        # the controller still decides every step, so results change within tolerance
+        # (the unseeded runs above are bit-identical to unbinned ones)
+        "dt0_seeding": {
+            "persistent_binned_seeded_ms": runs["f32k_persistent_binned_seeded"]["kernel_ms"],
+            "one_launch_binned_seeded_ms": runs["f32k_one_launch_binned_seeded"]["kernel_ms"],
+            "speedup_persistent_with_plan":
+                base_ms / (runs["f32k_persistent_binned_seeded"]["kernel_ms"] + plan_ms),
+            "attempts": {k: runs[k]["attempts"] for k in ("f32k_persistent_binned",
+                                                          "f32k_persistent_binned_seeded")},
+            "rejected": {k: runs[k]["attempts"] - runs[k]["accepts"]
+                         for k in ("f32k_persistent_binned", "f32k_persistent_binned_seeded")},
+            "error_vs_avern9_1e-14": err_seeded},
     }
 
     traffic64 = MEASURED_TRAFFIC.get(("f64", n, sched64, "costmodel"))

@@ -82,6 +82,24 @@ def _plan(S, cm):
     return cm.plan_lorenz(S["pd"], S["u0d"], S["tsd"], S["stream"], pred=False)
 
 
+def _median_error(ctx, S, g):
+    """Median normwise relative endpoint error of g's first N_ERR trajectories against
+    binary64 AVern9 at 1e-14 (solved once per module)."""
+    if "uref" not in S["rec"]:
+        dr = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_LORENZ, n_traj=N_ERR, dt=F32_01,
+                               abstol=1e-14, reltol=1e-14)
+        gr = runtime.EnsembleBuffers(dr, 3, S["u0"][:, :N_ERR], S["p"][:, :N_ERR],
+                                     tspan=S["ts"][:, :N_ERR], backend="torch", device=S["dev"])
+        ctx.solve(dr, gr, sync=True)
+        ok = gr.numpy("retcode") == 0
+        assert ok.mean() > 0.999
+        S["rec"]["uref"], S["rec"]["ok"] = gr.numpy("u_end"), ok
+    uref, ok = S["rec"]["uref"], S["rec"]["ok"]
+    ue = g.numpy("u_end")[:, :N_ERR].astype(np.float64)
+    e = np.max(np.abs(ue - uref), axis=0) / np.max(np.abs(uref), axis=0)
+    return float(np.median(e[ok]))
+
+
 def test_trained_plan_against_float64_network(ctx, setup):
     """The f16 planning pass of the trained model (sdeh_mlp_plan_lorenz) against the
     float64 reference network (oracle/mlp_ref.py) on the same features, first 65,536
@@ -124,23 +142,8 @@ def test_trained_binning_meets_target(ctx, setup):
     du, gu, _ = _solve(ctx, S, _abi.F32, _abi.SCHED_PERSISTENT, _abi.PREC_KAHAN)
     for f in ("u_end", "t_end", "naccept", "nreject"):
         assert np.array_equal(res["persistent"][1].numpy(f), gu.numpy(f)), f
-    # error against binary64 AVern9 at 1e-14 on the first N_ERR trajectories
-    dr = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_LORENZ, n_traj=N_ERR, dt=F32_01,
-                           abstol=1e-14, reltol=1e-14)
-    gr = runtime.EnsembleBuffers(dr, 3, S["u0"][:, :N_ERR], S["p"][:, :N_ERR],
-                                 tspan=S["ts"][:, :N_ERR], backend="torch", device=S["dev"])
-    ctx.solve(dr, gr, sync=True)
-    uref = gr.numpy("u_end")
-    ok = gr.numpy("retcode") == 0
-    assert ok.mean() > 0.999
-
-    def err(g):
-        ue = g.numpy("u_end")[:, :N_ERR].astype(np.float64)
-        e = np.max(np.abs(ue - uref), axis=0) / np.max(np.abs(uref), axis=0)
-        return float(np.median(e[ok]))
-
-    e32 = err(res["persistent"][1])
-    e64 = err(g0)
+    e32 = _median_error(ctx, S, res["persistent"][1])
+    e64 = _median_error(ctx, S, g0)
     speed_p = ms_base / (plan_ms + res["persistent"][0])
     speed_o = ms_base / (plan_ms + res["one_launch"][0])
     S["rec"].update(base_ms=ms_base, plan_ms=plan_ms, order=order)
@@ -191,3 +194,40 @@ def test_network_gain_ablation(ctx, setup):
     assert eff["trained"] > eff["untrained"] + 0.2 and eff["trained"] > eff["none"] + 0.2
     assert ms["trained"] + plan_ms["trained"] < ms["untrained"] + plan_ms["untrained"]
     assert ms["trained"] + plan_ms["trained"] < ms["none"]
+
+
+def test_dt0_seeding(ctx, setup):
+    """The model's second output seeds each trajectory's first step (dt0 =
+    exp(pred_1), the predicted mean accepted step, instead of 0.1f0).  The PI
+    controller still decides every accept and reject, so the seeded solve differs from
+    the unseeded one within tolerance: every trajectory succeeds, the median error
+    against binary64 AVern9 at 1e-14 stays <= 1e-5, and the rejected attempts that
+    shrink an oversized first step are fewer."""
+    S = setup
+    torch, cm = S["torch"], S["cm"]
+    _, order, dt0 = _plan(S, cm)
+    torch.cuda.synchronize()
+    d = dt0.cpu().numpy()
+    assert np.isfinite(d).all() and (d > 0).all()
+    out = {}
+    for seeded in (False, True):
+        order = _plan(S, cm)[1]  # the last planned order: no permutation check
+        torch.cuda.synchronize()
+        dd = runtime.make_desc(_abi.ALG_ATSIT5, dtype=_abi.F32, rhs=_abi.RHS_LORENZ,
+                               n_traj=N_EVAL, dt=F32_01, schedule=_abi.SCHED_PERSISTENT,
+                               precision=_abi.PREC_KAHAN, **TOL)
+        g = runtime.EnsembleBuffers(dd, 3, S["u0d"], S["pd"], tspan=S["tsd"],
+                                    dt0=dt0 if seeded else None, order=order,
+                                    backend="torch", device=S["dev"])
+        ctx.solve(dd, g, sync=True)
+        assert (g.numpy("retcode") == 0).all()
+        ms = _time(ctx, S, dd, g)
+        out[seeded] = dict(ms=ms, naccept=int(g.numpy("naccept").astype(np.int64).sum()),
+                           nreject=int(g.numpy("nreject").astype(np.int64).sum()),
+                           err=_median_error(ctx, S, g))
+    print(json.dumps({"dt0_seeding": {"unseeded": out[False], "seeded": out[True]}}),
+          flush=True)
+    assert out[True]["err"] <= 1e-5
+    assert out[True]["nreject"] < out[False]["nreject"]
+    # the controller still sets the step sizes: the accepted steps barely move
+    assert abs(out[True]["naccept"] - out[False]["naccept"]) < 0.05 * out[False]["naccept"]
