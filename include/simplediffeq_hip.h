@@ -204,9 +204,8 @@ typedef struct sdeh_solve_desc {
                                them are waited for once per batch.  0 = tuned (else 4
                                with one lane per trajectory, 1 with several); values
                                above the wave's slot count (64 / lanes) act as that
-                               count; the wave-layout and two-per-lane kernels refill
-                               as soon as a slot is free.  Same results.  (ABI 5; was
-                               reserved, 0.) */
+                               count; the two-per-lane kernels refill as soon as a slot
+                               is free.  Same results.  (ABI 5; was reserved, 0.) */
 } sdeh_solve_desc;
 
 /* Solve buffers.  With memory == SDEH_MEM_HOST these are host pointers and the

@@ -1071,7 +1071,12 @@ __device__ __forceinline__ void adaptive_body_wave(const sdeh_kargs& a) {
 
   for (;;) {
     if (a.persistent) {
-      int64_t slot = wave_claim(a, !active && lead, qbase, qlim, drained);
+      bool want = !active && lead;
+      if (a.refill_min > 1) {  // a batch of free slots, as in adaptive_body
+        const int kmin = a.refill_min < SLOTS ? a.refill_min : SLOTS;
+        if (__popcll(__ballot(want)) < kmin) want = false;
+      }
+      int64_t slot = wave_claim(a, want, qbase, qlim, drained);
       slot = (int64_t)__shfl((long long)slot, lane & (SLOTS - 1));  // from the slot's lead lane
       if (slot >= 0) {
         traj = slot_traj(a, slot);

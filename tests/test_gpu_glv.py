@@ -52,6 +52,22 @@ def test_glv16_binary64_bitexact(ctx, alg, saveat, lanes, sched, layout):
         assert (g.numpy("naccept") > 5).all()
 
 
+@pytest.mark.parametrize("lanes", [1, 16, 4])
+def test_glv16_refill_batches_bitexact(ctx, lanes):
+    """Persistent refill batches (a wave starts new trajectories once `refill` of its
+    slots are free), up to the wave's whole slot count, in every lane layout: the
+    oracle's bits, saveat included."""
+    n = 203
+    u0, r, A = problem(_abi.RHS_GLV16, n, seed=3)
+    for k in (2, 64):
+        d = _desc(_abi.ALG_AVERN9, n, True, lanes, schedule=_abi.SCHED_PERSISTENT, chunk=3,
+                  refill=k)
+        st_g, g, st_c, c = run_pair(ctx, d, u0, r, A)
+        assert st_g == 0 and st_c == 0
+        assert ctx.stats()["refill"] == min(k, 64 // lanes)
+        assert_bit_identical(g, c, FIELDS + ("u_saveat",))
+
+
 @pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT])
 @pytest.mark.parametrize("kahan,saveat,layout",
                          [(False, False, _abi.SAVE_POINT_MAJOR),

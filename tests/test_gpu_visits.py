@@ -106,13 +106,15 @@ def test_group_and_matrix_core_layouts(ctx):
         g = runtime.EnsembleBuffers(d, 16, u0, r, A, backend="torch", device="cuda:0")
         v, st = _counted(ctx, d, g)
         assert st["lanes"] == lanes and (v == 1).all()
-    # the group layout (16 lanes, 4 slots per wave) with the largest batch
-    d = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_GLV16, n_traj=n, tspan=(0.0, 2.0),
-                          dt=0.1, abstol=1e-8, reltol=1e-8, lanes=16,
-                          schedule=_abi.SCHED_PERSISTENT, refill=100)
-    g = runtime.EnsembleBuffers(d, 16, u0, r, A, backend="torch", device="cuda:0")
-    v, st = _counted(ctx, d, g)
-    assert st["lanes"] == 16 and st["refill"] == 4 and (v == 1).all()
+    # the largest refill batch: every slot of the wave (16 lanes: 4 slots; the
+    # matrix-core layout, 4 lanes: 16)
+    for lanes, slots in ((16, 4), (4, 16)):
+        d = runtime.make_desc(_abi.ALG_AVERN9, rhs=_abi.RHS_GLV16, n_traj=n, tspan=(0.0, 2.0),
+                              dt=0.1, abstol=1e-8, reltol=1e-8, lanes=lanes,
+                              schedule=_abi.SCHED_PERSISTENT, refill=100)
+        g = runtime.EnsembleBuffers(d, 16, u0, r, A, backend="torch", device="cuda:0")
+        v, st = _counted(ctx, d, g)
+        assert st["lanes"] == lanes and st["refill"] == slots and (v == 1).all()
 
 
 def test_static_solve_and_graph_replays(ctx):
