@@ -614,11 +614,14 @@ def headline(args):
             ("f32k_staged", _abi.F32, 3, _abi.PREC_KAHAN, None, True)):
         # every run, the baseline included, takes the fastest of a few launch shapes
         # of its own schedule and order (the library's default, blocks of 64 and 128,
-        # and for binary32 the 4- and 6-wave register caps)
+        # for binary32 the 4- and 6-wave register caps, and for the persistent and
+        # staged schedules refill batches of 8 and 64 beside the default 4)
         base_kw = dict(per_lane=2) if name.endswith("two_per_lane") else {}
         shapes = [{}, dict(block=64), dict(block=128)]
         if dtype == _abi.F32 and not base_kw and sched != 3:
             shapes += [dict(waves=4), dict(waves=6)]
+        if sched in (2, 3) and not base_kw:
+            shapes += [dict(refill=8), dict(refill=64)]
         seeded = name.endswith("_seeded")  # the model's dt0 as each trajectory's first dt
         best = None
         for shape in shapes:
