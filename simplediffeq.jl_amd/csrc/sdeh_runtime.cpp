@@ -719,10 +719,12 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
   // refill batch: the caller's, else the tuned one, else 4 slots with one lane per
   // trajectory (measured on the 2^22 Lorenz ensemble: 4.5% faster binary64, 7%
   // binary32, than claiming for each free slot) and 1 with several lanes
-  if (a.persistent) {
+  if (a.persistent && pl->k.tpl > 1) {
+    sh.refill = 1;  // the two-per-lane kernels claim for each free slot
+  } else if (a.persistent) {
     const int slots = 64 / (pl->k.lanes > 0 ? pl->k.lanes : 1);
     int rf = d->refill ? d->refill : sh.refill;
-    if (!rf) rf = pl->k.lanes == 1 && pl->k.tpl == 1 ? 4 : 1;
+    if (!rf) rf = pl->k.lanes == 1 ? 4 : 1;
     sh.refill = rf < slots ? rf : slots;
   } else {
     sh.refill = 0;

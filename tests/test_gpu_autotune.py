@@ -90,8 +90,10 @@ MULTI = [(_abi.ALG_ATSIT5, _abi.F64, _abi.PREC_PLAIN), (_abi.ALG_ATSIT5, _abi.F3
 def test_two_trajectories_per_lane_give_the_same_bits(fresh_ctx, alg, dtype, prec, sched, n):
     ctx = fresh_ctx
     _, ref, _, _ = _run(ctx, alg, dtype, prec, 0, sched, n=n)
-    d, g, st, (u0, p, ts) = _run(ctx, alg, dtype, prec, 0, sched, n=n, per_lane=2)
+    d, g, st, (u0, p, ts) = _run(ctx, alg, dtype, prec, 0, sched, n=n, per_lane=2, refill=8)
     assert st["per_lane"] == 2 and st["schedule"] != _abi.SCHED_STAGED
+    # the two-per-lane kernels claim for each free slot whatever the batch asked for
+    assert st["refill"] == (0 if st["schedule"] == _abi.SCHED_ONE_LAUNCH else 1)
     assert_bit_identical(g, ref)
     if dtype == _abi.F64 and n > 2:
         c = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts)
