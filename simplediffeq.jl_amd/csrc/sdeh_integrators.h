@@ -980,6 +980,54 @@ __device__ __forceinline__ int64_t wave_claim(const sdeh_kargs& a, bool need_me,
   return mine;
 }
 
+// wave_claim that also names the claimed slot's trajectory, as slot_traj does.  The
+// wave caches the hand-out order's entries for 64 consecutive slots, win .. win + 63,
+// one per lane in `word`, and reloads them with one coalesced load when the slots it
+// hands out leave that window; a starting lane then takes its entry from the cache
+// with a lane shuffle instead of loading it after the claim, a memory round trip the
+// whole wave would wait for.  Without an order the cache holds the slots themselves.
+// `win` starts below -64, so that the first hand-out loads the window.
+__device__ __forceinline__ int64_t wave_claim_win(const sdeh_kargs& a, bool need_me,
+                                                  int64_t& base, int64_t& lim, bool& drained,
+                                                  int64_t& win, int64_t& word, int64_t& traj) {
+  int64_t mine = -1;
+  traj = -1;
+  unsigned long long need = __ballot(need_me);
+  while (need != 0ull && !drained) {
+    const int cnt = __popcll(need);
+    if (base >= lim) {
+      unsigned long long old = 0;
+      if (__lane_id() == 0) old = atomicAdd(a.queue, (unsigned long long)a.chunk);
+      old = (unsigned long long)__shfl((long long)old, 0);
+      base = (int64_t)old;
+      lim = base + a.chunk < a.n ? base + a.chunk : a.n;
+      if (base >= a.n) {
+        drained = true;
+        break;
+      }
+    }
+    const int64_t avail = lim - base;
+    const bool in_need = ((need >> __lane_id()) & 1ull) != 0ull;
+    const int64_t rank = (int64_t)lane_rank(need);
+    const int64_t taken = (int64_t)cnt < avail ? (int64_t)cnt : avail;
+    if (base < win || base + taken > win + 64) {  // the hand-out leaves the window
+      win = base;
+      const int64_t e = win + (int64_t)__lane_id();
+      word = !a.order ? e : (e < a.n ? a.order[e] : int64_t(-1));
+    }
+    const bool gets = in_need && rank < avail;
+    const int src = gets ? (int)(base + rank - win) : (int)__lane_id();
+    const int64_t v = (int64_t)__shfl((long long)word, src);
+    if (gets) {
+      mine = base + rank;
+      traj = (uint64_t)v < (uint64_t)a.n ? v : int64_t(-1);
+    }
+    base += taken;
+    need = __ballot(in_need && mine < 0);
+  }
+  return mine;
+}
+
 // The trajectory handed out at `slot`: order[slot], or slot itself without an order.
 // An entry outside [0, n) names no trajectory: it gives -1, which no lane takes up, so
 // that a hand-out order is never dereferenced out of bounds (the host checks caller
@@ -1381,6 +1429,12 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
 
   int64_t qbase = 0, qlim = 0;
   bool drained = !a.persistent;
+  // the persistent queue's cache of hand-out order entries (wave_claim_win): in the
+  // binary32 one-lane endpoint kernels, whose occupancy is set by scalar registers, so
+  // the vector registers it takes cost no wave (the binary64 and save modes' kernels
+  // would lose one)
+  constexpr bool ORDER_WIN = sizeof(T) == 4 && R::G == 1 && SAVE == SAVE_END;
+  int64_t qwin = -128, qword = -1;
   const bool lead = lane_comp<R>() == 0;  // writes the per-trajectory scalars
   if (!a.persistent) {
     const int64_t slot = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / R::G;
@@ -1408,11 +1462,15 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
         if (__popcll(__ballot(want)) < kmin) want = false;
       }
       for (;;) {
-        int64_t slot = wave_claim(a, want, qbase, qlim, drained);
+        int64_t slot, claimed = -1;
+        if constexpr (ORDER_WIN)
+          slot = wave_claim_win(a, want, qbase, qlim, drained, qwin, qword, claimed);
+        else
+          slot = wave_claim(a, want, qbase, qlim, drained);
         if constexpr (R::G > 1)  // the group leader claimed; its group follows
           slot = (int64_t)__shfl((long long)slot, (int)(__lane_id() & ~(unsigned)(R::G - 1)));
         if (slot >= 0) {
-          traj = slot_traj(a, slot);
+          traj = ORDER_WIN ? claimed : slot_traj(a, slot);
           active = fresh = traj >= 0;
           // a resumed launch skips the trajectories that ended in the first stage
           if (STAGED && a.resume && active && Park::flag(a, traj) == 0) active = fresh = false;
