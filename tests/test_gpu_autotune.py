@@ -121,6 +121,38 @@ def test_two_per_lane_traced_rhs(fresh_ctx):
     assert_bit_identical(out[1], out[0])
 
 
+def test_traced_rhs_persistent_paths_match_builtin(fresh_ctx):
+    """A registered (hipRTC) right-hand side gets the persistent queue's refill
+    batches, order cache and staged resume compiled from the same headers: binary32
+    compensated ATsit5 with a caller's order, every batch, and the staged schedule all
+    give the built-in kernel's bits."""
+    from simplediffeq_hip import tracer
+    ctx = fresh_ctx
+    src, fn = tracer.trace(lambda u, p, t: [p[0] * (u[1] - u[0]), u[0] * (p[1] - u[2]) - u[1],
+                                            u[0] * u[1] - p[2] * u[2]], 3, 3, "lorenz_queue")
+    h = ctx.compile_rhs(src, fn, 3, 3)
+    n = 20_011
+    u0, p, ts = inputs.lorenz_uneven(n, seed=5)
+    ts[1] = np.minimum(ts[1], 4.0)
+    perm = np.random.default_rng(5).permutation(n).astype(np.int64)
+    out = {}
+    for rhs in (_abi.RHS_LORENZ, h):
+        for name, sched, order, refill in (("persistent_r1", _abi.SCHED_PERSISTENT, perm, 1),
+                                           ("persistent_r8", _abi.SCHED_PERSISTENT, perm, 8),
+                                           ("staged", _abi.SCHED_STAGED, None, 0)):
+            d = runtime.make_desc(_abi.ALG_ATSIT5, dtype=_abi.F32, rhs=rhs, n_traj=n, dt=F32_01,
+                                  abstol=1e-6, reltol=1e-4, precision=_abi.PREC_KAHAN,
+                                  schedule=sched, refill=refill, chunk=16)
+            g = runtime.EnsembleBuffers(d, 3, u0, p, tspan=ts, order=order, backend="torch",
+                                        device="cuda:0")
+            ctx.solve(d, g, sync=True)
+            assert ctx.stats()["schedule"] == sched
+            out[(rhs, name)] = g
+    for name in ("persistent_r1", "persistent_r8", "staged"):
+        assert_bit_identical(out[(h, name)], out[(_abi.RHS_LORENZ, name)])
+        assert_bit_identical(out[(h, name)], out[(h, "persistent_r1")])
+
+
 def test_register_cap_refusals(ctx):
     n = 64
     u0, p = inputs.lorenz_wide(n, seed=1)
