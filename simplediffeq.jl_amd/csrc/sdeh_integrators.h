@@ -340,6 +340,32 @@ __device__ __forceinline__ T err_sumsq(T (&tmp)[R::N], const T (&uprev)[R::N],
   }
 }
 
+// err_sumsq of one lane's trajectory with the scaled errors divided by bare Newton
+// iterations (Ops<T>::div_in), err left unchanged.  Valid when every scale b =
+// m * reltol + abstol is a normal number below 2^1022 (1/b normal too) and at least
+// 2^-420 (sdeh_kargs::err_bare) and the sum it returns lies in [2^-700, 2^700]: then
+// it is err_sumsq's sum, bit for bit.  Each term t = e / b falls in one of three cases.
+// |t| in [2^-540, 2^350] (the sum bounds every term by 2^350): the numerator is
+// at least 2^-960 and exceeds the divisor by fewer than 768 binades, the quotient is
+// normal, and the IEEE sequence's scalings and fix-up do nothing, so div_in runs
+// the same operations.  |t| < 2^-540 (or e = 0): both quotients square to +0.
+// |t| > 2^350 (or e Inf or NaN): the bare quotient is at least 2^349, Inf or NaN, so
+// the returned sum leaves the range and the caller computes err_sumsq instead.
+template <class T, class R>
+__device__ __forceinline__ T err_sumsq_in(const T (&e)[R::N], const T (&uprev)[R::N],
+                                          const T (&u)[R::N], T abstol, T reltol) {
+  static_assert(R::G == 1, "one lane per trajectory");
+  T s = T(0);
+#pragma unroll
+  for (int c = 0; c < R::N; ++c) {
+    const T au = fabs(u[c]), ap = fabs(uprev[c]);
+    const T m = au < ap ? ap : au;
+    const T t = Ops<T>::div_in(e[c], fma(m, reltol, abstol));
+    s = c == 0 ? t * t : s + t * t;
+  }
+  return s;
+}
+
 template <class R>
 struct NormLen {
   static constexpr int value = R::N * R::G;
@@ -1552,16 +1578,25 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
       }
 #pragma unroll
       for (int c = 0; c < N; ++c) err[c] = dt * err[c];
-      const T ssq = err_sumsq<T, R>(err, uprev, u, abstol, reltol);
       CtrlOut<T> co;
       // The range-checked controller runs for every lane; the IEEE one again only
       // where the check fails (a branch the wave skips when no lane needs it).
       // Measured on the 2^20-trajectory Lorenz ATsit5 solve: 1.135 ms, against 1.27
       // ms for an if/else on the check and 1.19 ms with IEEE operations only.
-      if constexpr (Ops<T>::kFast) {
+      // One lane per trajectory: the error norm's divisions are range-checked too,
+      // by the same check on the sum they give.
+      if constexpr (Ops<T>::kFast && R::G == 1) {
+        T ssq = err_sumsq_in<T, R>(err, uprev, u, abstol, reltol);
+        const bool in = ctrl_in_range(ssq, dt) && a.err_bare;
+        if (!in) ssq = err_sumsq<T, R>(err, uprev, u, abstol, reltol);
+        co = controller<T, R, true>(ssq, dt, qold_pow, ck);
+        if (!in) co = controller<T, R, false>(ssq, dt, qold_pow, ck);
+      } else if constexpr (Ops<T>::kFast) {
+        const T ssq = err_sumsq<T, R>(err, uprev, u, abstol, reltol);
         co = controller<T, R, true>(ssq, dt, qold_pow, ck);
         if (!ctrl_in_range(ssq, dt)) co = controller<T, R, false>(ssq, dt, qold_pow, ck);
       } else {
+        const T ssq = err_sumsq<T, R>(err, uprev, u, abstol, reltol);
         co = controller<T, R, false>(ssq, dt, qold_pow, ck);
       }
       const T EEst = co.EEst;

@@ -68,6 +68,12 @@ struct sdeh_kargs {
   // refill_min of its trajectory slots are free (0 or 1: as soon as one is), so that
   // the loads that start them are waited for once per batch.
   int32_t refill_min;
+  // 1 when abstol is in [2^-420, 2^1020] and reltol in [0, 2^-3], so every scale
+  // m * reltol + abstol of the error norm (m = max(|uprev|, |u|) finite) is a normal
+  // number below 2^1022 with a normal reciprocal: the binary64 one-lane kernels may
+  // then divide the scaled errors by bare Newton iterations (the error sum of
+  // squares is the IEEE one whenever it lies in the controller's checked range).
+  int32_t err_bare;
   // Optional [n] debug counters: a lane that takes trajectory i up in a launch
   // (a fresh start, or a staged solve resuming it) adds 1 to visits[i].
   int32_t* visits;

@@ -621,6 +621,9 @@ int plan_solve(sdeh_ctx* c, const sdeh_solve_desc* d, const sdeh_io* io, Plan* p
   a.dt = d->dt;
   a.abstol = d->abstol;
   a.reltol = d->reltol;
+  // Error-norm scales m * reltol + abstol in [2^-420, 2^1022) for every finite m.
+  a.err_bare = d->abstol >= 0x1p-420 && d->abstol <= 0x1p1020 && d->reltol >= 0.0 &&
+               d->reltol <= 0x1p-3;
   a.max_saves = pl->save == SAVE_STEPS && !pl->packed ? d->max_saves : 0;
   a.quirks = d->quirks;
   a.traj_major = d->save_layout == SDEH_SAVE_TRAJ_MAJOR;

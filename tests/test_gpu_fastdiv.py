@@ -71,3 +71,60 @@ def test_fast_division_and_root_are_ieee(ctx):
         assert bits_equal(out[3], a / 3.0)
         assert bits_equal(out[5], a / 0.9)
         assert bits_equal(out[7], np.sqrt(x))
+
+
+SRC_NORM = """
+template <class T>
+__device__ void probe(T* du, const T* u, const T* p, const T* __restrict__ ps, double t) {
+  const T bare = sdeh::Ops<T>::div_in(p[0], p[1]), ieee = p[0] / p[1];
+  du[0] = bare;
+  du[1] = ieee;
+  du[2] = bare * bare;
+  du[3] = ieee * ieee;
+}
+"""
+
+
+def test_error_norm_division_cases(ctx):
+    """The error norm's terms e / b with b = m * reltol + abstol in [2^-420, 2^1022)
+    (what sdeh_kargs::err_bare guarantees) and e any binary64: the bare quotient is
+    the IEEE one where |e / b| is in [2^-540, 2^350]; both square to +0 below; above
+    (and for e Inf or NaN) the bare square exceeds 2^700 or is NaN, so the kernels'
+    range check on the sum sends the lane to the IEEE norm."""
+    n = 1 << 20
+    rng = np.random.default_rng(78)
+    b = np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(-420, 1022, n))
+    q = np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(-700, 500, n))
+    k = 1 << 16
+    with np.errstate(over="ignore"):  # products past the largest binary64 are Inf numerators
+        e = b * q * rng.choice([-1.0, 1.0], n)  # quotients across every case and its edges
+        e[:k] = np.ldexp(rng.uniform(1.0, 2.0, k), rng.integers(-1074, 1024, k))  # any numerator
+        b[k:2 * k] = np.ldexp(rng.uniform(1.0, 2.0, k), rng.integers(1015, 1022, k))  # top
+        e[k:2 * k] = b[k:2 * k] * np.ldexp(rng.uniform(1.0, 2.0, k),
+                                           rng.integers(-545, 355, k))
+    edge = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 5e-324, np.finfo(np.float64).max,
+                     2.0 ** -540, 2.0 ** 350, 2.0 ** -540 * (1 - 2.0 ** -53), 2.0 ** 350 * 1.5])
+    e[2 * k:2 * k + len(edge)] = edge
+    b[2 * k:2 * k + len(edge)] = 1.0
+    b[2 * k + len(edge):2 * k + 2 * len(edge)] = 2.0 ** 1022 * (1 - 2.0 ** -53)
+    b[2 * k + 2 * len(edge):2 * k + 3 * len(edge)] = 2.0 ** -420
+    h = ctx.compile_rhs(SRC_NORM, "probe", 4, 2)
+    d = runtime.make_desc(_abi.ALG_EULER, rhs=h, n_traj=n, tspan=(0.0, 1.0), dt=1.0)
+    buf = runtime.EnsembleBuffers(d, 4, np.zeros((4, n)), np.stack([e, b]))
+    assert ctx.solve(d, buf) == 0
+    bare, ieee, bare2, ieee2 = buf.u_end
+    with np.errstate(all="ignore"):
+        ref = 0.0 + 1.0 * (e / b)  # the Euler step u0 + dt f with u0 = 0, dt = 1 (-0 -> +0)
+    num = ~np.isnan(ref)
+    bad = np.flatnonzero(num & (ieee.view(np.int64) != ref.view(np.int64)))
+    assert len(bad) == 0 and np.isnan(ieee[~num]).all(), (
+        len(bad), [(e[i], b[i], ieee[i], ref[i]) for i in bad[:4]])
+    mag = np.abs(ref)
+    mid = (mag >= 2.0 ** -540) & (mag <= 2.0 ** 350)
+    lo = mag < 2.0 ** -540
+    hi = ~(mid | lo)  # above 2^350, Inf or NaN
+    assert mid.sum() > n // 4 and lo.sum() > n // 8 and hi.sum() > n // 8
+    bad = np.flatnonzero(mid & (bare.view(np.int64) != ieee.view(np.int64)))
+    assert len(bad) == 0, (len(bad), [(e[i], b[i], bare[i], ieee[i]) for i in bad[:4]])
+    assert (bare2[lo] == 0).all() and (ieee2[lo] == 0).all()
+    assert not (bare2[hi] <= 2.0 ** 700).any()

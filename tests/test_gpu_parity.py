@@ -143,3 +143,32 @@ def test_controller_range_edges_bitexact(ctx, alg):
     assert_bit_identical(g, c)
     ret = c.numpy("retcode")[np.argsort(perm)]
     assert (ret[:512] == _abi.RET_DEFAULT).all() and (ret[512:] == _abi.RET_NAN).all()
+
+
+@pytest.mark.parametrize("tols", [(1e-6, 1e-3), (1e-8, 0.125), (1e-300, 1e-3), (1e-6, 0.5),
+                                  (0.0, 1e-3)],
+                         ids=["bare", "bare-reltol-edge", "tiny-abstol", "large-reltol",
+                              "abstol-0"])
+@pytest.mark.parametrize("alg", [_abi.ALG_ATSIT5, _abi.ALG_AVERN7, _abi.ALG_AVERN9],
+                         ids=["atsit5", "avern7", "avern9"])
+def test_error_norm_scales_bitexact(ctx, alg, tols):
+    """u' = lambda u with the three components of one trajectory at very different
+    magnitudes (1e-250 .. 1e300), so single terms of the error norm square to +0, to a
+    subnormal, or are divided by scales far above 2^60, while the sum stays in the
+    range-checked controller's range.  The binary64 kernels divide the norm's terms by
+    bare Newton iterations when abstol is in [2^-420, 2^1020] and reltol in [0, 2^-3]
+    (the first two tolerance pairs) and by the IEEE sequence otherwise; every output bit
+    for bit the oracle's either way."""
+    abstol, reltol = tols
+    n = 768
+    rng = np.random.default_rng(47)
+    mags = np.array([1e-250, 1e-170, 1e-160, 1e-100, 1.0, 1e100, 1e250, 1e300])
+    u0 = (mags[rng.integers(0, len(mags), (3, n))] * rng.uniform(0.5, 2.0, (3, n))
+          * rng.choice([-1.0, 1.0], (3, n)))
+    p = -rng.uniform(0.1, 3.0, (1, n))
+    d = runtime.make_desc(alg, rhs=_abi.RHS_EXPDECAY3, n_traj=n, tspan=(0.0, 1.0), dt=0.1,
+                          abstol=abstol, reltol=reltol, maxiters=1000)
+    st_g, g, st_c, c = run_pair(ctx, d, u0, p)
+    assert st_g == st_c
+    assert_bit_identical(g, c)
+    assert (c.numpy("retcode") == _abi.RET_DEFAULT).all()
