@@ -67,12 +67,12 @@ HBM_PEAK_GBS = 8000.0
 #   + 389,626 KiB written, mostly the parked records and the resumed pass's endpoint
 #   stores in hand-out order (partial lines).
 #   The default line, binary64, 2^22 trajectories with their own t_end, persistent
-#   (128 x 2,560, chunk 64, refill batches of 4), longest span first: 3,847,200 KiB
-#   read + 930,210 KiB written; each trajectory's eight input loads land on eight
+#   (256 x 1,280, chunk 64, refill batches of 4), longest span first: 3,851,216 KiB
+#   read + 930,024 KiB written; each trajectory's eight input loads land on eight
 #   separate lines, since the span order hands neighbouring lanes trajectories about a
 #   hundred indices apart.
 MEASURED_TRAFFIC = {("f64", 1 << 20, 2): 134.79e6, ("f64", 1 << 20, 3): 639.44e6,
-                    ("f64", 1 << 22, 2, "costmodel"): 4.892e9}
+                    ("f64", 1 << 22, 2, "costmodel"): 4.896e9}
 # Algorithmic bytes per trajectory of an endpoint solve with its own tspan (binary64
 # Lorenz): reads u0, p (48), tspan (16), the hand-out slot (8); writes u_end (24),
 # t_end (8), retcode, naccept, nreject (12).
