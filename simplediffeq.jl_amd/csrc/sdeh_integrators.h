@@ -1583,12 +1583,16 @@ __device__ __forceinline__ void adaptive_body(const sdeh_kargs& a) {
       // where the check fails (a branch the wave skips when no lane needs it).
       // Measured on the 2^20-trajectory Lorenz ATsit5 solve: 1.135 ms, against 1.27
       // ms for an if/else on the check and 1.19 ms with IEEE operations only.
-      // One lane per trajectory: the error norm's divisions are range-checked too,
-      // by the same check on the sum they give.
+      // One lane per trajectory: under tolerances that allow it (a.err_bare, the
+      // same for every lane) the error norm's divisions are range-checked too, by the
+      // same check on the sum they give; a bare sum out of range is recomputed with
+      // IEEE divisions.
       if constexpr (Ops<T>::kFast && R::G == 1) {
-        T ssq = err_sumsq_in<T, R>(err, uprev, u, abstol, reltol);
-        const bool in = ctrl_in_range(ssq, dt) && a.err_bare;
-        if (!in) ssq = err_sumsq<T, R>(err, uprev, u, abstol, reltol);
+        T ssq;
+        if (a.err_bare) ssq = err_sumsq_in<T, R>(err, uprev, u, abstol, reltol);
+        else ssq = err_sumsq<T, R>(err, uprev, u, abstol, reltol);
+        const bool in = ctrl_in_range(ssq, dt);
+        if (!in && a.err_bare) ssq = err_sumsq<T, R>(err, uprev, u, abstol, reltol);
         co = controller<T, R, true>(ssq, dt, qold_pow, ck);
         if (!in) co = controller<T, R, false>(ssq, dt, qold_pow, ck);
       } else if constexpr (Ops<T>::kFast) {
