@@ -151,14 +151,18 @@ def test_controller_range_edges_bitexact(ctx, alg):
                               "abstol-0"])
 @pytest.mark.parametrize("alg", [_abi.ALG_ATSIT5, _abi.ALG_AVERN7, _abi.ALG_AVERN9],
                          ids=["atsit5", "avern7", "avern9"])
-def test_error_norm_scales_bitexact(ctx, alg, tols):
+@pytest.mark.parametrize("sched", [_abi.SCHED_ONE_LAUNCH, _abi.SCHED_PERSISTENT,
+                                   _abi.SCHED_STAGED], ids=["one", "persistent", "staged"])
+def test_error_norm_scales_bitexact(ctx, alg, tols, sched):
     """u' = lambda u with the three components of one trajectory at very different
     magnitudes (1e-250 .. 1e300), so single terms of the error norm square to +0, to a
     subnormal, or are divided by scales far above 2^60, while the sum stays in the
     range-checked controller's range.  The binary64 kernels divide the norm's terms by
     bare Newton iterations when abstol is in [2^-420, 2^1020] and reltol in [0, 2^-3]
     (the first two tolerance pairs) and by the IEEE sequence otherwise; every output bit
-    for bit the oracle's either way."""
+    for bit the oracle's either way, in every schedule (the staged one parks the ATsit5
+    trajectories that need more than 32 attempts, up to 215 of them, and resumes them in a
+    second launch)."""
     abstol, reltol = tols
     n = 768
     rng = np.random.default_rng(47)
@@ -166,8 +170,8 @@ def test_error_norm_scales_bitexact(ctx, alg, tols):
     u0 = (mags[rng.integers(0, len(mags), (3, n))] * rng.uniform(0.5, 2.0, (3, n))
           * rng.choice([-1.0, 1.0], (3, n)))
     p = -rng.uniform(0.1, 3.0, (1, n))
-    d = runtime.make_desc(alg, rhs=_abi.RHS_EXPDECAY3, n_traj=n, tspan=(0.0, 1.0), dt=0.1,
-                          abstol=abstol, reltol=reltol, maxiters=1000)
+    d = runtime.make_desc(alg, rhs=_abi.RHS_EXPDECAY3, n_traj=n, tspan=(0.0, 10.0), dt=0.1,
+                          abstol=abstol, reltol=reltol, maxiters=1000, schedule=sched)
     st_g, g, st_c, c = run_pair(ctx, d, u0, p)
     assert st_g == st_c
     assert_bit_identical(g, c)
