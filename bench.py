@@ -62,16 +62,16 @@ HBM_PEAK_GBS = 8000.0
 # 2.000 + WRITE_SIZE x 1.000, factors from a 256 MiB device copy in the same run).
 # Counters cannot be read from inside a timed run, so each figure applies only to the
 # configuration it was measured on.
-#   The wide ensemble, binary64, 2^20 trajectories: persistent 49,405 KiB read +
-#   82,225 KiB written; staged (first pass, key sort, resumed pass) 234,824 KiB read
-#   + 389,626 KiB written, mostly the parked records and the resumed pass's endpoint
-#   stores in hand-out order (partial lines).
+#   The wide ensemble, binary64, 2^20 trajectories, default shapes (refill batches of
+#   4): persistent 49,660 KiB read + 104,942 KiB written; staged (first pass, key sort,
+#   resumed pass) 250,358 KiB read + 390,722 KiB written, mostly the parked records and
+#   the resumed pass's endpoint stores in hand-out order (partial lines).
 #   The default line, binary64, 2^22 trajectories with their own t_end, persistent
 #   (256 x 1,280, chunk 64, refill batches of 4), longest span first: 3,851,216 KiB
 #   read + 930,024 KiB written; each trajectory's eight input loads land on eight
 #   separate lines, since the span order hands neighbouring lanes trajectories about a
 #   hundred indices apart.
-MEASURED_TRAFFIC = {("f64", 1 << 20, 2): 134.79e6, ("f64", 1 << 20, 3): 639.44e6,
+MEASURED_TRAFFIC = {("f64", 1 << 20, 2): 158.30e6, ("f64", 1 << 20, 3): 656.42e6,
                     ("f64", 1 << 22, 2, "costmodel"): 4.896e9}
 # Algorithmic bytes per trajectory of an endpoint solve with its own tspan (binary64
 # Lorenz): reads u0, p (48), tspan (16), the hand-out slot (8); writes u_end (24),
