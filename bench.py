@@ -73,6 +73,15 @@ HBM_PEAK_GBS = 8000.0
 #   hundred indices apart.
 MEASURED_TRAFFIC = {("f64", 1 << 20, 2): 158.30e6, ("f64", 1 << 20, 3): 656.42e6,
                     ("f64", 1 << 22, 2, "costmodel"): 4.896e9}
+
+
+def measured_traffic(key, launch):
+    """The counted bytes for `key`, or None: the figures were counted on the default
+    kernel (no register cap, one trajectory per lane), so a launch the autotuner moved to
+    another kernel variant reports none."""
+    if launch.get("waves", 0) != 0 or launch.get("per_lane", 1) != 1:
+        return None
+    return MEASURED_TRAFFIC.get(key)
 # Algorithmic bytes per trajectory of an endpoint solve with its own tspan (binary64
 # Lorenz): reads u0, p (48), tspan (16), the hand-out slot (8); writes u_end (24),
 # t_end (8), retcode, naccept, nreject (12).
@@ -475,7 +484,7 @@ def lorenz_wide(args, ctx, stream, steps, warmup):
         out[name] = {"value": n * steps / wall, "unit": "trajectories/s", "kernel_ms": ms,
                      "ms_per_step": wall * 1e3 / steps,
                      "roofline": _roofline(flops, ms, PEAK_TFLOPS[name], _kernel_name(name, sched),
-                                           MEASURED_TRAFFIC.get((name, n, sched)),
+                                           measured_traffic((name, n, sched), st),
                                            {"attempts_per_launch": st["attempts"]}),
                      "error_vs_avern9_1e-14": ref.error(g),
                      "launch": {k: st[k] for k in ("block", "grid", "chunk", "schedule", "waves",
@@ -694,7 +703,7 @@ def headline(args):
             "error_vs_avern9_1e-14": err_seeded},
     }
 
-    traffic64 = MEASURED_TRAFFIC.get(("f64", n, sched64, "costmodel"))
+    traffic64 = measured_traffic(("f64", n, sched64, "costmodel"), launch64)
     out = {
         "metric": "adaptive trajectories/s at stated tol vs FP64 oracle; % FP32/FP64 VALU peak",
         "value": n * steps / wall64,
